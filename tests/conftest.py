@@ -1,0 +1,16 @@
+"""Shared pytest setup: the ``gpu`` marker and import paths.
+
+GPU tests are marked ``@pytest.mark.gpu`` and run only on a machine with a
+ROCm device; everything else runs on the CPU.
+"""
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (ROOT, os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (ROCm device) to run")
