@@ -1,0 +1,201 @@
+/*
+ * trs_gpu.h -- C ABI of the MI355X (gfx950) Reed-Solomon bulk coding library.
+ *
+ * Drop-in boundary for tape-reed-solomon 0.2.2's bulk coding path.  Each entry
+ * point names the reference method it replaces (file:line in the reference
+ * crate).  Conventions shared by every call:
+ *
+ *  - Shard buffers are DEVICE pointers (HBM).  A shard set is a pointer array
+ *    plus a parallel array of buffer lengths in bytes, so the reference's
+ *    length checks (and their error order) are kept.
+ *  - `stream` is a hipStream_t passed as void*; NULL means the default stream.
+ *    Work is enqueued on that stream and the call returns; results are ready
+ *    once the stream reaches that point.  trs_*_verify waits for its answer.
+ *  - Return value: 0 = OK, 1..14 = the reference's Error variants in
+ *    declaration order (src/errors.rs:6-21), negative = device / runtime
+ *    failures.  trs_status_message() describes every code.
+ *  - Handles are thread-safe: calls may share one codec across threads.  The
+ *    decode-plan cache keeps the reference's 64-entry most-recent-first policy
+ *    (src/reedsolomon.rs:24, :592-613) and lives in device memory.
+ */
+#ifndef TRS_GPU_H
+#define TRS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: src/errors.rs:6-21, in order -------------------------- */
+enum trs_status {
+    TRS_OK = 0,
+    TRS_TOO_FEW_SHARDS = 1,
+    TRS_TOO_MANY_SHARDS = 2,
+    TRS_TOO_FEW_DATA_SHARDS = 3,
+    TRS_TOO_MANY_DATA_SHARDS = 4,
+    TRS_TOO_FEW_PARITY_SHARDS = 5,
+    TRS_TOO_MANY_PARITY_SHARDS = 6,
+    TRS_TOO_FEW_BUFFER_SHARDS = 7,
+    TRS_TOO_MANY_BUFFER_SHARDS = 8,
+    TRS_INCORRECT_SHARD_SIZE = 9,
+    TRS_TOO_FEW_SHARDS_PRESENT = 10,
+    TRS_EMPTY_SHARD = 11,
+    TRS_INVALID_SHARD_FLAGS = 12,
+    TRS_INVALID_INDEX = 13,
+    TRS_SINGULAR_MATRIX = 14,
+    /* device / runtime failures (no reference counterpart) */
+    TRS_ERR_DEVICE = -1,      /* a HIP runtime call failed */
+    TRS_ERR_OUT_OF_MEMORY = -2,
+    TRS_ERR_INVALID_ARGUMENT = -3, /* where the reference panics (length asserts) */
+    TRS_ERR_UNSUPPORTED = -4
+};
+
+/* Human-readable text for a status code (src/errors.rs:24-41 for 1..14). */
+const char *trs_status_message(int status);
+
+/* Library version string. */
+const char *trs_version(void);
+
+/* ---- slice API: src/gf/mod.rs:69-85 (gf::mul_slice / gf::mul_slice_xor) -- */
+/* out[i] = c * in[i] (xor_into == 0) or out[i] ^= c * in[i] (xor_into != 0).
+ * Unequal lengths return TRS_ERR_INVALID_ARGUMENT (the reference panics,
+ * src/gf/scalar.rs:8). */
+int trs_gf8_mul_slice(uint8_t *out, size_t out_len, const uint8_t *in, size_t in_len,
+                      uint8_t c, int xor_into, void *stream);
+
+/* ===================== ReedSolomon over GF(2^8) ========================== */
+typedef struct trs_rs8 trs_rs8;
+typedef struct trs_dec8 trs_dec8;
+
+/* ReedSolomon::new, src/reedsolomon.rs:166-196. */
+int trs_rs8_new(size_t data_shards, size_t parity_shards, trs_rs8 **out);
+void trs_rs8_free(trs_rs8 *rs);
+/* src/reedsolomon.rs:255-265 */
+size_t trs_rs8_data_shard_count(const trs_rs8 *rs);
+size_t trs_rs8_parity_shard_count(const trs_rs8 *rs);
+size_t trs_rs8_total_shard_count(const trs_rs8 *rs);
+/* ReedSolomon::encode_route, src/reedsolomon.rs:239-253: the device route
+ * this codec's encode takes at the given shard length. */
+const char *trs_rs8_encode_route(const trs_rs8 *rs, size_t shard_len);
+/* Host copy of the n x k generator matrix (row-major, n*k bytes). */
+int trs_rs8_generator(const trs_rs8 *rs, uint8_t *out, size_t out_len);
+
+/* ReedSolomon::encode, src/reedsolomon.rs:271-279: shards = k data then m
+ * parity buffers; parity overwritten. */
+int trs_rs8_encode(const trs_rs8 *rs, uint8_t *const *shards, const size_t *lens,
+                   size_t n_shards, void *stream);
+/* ReedSolomon::encode_sep, src/reedsolomon.rs:284-309. */
+int trs_rs8_encode_sep(const trs_rs8 *rs, const uint8_t *const *data, const size_t *data_lens,
+                       size_t n_data, uint8_t *const *parity, const size_t *parity_lens,
+                       size_t n_parity, void *stream);
+/* ReedSolomon::encode_rows_into, src/reedsolomon.rs:667-682: data holds k
+ * rows of row_len bytes back to back, parity m rows. */
+int trs_rs8_encode_rows_into(const trs_rs8 *rs, const uint8_t *data, size_t data_len,
+                             size_t row_len, uint8_t *parity, size_t parity_len, void *stream);
+/* ReedSolomon::verify, src/reedsolomon.rs:430-463: *ok = 1 when every parity
+ * shard matches the data, else 0.  Synchronises with `stream`. */
+int trs_rs8_verify(const trs_rs8 *rs, const uint8_t *const *shards, const size_t *lens,
+                   size_t n_shards, int *ok, void *stream);
+/* ReedSolomon::reconstruct / reconstruct_data, src/reedsolomon.rs:470-477,
+ * :615-641, :829-891, for (buffer, present) shards: present[i] != 0 marks a
+ * valid shard; missing shards are overwritten (missing parity left untouched
+ * when data_only != 0). lens[i] is every buffer's length. */
+int trs_rs8_reconstruct(const trs_rs8 *rs, uint8_t *const *shards, const size_t *lens,
+                        const uint8_t *present, size_t n_shards, int data_only, void *stream);
+/* ReedSolomon::reconstruct_rows, src/reedsolomon.rs:514-535. */
+int trs_rs8_reconstruct_rows(const trs_rs8 *rs, uint8_t *rows, size_t rows_len, size_t row_len,
+                             const uint8_t *present, size_t n_present, void *stream);
+/* ReedSolomon::prepare_decode, src/reedsolomon.rs:484-507. */
+int trs_rs8_prepare_decode(const trs_rs8 *rs, const uint8_t *present, size_t n_present,
+                           trs_dec8 **out);
+/* PreparedDecoder::{reconstruct, reconstruct_data}, src/reedsolomon.rs:762-797. */
+int trs_dec8_reconstruct(const trs_dec8 *dec, uint8_t *const *shards, const size_t *lens,
+                         const uint8_t *present, size_t n_shards, int data_only, void *stream);
+void trs_dec8_free(trs_dec8 *dec);
+
+/* ================= ReedSolomon16 over GF((2^8)^2) ======================== */
+typedef struct trs_rs16 trs_rs16;
+typedef struct trs_dec16 trs_dec16;
+
+/* ReedSolomon16::new, src/reedsolomon16.rs:281-319 (k + m <= 65536). */
+int trs_rs16_new(size_t data_shards, size_t parity_shards, trs_rs16 **out);
+void trs_rs16_free(trs_rs16 *rs);
+size_t trs_rs16_data_shard_count(const trs_rs16 *rs);
+size_t trs_rs16_parity_shard_count(const trs_rs16 *rs);
+size_t trs_rs16_total_shard_count(const trs_rs16 *rs);
+/* ReedSolomon16::encode_route, src/reedsolomon16.rs:338-346. */
+const char *trs_rs16_encode_route(const trs_rs16 *rs);
+/* Host copy of the n x k tower generator (row-major, n*k uint16). */
+int trs_rs16_generator(const trs_rs16 *rs, uint16_t *out, size_t out_len);
+/* ReedSolomon16::encode, src/reedsolomon16.rs:412-418 (split-plane shards of
+ * equal even length: N low bytes then N high bytes). */
+int trs_rs16_encode(const trs_rs16 *rs, uint8_t *const *shards, const size_t *lens,
+                    size_t n_shards, void *stream);
+/* ReedSolomon16::encode_sep, src/reedsolomon16.rs:422-447. */
+int trs_rs16_encode_sep(const trs_rs16 *rs, const uint8_t *const *data, const size_t *data_lens,
+                        size_t n_data, uint8_t *const *parity, const size_t *parity_lens,
+                        size_t n_parity, void *stream);
+/* ReedSolomon16::encode_sep_matrix, src/reedsolomon16.rs:324-336: the matrix
+ * route regardless of routing. */
+int trs_rs16_encode_sep_matrix(const trs_rs16 *rs, const uint8_t *const *data,
+                               const size_t *data_lens, size_t n_data, uint8_t *const *parity,
+                               const size_t *parity_lens, size_t n_parity, void *stream);
+/* ReedSolomon16::verify, src/reedsolomon16.rs:453-505. */
+int trs_rs16_verify(const trs_rs16 *rs, const uint8_t *const *shards, const size_t *lens,
+                    size_t n_shards, int *ok, void *stream);
+/* ReedSolomon16::reconstruct / reconstruct_data, src/reedsolomon16.rs:508-515. */
+int trs_rs16_reconstruct(const trs_rs16 *rs, uint8_t *const *shards, const size_t *lens,
+                         const uint8_t *present, size_t n_shards, int data_only, void *stream);
+/* ReedSolomon16::reconstruct_rows, src/reedsolomon16.rs:546-569. */
+int trs_rs16_reconstruct_rows(const trs_rs16 *rs, uint8_t *rows, size_t rows_len,
+                              size_t row_len, const uint8_t *present, size_t n_present,
+                              void *stream);
+/* ReedSolomon16::prepare_decode, src/reedsolomon16.rs:518-543. */
+int trs_rs16_prepare_decode(const trs_rs16 *rs, const uint8_t *present, size_t n_present,
+                            trs_dec16 **out);
+/* PreparedDecoder16::{reconstruct, reconstruct_data}, src/reedsolomon16.rs:722-753. */
+int trs_dec16_reconstruct(const trs_dec16 *dec, uint8_t *const *shards, const size_t *lens,
+                          const uint8_t *present, size_t n_shards, int data_only, void *stream);
+void trs_dec16_free(trs_dec16 *dec);
+
+/* ================= batched GF(2^8) coding (no reference counterpart) ====== */
+/* The reference is one-call-one-stripe and leaves batching to its callers
+ * (BENCH-RESULTS.md:98-100).  A batch collects many encode / reconstruct
+ * requests on GF(2^8) codecs, of mixed shapes, lengths and erasure patterns,
+ * and runs them all in one persistent kernel launch over a device work list.
+ * Each request is validated exactly as the single-stripe call would be, and
+ * each result is byte-identical to that call's. */
+typedef struct trs_batch8 trs_batch8;
+
+int trs_batch8_new(trs_batch8 **out);
+void trs_batch8_free(trs_batch8 *b);
+/* Drop every queued request (keeps device buffers for reuse). */
+void trs_batch8_clear(trs_batch8 *b);
+/* Queue rs.encode(shards) (same checks as trs_rs8_encode). */
+int trs_batch8_add_encode(trs_batch8 *b, const trs_rs8 *rs, uint8_t *const *shards,
+                          const size_t *lens, size_t n_shards);
+/* Queue rs.reconstruct(shards) / reconstruct_data (same checks as
+ * trs_rs8_reconstruct; the decode plan comes from rs's device plan cache). */
+int trs_batch8_add_reconstruct(trs_batch8 *b, const trs_rs8 *rs, uint8_t *const *shards,
+                               const size_t *lens, const uint8_t *present, size_t n_shards,
+                               int data_only);
+/* Upload the work list for the queued requests (once per batch shape). */
+int trs_batch8_prepare(trs_batch8 *b);
+/* Run every queued request in one launch on `stream`. */
+int trs_batch8_run(trs_batch8 *b, void *stream);
+/* Algorithmic bytes of the queued work: k reads + outputs written, per stripe. */
+uint64_t trs_batch8_bytes(const trs_batch8 *b);
+size_t trs_batch8_size(const trs_batch8 *b);
+
+/* ================= utilities (bench / test helpers) ======================= */
+/* Fill a device buffer with the seeded splitmix64 byte stream: byte j of
+ * 8-byte word w is byte j (little-endian) of mix64(seed + (w+1)*0x9E3779B97F4A7C15). */
+int trs_fill_random(uint8_t *dst, size_t len, uint64_t seed, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRS_GPU_H */

@@ -1,0 +1,432 @@
+"""MI355X (gfx950) backend for tape-reed-solomon's bulk coding path.
+
+Python mirror of the reference crate's codec API over the C ABI of
+``libtrs_gpu.so`` (include/trs_gpu.h).  Shards are ``torch.uint8`` CUDA
+tensors (device memory); every byte of coding work runs in the library's HIP
+kernels.  Work is enqueued on torch's current stream; ``verify`` returns once
+its answer is known.
+
+Reference surface kept (tape-reed-solomon 0.2.2):
+
+* ``ReedSolomon`` -- src/reedsolomon.rs: ``new`` (the constructor),
+  ``encode``, ``encode_sep``, ``encode_rows``, ``encode_rows_into``,
+  ``verify``, ``reconstruct``, ``reconstruct_data``, ``prepare_decode``,
+  ``reconstruct_rows``, ``data/parity/total_shard_count``, ``encode_route``.
+* ``ReedSolomon16`` -- src/reedsolomon16.rs: the same, minus
+  ``encode_rows(_into)`` (absent in the reference), plus ``encode_sep_matrix``.
+* ``PreparedDecoder`` / ``PreparedDecoder16`` -- ``reconstruct``,
+  ``reconstruct_data``.
+* ``Error`` -- src/errors.rs, same variant names and messages.
+* ``mul_slice`` / ``mul_slice_xor`` -- src/gf/mod.rs:69-85.
+
+Reconstruct accepts the reference's two shard containers: ``(tensor, bool)``
+pairs (the bool is the "present" flag, src/reedsolomon.rs:44-77) and
+``Optional`` entries (``None`` is missing; the codec allocates a zeroed shard
+in its place, src/reedsolomon.rs:79-102).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import lib
+
+__all__ = [
+    "Error",
+    "ReedSolomon",
+    "ReedSolomon16",
+    "PreparedDecoder",
+    "PreparedDecoder16",
+    "Batch",
+    "mul_slice",
+    "mul_slice_xor",
+    "fill_random",
+    "version",
+]
+
+ERROR_NAMES = [
+    None,
+    "TooFewShards",
+    "TooManyShards",
+    "TooFewDataShards",
+    "TooManyDataShards",
+    "TooFewParityShards",
+    "TooManyParityShards",
+    "TooFewBufferShards",
+    "TooManyBufferShards",
+    "IncorrectShardSize",
+    "TooFewShardsPresent",
+    "EmptyShard",
+    "InvalidShardFlags",
+    "InvalidIndex",
+    "SingularMatrix",
+]
+
+
+class Error(Exception):
+    """A codec error; ``.name`` is the reference's Error variant (src/errors.rs:6-21)."""
+
+    def __init__(self, code: int):
+        self.code = code
+        self.name = ERROR_NAMES[code] if 0 < code < len(ERROR_NAMES) else f"Device({code})"
+        super().__init__(f"{self.name}: {lib.trs_status_message(code).decode()}")
+
+    def __eq__(self, other):
+        if isinstance(other, Error):
+            return self.code == other.code
+        if isinstance(other, str):
+            return self.name == other
+        return NotImplemented
+
+    def __hash__(self):
+        return hash(self.code)
+
+
+def _check(code: int) -> None:
+    if code != 0:
+        raise Error(code)
+
+
+def version() -> str:
+    return lib.trs_version().decode()
+
+
+def _stream():
+    import torch
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev_ptr(t) -> int:
+    import torch
+
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("shards must be torch tensors")
+    if t.dtype != torch.uint8 or not t.is_cuda:
+        raise TypeError("shards must be torch.uint8 tensors on a CUDA (ROCm) device")
+    if not t.is_contiguous():
+        raise ValueError("shards must be contiguous")
+    return t.data_ptr()
+
+
+def _ptrs(tensors: Sequence) -> tuple:
+    arr = (ctypes.c_void_p * max(1, len(tensors)))()
+    lens = (ctypes.c_size_t * max(1, len(tensors)))()
+    for i, t in enumerate(tensors):
+        arr[i] = _dev_ptr(t)
+        lens[i] = t.numel()
+    return arr, lens
+
+
+def _flags(flags: Sequence[bool]):
+    a = (ctypes.c_uint8 * max(1, len(flags)))()
+    for i, f in enumerate(flags):
+        a[i] = 1 if f else 0
+    return a
+
+
+def _normalise_reconstruct(shards: list, data_only: bool, k: int):
+    """Turns the reference's containers into (tensors, present flags).
+
+    ``(tensor, bool)`` entries pass through.  ``None`` entries are missing
+    Option shards: they get a zeroed tensor of the shared length, written back
+    into the list (get_or_initialize, src/reedsolomon.rs:88-101), except parity
+    entries of a data-only reconstruct, which stay ``None``.
+    """
+    import torch
+
+    tensors, present, fill = [], [], []
+    length, device = None, None
+    for i, s in enumerate(shards):
+        if isinstance(s, tuple):
+            t, p = s
+            tensors.append(t)
+            present.append(bool(p))
+            if p and length is None:
+                length, device = t.numel(), t.device
+        elif s is None:
+            tensors.append(None)
+            present.append(False)
+            fill.append(i)
+        else:
+            tensors.append(s)
+            present.append(True)
+            if length is None:
+                length, device = s.numel(), s.device
+    if fill:
+        if length is None:
+            return None, present, fill
+        dummy = None
+        for i in fill:
+            if data_only and i >= k:
+                # never read or written: a data-only reconstruct skips missing parity
+                if dummy is None:
+                    dummy = torch.zeros(length, dtype=torch.uint8, device=device)
+                tensors[i] = dummy
+            else:
+                tensors[i] = torch.zeros(length, dtype=torch.uint8, device=device)
+    return tensors, present, fill
+
+
+class _Codec:
+    _fam = ""
+
+    def __init__(self, data_shards: int, parity_shards: int):
+        h = ctypes.c_void_p()
+        _check(getattr(lib, f"trs_{self._fam}_new")(data_shards, parity_shards, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            getattr(lib, f"trs_{self._fam}_free")(h)
+            self._h = None
+
+    def data_shard_count(self) -> int:
+        return getattr(lib, f"trs_{self._fam}_data_shard_count")(self._h)
+
+    def parity_shard_count(self) -> int:
+        return getattr(lib, f"trs_{self._fam}_parity_shard_count")(self._h)
+
+    def total_shard_count(self) -> int:
+        return getattr(lib, f"trs_{self._fam}_total_shard_count")(self._h)
+
+    def __eq__(self, other):  # src/reedsolomon.rs:741-746
+        return (type(self) is type(other)
+                and self.data_shard_count() == other.data_shard_count()
+                and self.parity_shard_count() == other.parity_shard_count())
+
+    def encode(self, shards: Sequence) -> None:
+        """Parity shards (the last m of ``shards``) are overwritten."""
+        p, l = _ptrs(shards)
+        _check(getattr(lib, f"trs_{self._fam}_encode")(self._h, p, l, len(shards), _stream()))
+
+    def encode_sep(self, data: Sequence, parity: Sequence) -> None:
+        dp, dl = _ptrs(data)
+        pp, pl = _ptrs(parity)
+        _check(getattr(lib, f"trs_{self._fam}_encode_sep")(self._h, dp, dl, len(data), pp, pl,
+                                                            len(parity), _stream()))
+
+    def verify(self, shards: Sequence) -> bool:
+        p, l = _ptrs(shards)
+        ok = ctypes.c_int(0)
+        _check(getattr(lib, f"trs_{self._fam}_verify")(self._h, p, l, len(shards),
+                                                        ctypes.byref(ok), _stream()))
+        return bool(ok.value)
+
+    def _reconstruct(self, shards: list, data_only: bool) -> None:
+        tensors, present, filled = _normalise_reconstruct(shards, data_only, self.data_shard_count())
+        n = len(shards)
+        if tensors is None:  # nothing present at all
+            total = self.total_shard_count()
+            if n != total:
+                raise Error(1 if n < total else 2)
+            raise Error(10)
+        p, l = _ptrs(tensors)
+        _check(getattr(lib, f"trs_{self._fam}_reconstruct")(
+            self._h, p, l, _flags(present), n, int(data_only), _stream()))
+        k = self.data_shard_count()
+        for i in filled:
+            if not (data_only and i >= k):
+                shards[i] = tensors[i]
+
+    def reconstruct(self, shards: list) -> None:
+        self._reconstruct(shards, False)
+
+    def reconstruct_data(self, shards: list) -> None:
+        self._reconstruct(shards, True)
+
+    def reconstruct_rows(self, rows, row_len: int, present: Sequence[bool]) -> None:
+        _check(getattr(lib, f"trs_{self._fam}_reconstruct_rows")(
+            self._h, ctypes.c_void_p(_dev_ptr(rows)), rows.numel(), row_len, _flags(present),
+            len(present), _stream()))
+
+    def prepare_decode(self, present: Sequence[bool]):
+        h = ctypes.c_void_p()
+        _check(getattr(lib, f"trs_{self._fam}_prepare_decode")(
+            self._h, _flags(present), len(present), ctypes.byref(h)))
+        return self._decoder_type(h, self)
+
+
+class _Decoder:
+    _fam = ""
+
+    def __init__(self, handle, codec):
+        self._h = handle
+        self._codec = codec  # the plan's device memory outlives no codec
+        self._k = codec.data_shard_count()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            getattr(lib, f"trs_{self._fam}_free")(h)
+            self._h = None
+
+    def _reconstruct(self, shards: list, data_only: bool) -> None:
+        tensors, present, filled = _normalise_reconstruct(shards, data_only, self._k)
+        if tensors is None:
+            total = self._codec.total_shard_count()
+            if len(shards) != total:
+                raise Error(1 if len(shards) < total else 2)
+            raise Error(12)
+        p, l = _ptrs(tensors)
+        _check(getattr(lib, f"trs_{self._fam}_reconstruct")(
+            self._h, p, l, _flags(present), len(shards), int(data_only), _stream()))
+        for i in filled:
+            if not (data_only and i >= self._k):
+                shards[i] = tensors[i]
+
+    def reconstruct(self, shards: list) -> None:
+        self._reconstruct(shards, False)
+
+    def reconstruct_data(self, shards: list) -> None:
+        self._reconstruct(shards, True)
+
+
+class PreparedDecoder(_Decoder):
+    """src/reedsolomon.rs:748-797."""
+
+    _fam = "dec8"
+
+
+class PreparedDecoder16(_Decoder):
+    """src/reedsolomon16.rs:709-753."""
+
+    _fam = "dec16"
+
+
+class ReedSolomon(_Codec):
+    """GF(2^8) systematic Reed-Solomon, poly 0x11d (src/reedsolomon.rs)."""
+
+    _fam = "rs8"
+    _decoder_type = PreparedDecoder
+
+    def encode_route(self, shard_len: int) -> str:
+        return lib.trs_rs8_encode_route(self._h, shard_len).decode()
+
+    def encode_fused(self, shards: Sequence) -> None:
+        """Alias of encode (src/reedsolomon.rs:354-359)."""
+        self.encode(shards)
+
+    def encode_rows_into(self, data, row_len: int, parity) -> None:
+        _check(lib.trs_rs8_encode_rows_into(self._h, ctypes.c_void_p(_dev_ptr(data)), data.numel(),
+                                            row_len, ctypes.c_void_p(_dev_ptr(parity)),
+                                            parity.numel(), _stream()))
+
+    def encode_rows(self, data, row_len: int):
+        """Panics (AssertionError) on a bad length, like src/reedsolomon.rs:652-657."""
+        import torch
+
+        assert data.numel() == self.data_shard_count() * row_len, \
+            "data must be data_shard_count * row_len bytes"
+        parity = torch.zeros(self.parity_shard_count() * row_len, dtype=torch.uint8,
+                             device=data.device)
+        self.encode_rows_into(data, row_len, parity)
+        return parity
+
+    def generator(self) -> np.ndarray:
+        n, k = self.total_shard_count(), self.data_shard_count()
+        out = np.zeros(n * k, np.uint8)
+        _check(lib.trs_rs8_generator(self._h, out.ctypes.data_as(_lib.c_u8p), out.size))
+        return out.reshape(n, k)
+
+
+class ReedSolomon16(_Codec):
+    """GF((2^8)^2) tower Reed-Solomon over split-plane shards (src/reedsolomon16.rs)."""
+
+    _fam = "rs16"
+    _decoder_type = PreparedDecoder16
+
+    def encode_route(self) -> str:
+        return lib.trs_rs16_encode_route(self._h).decode()
+
+    def encode_sep_matrix(self, data: Sequence, parity: Sequence) -> None:
+        dp, dl = _ptrs(data)
+        pp, pl = _ptrs(parity)
+        _check(lib.trs_rs16_encode_sep_matrix(self._h, dp, dl, len(data), pp, pl, len(parity),
+                                              _stream()))
+
+    def generator(self) -> np.ndarray:
+        n, k = self.total_shard_count(), self.data_shard_count()
+        out = np.zeros(n * k, np.uint16)
+        _check(lib.trs_rs16_generator(self._h, out.ctypes.data_as(_lib.c_u16p), out.size))
+        return out.reshape(n, k)
+
+
+class Batch:
+    """Many GF(2^8) encode / reconstruct requests run in one persistent launch.
+
+    No reference counterpart: the reference codes one stripe per call and leaves
+    batching to callers (BENCH-RESULTS.md:98-100).  Each queued request is
+    validated like the single call and produces byte-identical output.
+    """
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(lib.trs_batch8_new(ctypes.byref(h)))
+        self._h = h
+        self._keep = []
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.trs_batch8_free(h)
+            self._h = None
+
+    def clear(self) -> None:
+        lib.trs_batch8_clear(self._h)
+        self._keep = []
+
+    def add_encode(self, rs: ReedSolomon, shards: Sequence) -> None:
+        p, l = _ptrs(shards)
+        _check(lib.trs_batch8_add_encode(self._h, rs._h, p, l, len(shards)))
+        self._keep.append(rs)
+
+    def add_reconstruct(self, rs: ReedSolomon, shards: Sequence, present: Sequence[bool],
+                        data_only: bool = False) -> None:
+        p, l = _ptrs(shards)
+        _check(lib.trs_batch8_add_reconstruct(self._h, rs._h, p, l, _flags(present), len(shards),
+                                              int(data_only)))
+        self._keep.append(rs)
+
+    def prepare(self) -> None:
+        _check(lib.trs_batch8_prepare(self._h))
+
+    def run(self, stream=None) -> None:
+        s = _stream() if stream is None else ctypes.c_void_p(stream)
+        _check(lib.trs_batch8_run(self._h, s))
+
+    def bytes(self) -> int:
+        return lib.trs_batch8_bytes(self._h)
+
+    def __len__(self) -> int:
+        return lib.trs_batch8_size(self._h)
+
+
+def _slice(out, inp, c: int, xor: bool) -> None:
+    _check(lib.trs_gf8_mul_slice(ctypes.c_void_p(_dev_ptr(out)), out.numel(),
+                                 ctypes.c_void_p(_dev_ptr(inp)), inp.numel(), c, int(xor),
+                                 _stream()))
+
+
+def mul_slice(out, inp, c: int) -> None:
+    """out = c * inp (src/gf/mod.rs:71-76).  Unequal lengths raise like the reference's panic."""
+    if out.numel() != inp.numel():
+        raise AssertionError("assertion `left == right` failed: slice lengths differ")
+    _slice(out, inp, c, False)
+
+
+def mul_slice_xor(out, inp, c: int) -> None:
+    """out ^= c * inp (src/gf/mod.rs:80-85)."""
+    if out.numel() != inp.numel():
+        raise AssertionError("assertion `left == right` failed: slice lengths differ")
+    _slice(out, inp, c, True)
+
+
+def fill_random(t, seed: int) -> None:
+    """Fill a device tensor with the seeded splitmix64 byte stream."""
+    _check(lib.trs_fill_random(ctypes.c_void_p(_dev_ptr(t)), t.numel(),
+                               seed & 0xFFFFFFFFFFFFFFFF, _stream()))

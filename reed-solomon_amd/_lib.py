@@ -1,0 +1,85 @@
+"""ctypes binding of libtrs_gpu.so (the C ABI declared in include/trs_gpu.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C reed-solomon_amd/csrc``).  There is no fallback: if the shared
+library is missing, importing the package fails with an error that says so.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtrs_gpu.so")
+HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "trs_gpu.h")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+        "or `make -C reed-solomon_amd/csrc`"
+    )
+
+lib = ctypes.CDLL(LIB_PATH)
+
+c_size = ctypes.c_size_t
+c_int = ctypes.c_int
+c_u8p = ctypes.POINTER(ctypes.c_uint8)
+c_u16p = ctypes.POINTER(ctypes.c_uint16)
+c_vp = ctypes.c_void_p
+c_sizep = ctypes.POINTER(ctypes.c_size_t)
+c_ptrs = ctypes.POINTER(ctypes.c_void_p)
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_sig("trs_status_message", ctypes.c_char_p, c_int)
+_sig("trs_version", ctypes.c_char_p)
+_sig("trs_gf8_mul_slice", c_int, c_vp, c_size, c_vp, c_size, ctypes.c_uint8, c_int, c_vp)
+_sig("trs_fill_random", c_int, c_vp, c_size, ctypes.c_uint64, c_vp)
+
+for fam, elem in (("rs8", c_u8p), ("rs16", c_u16p)):
+    _sig(f"trs_{fam}_new", c_int, c_size, c_size, ctypes.POINTER(c_vp))
+    _sig(f"trs_{fam}_free", None, c_vp)
+    for what in ("data", "parity", "total"):
+        _sig(f"trs_{fam}_{what}_shard_count", c_size, c_vp)
+    _sig(f"trs_{fam}_generator", c_int, c_vp, elem, c_size)
+    _sig(f"trs_{fam}_encode", c_int, c_vp, c_ptrs, c_sizep, c_size, c_vp)
+    _sig(f"trs_{fam}_encode_sep", c_int, c_vp, c_ptrs, c_sizep, c_size, c_ptrs, c_sizep, c_size,
+         c_vp)
+    _sig(f"trs_{fam}_verify", c_int, c_vp, c_ptrs, c_sizep, c_size, ctypes.POINTER(c_int), c_vp)
+    _sig(f"trs_{fam}_reconstruct", c_int, c_vp, c_ptrs, c_sizep, c_u8p, c_size, c_int, c_vp)
+    _sig(f"trs_{fam}_reconstruct_rows", c_int, c_vp, c_vp, c_size, c_size, c_u8p, c_size, c_vp)
+    _sig(f"trs_{fam}_prepare_decode", c_int, c_vp, c_u8p, c_size, ctypes.POINTER(c_vp))
+
+_sig("trs_rs8_encode_route", ctypes.c_char_p, c_vp, c_size)
+_sig("trs_rs16_encode_route", ctypes.c_char_p, c_vp)
+_sig("trs_rs8_encode_rows_into", c_int, c_vp, c_vp, c_size, c_size, c_vp, c_size, c_vp)
+_sig("trs_rs16_encode_sep_matrix", c_int, c_vp, c_ptrs, c_sizep, c_size, c_ptrs, c_sizep, c_size,
+     c_vp)
+for fam in ("dec8", "dec16"):
+    _sig(f"trs_{fam}_reconstruct", c_int, c_vp, c_ptrs, c_sizep, c_u8p, c_size, c_int, c_vp)
+    _sig(f"trs_{fam}_free", None, c_vp)
+
+_sig("trs_batch8_new", c_int, ctypes.POINTER(c_vp))
+_sig("trs_batch8_free", None, c_vp)
+_sig("trs_batch8_clear", None, c_vp)
+_sig("trs_batch8_add_encode", c_int, c_vp, c_vp, c_ptrs, c_sizep, c_size)
+_sig("trs_batch8_add_reconstruct", c_int, c_vp, c_vp, c_ptrs, c_sizep, c_u8p, c_size, c_int)
+_sig("trs_batch8_prepare", c_int, c_vp)
+_sig("trs_batch8_run", c_int, c_vp, c_vp)
+_sig("trs_batch8_bytes", ctypes.c_uint64, c_vp)
+_sig("trs_batch8_size", c_size, c_vp)
+
+
+def declared_symbols() -> list:
+    """Every function name include/trs_gpu.h declares."""
+    import re
+
+    with open(HEADER_PATH, encoding="utf-8") as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(trs_[a-z0-9_]+)\s*\(", text)))

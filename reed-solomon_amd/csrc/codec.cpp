@@ -1,0 +1,1064 @@
+// Host side of the MI355X Reed-Solomon library: the C ABI of trs_gpu.h.
+//
+// This layer keeps the reference codec's semantics -- validation order,
+// error codes, the systematic generator, first-k-present decode plans and
+// the 64-entry most-recent-first plan cache -- and hands every byte of shard
+// traffic to the HIP kernels in kernels.hip.  Nothing here touches shard
+// bytes on the host.
+//
+// Reconstruct runs as ONE fused apply: the plan's matrix maps the k
+// survivors straight to every missing shard.  Rows for missing data are rows
+// of inv(G_valid) (src/reedsolomon.rs:539-571); rows for missing parity are
+// G_parity[p] * inv(G_valid), i.e. the parity row applied to the data that
+// inv(G_valid) rebuilds.  The reference applies those two steps in sequence
+// (src/reedsolomon.rs:866-889); over a field the composition is exact, so the
+// fused result is byte-identical while the data re-read of the second pass
+// disappears.
+#include <hip/hip_runtime_api.h>
+
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
+#include <mutex>
+#include <shared_mutex>
+#include <utility>
+#include <vector>
+
+#include "../../include/trs_gpu.h"
+#include "field.hpp"
+#include "kernels.hpp"
+
+namespace trs {
+
+const Gf8 &gf8() {
+  static const Gf8 t;
+  return t;
+}
+
+Gf16::Gf16() : log(65536, 0), exp(2 * 65535, 0) {
+  const Gf8 &f = gf8();
+  auto pw = [&](uint16_t base, unsigned e) {
+    uint16_t r = 1;
+    while (e) {
+      if (e & 1) r = tower_mul_def(f, r, base);
+      base = tower_mul_def(f, base, base);
+      e >>= 1;
+    }
+    return r;
+  };
+  // Smallest element of full multiplicative order 65535 = 3*5*17*257.
+  uint16_t g = 0;
+  for (unsigned c = 2; c < 65536 && !g; ++c) {
+    const uint16_t x = uint16_t(c);
+    if (pw(x, 65535 / 3) != 1 && pw(x, 65535 / 5) != 1 && pw(x, 65535 / 17) != 1 &&
+        pw(x, 65535 / 257) != 1)
+      g = x;
+  }
+  uint16_t x = 1;
+  for (unsigned i = 0; i < 65535; ++i) {
+    exp[i] = x;
+    exp[i + 65535] = x;
+    log[x] = uint16_t(i);
+    x = tower_mul_def(f, x, g);
+  }
+}
+
+const Gf16 &gf16() {
+  static const Gf16 t;
+  return t;
+}
+
+namespace {
+
+int hip_status(hipError_t e) {
+  if (e == hipSuccess) return TRS_OK;
+  return e == hipErrorOutOfMemory ? TRS_ERR_OUT_OF_MEMORY : TRS_ERR_DEVICE;
+}
+
+#define TRS_TRY(expr)                      \
+  do {                                     \
+    const hipError_t e_ = (expr);          \
+    if (e_ != hipSuccess) return hip_status(e_); \
+  } while (0)
+
+// Owned device allocation.
+struct DevBuf {
+  void *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t bytes) {
+    reset();
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipSuccess) n = bytes;
+    else p = nullptr;
+    return e;
+  }
+  hipError_t ensure(size_t bytes) { return bytes <= n ? hipSuccess : alloc(bytes); }
+};
+
+size_t pad_rows(size_t nout) {
+  const size_t s = size_t(kOutStride);
+  return ((nout ? nout : 1) + s - 1) / s * s;
+}
+
+// Coefficients in the kernels' layout: for each plane, byte (i*ostride + j)
+// is output row j's coefficient for input i; rows past nout are zero.
+struct DevCoef {
+  DevBuf buf;
+  uint32_t k = 0, nout = 0, ostride = 0;
+  const uint32_t *ptr() const { return static_cast<const uint32_t *>(buf.p); }
+};
+
+// rows: nout x k GF(2^8) coefficients, row-major.
+int upload_coef8(DevCoef &dc, const uint8_t *rows, size_t nout, size_t k) {
+  const size_t os = pad_rows(nout);
+  std::vector<uint8_t> t(k * os, 0);
+  for (size_t j = 0; j < nout; ++j)
+    for (size_t i = 0; i < k; ++i) t[i * os + j] = rows[j * k + i];
+  TRS_TRY(dc.buf.alloc(t.size()));
+  TRS_TRY(hipMemcpy(dc.buf.p, t.data(), t.size(), hipMemcpyHostToDevice));
+  dc.k = uint32_t(k);
+  dc.nout = uint32_t(nout);
+  dc.ostride = uint32_t(os);
+  return TRS_OK;
+}
+
+// rows: nout x k tower coefficients; three planes of Karatsuba products
+// [c_lo, c_hi^c_lo, c_hi*L] (src/fft16.rs:176-180).
+int upload_coef16(DevCoef &dc, const uint16_t *rows, size_t nout, size_t k) {
+  const size_t os = pad_rows(nout);
+  const size_t plane = k * os;
+  std::vector<uint8_t> t(3 * plane, 0);
+  const Gf8 &f = gf8();
+  for (size_t j = 0; j < nout; ++j)
+    for (size_t i = 0; i < k; ++i) {
+      const uint16_t c = rows[j * k + i];
+      const uint8_t hi = c >> 8, lo = c & 0xff;
+      t[i * os + j] = lo;
+      t[plane + i * os + j] = hi ^ lo;
+      t[2 * plane + i * os + j] = f.mul[hi][kTowerL];
+    }
+  TRS_TRY(dc.buf.alloc(t.size()));
+  TRS_TRY(hipMemcpy(dc.buf.p, t.data(), t.size(), hipMemcpyHostToDevice));
+  dc.k = uint32_t(k);
+  dc.nout = uint32_t(nout);
+  dc.ostride = uint32_t(os);
+  return TRS_OK;
+}
+
+using Mask = std::vector<uint64_t>;
+
+Mask make_mask(const uint8_t *present, size_t n) {
+  Mask m((n + 63) / 64, 0);
+  for (size_t i = 0; i < n; ++i)
+    if (present[i]) m[i / 64] |= uint64_t(1) << (i % 64);
+  return m;
+}
+
+bool mask_get(const Mask &m, size_t i) { return (m[i / 64] >> (i % 64)) & 1; }
+
+// Everything one erasure pattern needs (src/reedsolomon.rs:117-130).
+struct Plan {
+  Mask presence;
+  std::vector<uint32_t> valid;           // first k present indices, ascending
+  std::vector<uint32_t> missing_data;    // ascending
+  std::vector<uint32_t> missing_parity;  // ascending
+  DevCoef coef;                          // [missing data rows; missing parity rows] x k
+  std::vector<uint8_t> rows8;            // host copy (GF(2^8) plans)
+};
+
+constexpr size_t kPlanCacheLimit = 64;  // src/reedsolomon.rs:24
+
+struct PlanCache {
+  mutable std::shared_mutex mu;
+  mutable std::vector<std::pair<Mask, std::shared_ptr<Plan>>> entries;
+
+  std::shared_ptr<Plan> find(const Mask &m) const {
+    std::shared_lock<std::shared_mutex> lk(mu);
+    for (auto &e : entries)
+      if (e.first == m) return e.second;
+    return nullptr;
+  }
+  // Insert at the front unless another thread already did (src/reedsolomon.rs:606-612).
+  std::shared_ptr<Plan> insert(const Mask &m, std::shared_ptr<Plan> p) const {
+    std::unique_lock<std::shared_mutex> lk(mu);
+    for (auto &e : entries)
+      if (e.first == m) return e.second;
+    entries.insert(entries.begin(), {m, p});
+    if (entries.size() > kPlanCacheLimit) entries.resize(kPlanCacheLimit);
+    return p;
+  }
+};
+
+// Splits indices by presence the way build_plan does (src/reedsolomon.rs:550-566).
+void split_indices(const Mask &m, size_t k, size_t n, Plan &p) {
+  for (size_t i = 0; i < n; ++i) {
+    if (mask_get(m, i)) {
+      if (p.valid.size() < k) p.valid.push_back(uint32_t(i));
+    } else if (i < k) {
+      p.missing_data.push_back(uint32_t(i));
+    } else {
+      p.missing_parity.push_back(uint32_t(i));
+    }
+  }
+}
+
+template <class F>
+bool fused_rows(const Matrix<F> &G, size_t k, const Plan &p, std::vector<typename F::T> &rows) {
+  Matrix<F> sub(k, k), inv;
+  for (size_t r = 0; r < k; ++r) std::copy(G.row(p.valid[r]), G.row(p.valid[r]) + k, sub.row(r));
+  if (!invert(sub, inv)) return false;
+  const size_t nd = p.missing_data.size(), np = p.missing_parity.size();
+  rows.assign((nd + np) * k, 0);
+  for (size_t d = 0; d < nd; ++d)
+    std::copy(inv.row(p.missing_data[d]), inv.row(p.missing_data[d]) + k, rows.data() + d * k);
+  for (size_t q = 0; q < np; ++q) {
+    const auto *g = G.row(p.missing_parity[q]);
+    auto *o = rows.data() + (nd + q) * k;
+    for (size_t i = 0; i < k; ++i) {
+      if (!g[i]) continue;
+      const auto *ir = inv.row(i);
+      for (size_t c = 0; c < k; ++c) o[c] ^= F::mul(g[i], ir[c]);
+    }
+  }
+  return true;
+}
+
+bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
+
+// Device flag per host thread for verify (verify waits for its answer, so one
+// thread never has two in flight).  Kept for the life of the process: freeing
+// it from a thread-exit or static destructor could run after the HIP runtime
+// has shut down.
+int verify_flag(uint32_t **out) {
+  thread_local void *flag = nullptr;
+  if (!flag) TRS_TRY(hipMalloc(&flag, sizeof(uint32_t)));
+  *out = static_cast<uint32_t *>(flag);
+  return TRS_OK;
+}
+
+// Single-coefficient matrices for the slice API: entry c holds c at byte 0.
+// Allocated once and kept for the life of the process (see verify_flag).
+int slice_coef(uint8_t c, const uint32_t **out) {
+  static std::once_flag once;
+  static void *buf = nullptr;
+  static hipError_t err = hipSuccess;
+  std::call_once(once, [] {
+    std::vector<uint8_t> t(256 * kOutStride, 0);
+    for (int c = 0; c < 256; ++c) t[size_t(c) * kOutStride] = uint8_t(c);
+    err = hipMalloc(&buf, t.size());
+    if (err == hipSuccess) err = hipMemcpy(buf, t.data(), t.size(), hipMemcpyHostToDevice);
+  });
+  if (err != hipSuccess) return hip_status(err);
+  *out = reinterpret_cast<const uint32_t *>(static_cast<const uint8_t *>(buf) +
+                                            size_t(c) * kOutStride);
+  return TRS_OK;
+}
+
+// Device copy of a pointer table, stream-ordered (allocated, filled and freed
+// on `stream`).  The staged copy from pageable memory has finished reading
+// the host vector when hipMemcpyAsync returns.
+struct PtrTable {
+  void *d = nullptr;
+  hipStream_t s = nullptr;
+  ~PtrTable() {
+    if (d) (void)hipFreeAsync(d, s);
+  }
+  int upload(const std::vector<const uint8_t *> &v, hipStream_t stream) {
+    s = stream;
+    const size_t bytes = v.size() * sizeof(void *);
+    TRS_TRY(hipMallocAsync(&d, bytes ? bytes : 8, stream));
+    if (bytes) TRS_TRY(hipMemcpyAsync(d, v.data(), bytes, hipMemcpyHostToDevice, stream));
+    return TRS_OK;
+  }
+};
+
+}  // namespace
+}  // namespace trs
+
+using namespace trs;
+
+// ============================================================================
+// codec objects
+// ============================================================================
+
+struct trs_rs8 {
+  size_t k = 0, m = 0, n = 0;
+  Matrix<F8> G;
+  DevCoef parity;
+  PlanCache plans;
+};
+
+struct trs_dec8 {
+  size_t k = 0, n = 0;
+  std::shared_ptr<Plan> plan;
+};
+
+struct trs_rs16 {
+  size_t k = 0, m = 0, n = 0;
+  Matrix<F16> G;
+  DevCoef parity;
+  PlanCache plans;
+};
+
+struct trs_dec16 {
+  size_t k = 0, n = 0;
+  std::shared_ptr<Plan> plan;
+};
+
+namespace {
+
+// ---- GF(2^8) plan build / lookup (src/reedsolomon.rs:537-613) --------------
+int plan8_for(const trs_rs8 *rs, const Mask &m, std::shared_ptr<Plan> *out) {
+  if (auto p = rs->plans.find(m)) {
+    *out = p;
+    return TRS_OK;
+  }
+  auto p = std::make_shared<Plan>();
+  p->presence = m;
+  split_indices(m, rs->k, rs->n, *p);
+  if (!fused_rows(rs->G, rs->k, *p, p->rows8)) return TRS_SINGULAR_MATRIX;
+  const int st = upload_coef8(p->coef, p->rows8.data(),
+                              p->missing_data.size() + p->missing_parity.size(), rs->k);
+  if (st) return st;
+  *out = rs->plans.insert(m, p);
+  return TRS_OK;
+}
+
+int plan16_for(const trs_rs16 *rs, const Mask &m, std::shared_ptr<Plan> *out) {
+  if (auto p = rs->plans.find(m)) {
+    *out = p;
+    return TRS_OK;
+  }
+  auto p = std::make_shared<Plan>();
+  p->presence = m;
+  split_indices(m, rs->k, rs->n, *p);
+  std::vector<uint16_t> rows;
+  if (!fused_rows(rs->G, rs->k, *p, rows)) return TRS_SINGULAR_MATRIX;
+  const int st = upload_coef16(p->coef, rows.data(),
+                               p->missing_data.size() + p->missing_parity.size(), rs->k);
+  if (st) return st;
+  *out = rs->plans.insert(m, p);
+  return TRS_OK;
+}
+
+// scan_shards (src/reedsolomon.rs:801-821; RS16 adds the even-length rule,
+// src/reedsolomon16.rs:758-789).  Returns the status; fills mask/count/len.
+int scan_shards(const size_t *lens, const uint8_t *present, size_t n, bool even, Mask &mask,
+                size_t &count, size_t &len) {
+  mask.assign((n + 63) / 64, 0);
+  count = 0;
+  len = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (!present[i]) continue;
+    const size_t l = lens[i];
+    if (l == 0) return TRS_EMPTY_SHARD;
+    if (even && (l & 1)) return TRS_INCORRECT_SHARD_SIZE;
+    if (len && l != len) return TRS_INCORRECT_SHARD_SIZE;
+    len = l;
+    mask[i / 64] |= uint64_t(1) << (i % 64);
+    ++count;
+  }
+  return TRS_OK;
+}
+
+// The gather of reconstruct_with_plan (src/reedsolomon.rs:829-864): survivors
+// are the plan's valid indices; every missing shard that will be written must
+// have a buffer of the shard length.  Fills ptrs as [inputs..., outputs...].
+int gather(const Plan &p, uint8_t *const *shards, const size_t *lens, size_t k, size_t len,
+           bool data_only, std::vector<const uint8_t *> &ptrs, size_t &nout) {
+  // The reference checks missing buffers in index order while gathering.
+  for (uint32_t i : p.missing_data)
+    if (lens[i] != len) return TRS_INCORRECT_SHARD_SIZE;
+  if (!data_only)
+    for (uint32_t i : p.missing_parity)
+      if (lens[i] != len) return TRS_INCORRECT_SHARD_SIZE;
+  ptrs.clear();
+  for (size_t r = 0; r < k; ++r) ptrs.push_back(shards[p.valid[r]]);
+  for (uint32_t i : p.missing_data) ptrs.push_back(shards[i]);
+  nout = p.missing_data.size();
+  if (!data_only) {
+    for (uint32_t i : p.missing_parity) ptrs.push_back(shards[i]);
+    nout += p.missing_parity.size();
+  }
+  for (auto q : ptrs)
+    if (!q) return TRS_ERR_INVALID_ARGUMENT;
+  return TRS_OK;
+}
+
+int run_gf8(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout, const DevCoef &coef,
+            size_t len, uint32_t mode, uint32_t *flag, hipStream_t stream) {
+  if (nout == 0 || len == 0) return TRS_OK;
+  if (k + nout > size_t(kInlinePtrs)) return TRS_ERR_INVALID_ARGUMENT;
+  Gf8Call c{};
+  bool al = true;
+  for (size_t i = 0; i < k + nout; ++i) {
+    c.ptr[i] = ptrs[i];
+    al = al && aligned16(ptrs[i]);
+  }
+  c.coef = coef.ptr();
+  c.len = len;
+  c.k = uint32_t(k);
+  c.nout = uint32_t(nout);
+  c.ostride = coef.ostride;
+  c.vec_ok = al ? 1u : 0u;
+  c.flag = flag;
+  return hip_status(launch_gf8_apply(c, mode, stream));
+}
+
+// Split-plane shards of `len` bytes (plane = len / 2).
+int run_gf16(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout,
+             const DevCoef &coef, size_t len, uint32_t mode, uint32_t *flag,
+             hipStream_t stream) {
+  if (nout == 0 || len == 0) return TRS_OK;
+  const size_t plane = len / 2;
+  bool al = (plane % 16) == 0;
+  for (auto q : ptrs) al = al && aligned16(q);
+  PtrTable table;
+  int st = table.upload(ptrs, stream);
+  if (st) return st;
+  Gf16Call c{};
+  c.ptrs = static_cast<const uint8_t *const *>(table.d);
+  c.coef = coef.ptr();
+  c.plane = plane;
+  c.k = uint32_t(k);
+  c.nout = uint32_t(nout);
+  c.ostride = coef.ostride;
+  c.vec_ok = al ? 1u : 0u;
+  c.mode = mode;
+  c.flag = flag;
+  return hip_status(launch_gf16_apply(c, stream));
+}
+
+int read_flag(uint32_t *flag, hipStream_t stream, int *ok) {
+  uint32_t h = 0;
+  TRS_TRY(hipMemcpyAsync(&h, flag, sizeof h, hipMemcpyDeviceToHost, stream));
+  TRS_TRY(hipStreamSynchronize(stream));
+  *ok = h ? 0 : 1;
+  return TRS_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+
+extern "C" {
+
+const char *trs_status_message(int s) {
+  switch (s) {
+    case TRS_OK: return "ok";
+    case TRS_TOO_FEW_SHARDS: return "The number of provided shards is smaller than the one in codec";
+    case TRS_TOO_MANY_SHARDS: return "The number of provided shards is greater than the one in codec";
+    case TRS_TOO_FEW_DATA_SHARDS: return "The number of provided data shards is smaller than the one in codec";
+    case TRS_TOO_MANY_DATA_SHARDS: return "The number of provided data shards is greater than the one in codec";
+    case TRS_TOO_FEW_PARITY_SHARDS: return "The number of provided parity shards is smaller than the one in codec";
+    case TRS_TOO_MANY_PARITY_SHARDS: return "The number of provided parity shards is greater than the one in codec";
+    case TRS_TOO_FEW_BUFFER_SHARDS: return "The number of provided buffer shards is smaller than the number of parity shards in codec";
+    case TRS_TOO_MANY_BUFFER_SHARDS: return "The number of provided buffer shards is greater than the number of parity shards in codec";
+    case TRS_INCORRECT_SHARD_SIZE: return "At least one of the provided shards is not of the correct size";
+    case TRS_TOO_FEW_SHARDS_PRESENT: return "The number of shards present is smaller than number of parity shards, cannot reconstruct missing shards";
+    case TRS_EMPTY_SHARD: return "The first shard provided is of zero length";
+    case TRS_INVALID_SHARD_FLAGS: return "The number of flags does not match the total number of shards";
+    case TRS_INVALID_INDEX: return "The data shard index provided is greater or equal to the number of data shards in codec";
+    case TRS_SINGULAR_MATRIX: return "The decode matrix for this shard pattern is singular and cannot be inverted";
+    case TRS_ERR_DEVICE: return "HIP runtime error";
+    case TRS_ERR_OUT_OF_MEMORY: return "device out of memory";
+    case TRS_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case TRS_ERR_UNSUPPORTED: return "unsupported";
+    default: return "unknown status";
+  }
+}
+
+const char *trs_version(void) { return "trs_gpu 0.1 (gfx950)"; }
+
+int trs_gf8_mul_slice(uint8_t *out, size_t out_len, const uint8_t *in, size_t in_len, uint8_t c,
+                      int xor_into, void *stream) {
+  if (out_len != in_len) return TRS_ERR_INVALID_ARGUMENT;
+  if (out_len == 0) return TRS_OK;
+  if (!out || !in) return TRS_ERR_INVALID_ARGUMENT;
+  const uint32_t *coef = nullptr;
+  int st = slice_coef(c, &coef);
+  if (st) return st;
+  Gf8Call call{};
+  call.ptr[0] = in;
+  call.ptr[1] = out;
+  call.coef = coef;
+  call.len = out_len;
+  call.k = 1;
+  call.nout = 1;
+  call.ostride = kOutStride;
+  call.vec_ok = aligned16(in) && aligned16(out);
+  return hip_status(launch_gf8_apply(call, xor_into ? kXorInto : kWrite, as_stream(stream)));
+}
+
+int trs_fill_random(uint8_t *dst, size_t len, uint64_t seed, void *stream) {
+  if (len && !dst) return TRS_ERR_INVALID_ARGUMENT;
+  return hip_status(launch_fill(dst, len, seed, as_stream(stream)));
+}
+
+// ---------------------------------------------------------------- RS8
+
+int trs_rs8_new(size_t k, size_t m, trs_rs8 **out) {
+  if (!out) return TRS_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  // src/reedsolomon.rs:167-175
+  if (k == 0) return TRS_TOO_FEW_DATA_SHARDS;
+  if (m == 0) return TRS_TOO_FEW_PARITY_SHARDS;
+  if (k + m > 256) return TRS_TOO_MANY_SHARDS;
+  auto rs = std::make_unique<trs_rs8>();
+  rs->k = k;
+  rs->m = m;
+  rs->n = k + m;
+  if (!build_generator<F8>(k, k + m, rs->G)) return TRS_SINGULAR_MATRIX;
+  const int st = upload_coef8(rs->parity, rs->G.row(k), m, k);
+  if (st) return st;
+  *out = rs.release();
+  return TRS_OK;
+}
+
+void trs_rs8_free(trs_rs8 *rs) { delete rs; }
+size_t trs_rs8_data_shard_count(const trs_rs8 *rs) { return rs ? rs->k : 0; }
+size_t trs_rs8_parity_shard_count(const trs_rs8 *rs) { return rs ? rs->m : 0; }
+size_t trs_rs8_total_shard_count(const trs_rs8 *rs) { return rs ? rs->n : 0; }
+
+const char *trs_rs8_encode_route(const trs_rs8 *rs, size_t shard_len) {
+  (void)rs;
+  (void)shard_len;
+  return "gpu-fused-matrix";
+}
+
+int trs_rs8_generator(const trs_rs8 *rs, uint8_t *out, size_t out_len) {
+  if (!rs || !out || out_len != rs->n * rs->k) return TRS_ERR_INVALID_ARGUMENT;
+  std::copy(rs->G.a.begin(), rs->G.a.end(), out);
+  return TRS_OK;
+}
+
+namespace {
+// check_piece_count_all + check_equal_lengths (src/reedsolomon.rs:701-722).
+int check_all8(size_t total, const size_t *lens, size_t n) {
+  if (n < total) return TRS_TOO_FEW_SHARDS;
+  if (n > total) return TRS_TOO_MANY_SHARDS;
+  const size_t len = lens[0];
+  if (len == 0) return TRS_EMPTY_SHARD;
+  for (size_t i = 0; i < n; ++i)
+    if (lens[i] != len) return TRS_INCORRECT_SHARD_SIZE;
+  return TRS_OK;
+}
+
+int encode8_ptrs(const trs_rs8 *rs, std::vector<const uint8_t *> &ptrs, size_t len,
+                 hipStream_t s) {
+  for (auto q : ptrs)
+    if (!q) return TRS_ERR_INVALID_ARGUMENT;
+  return run_gf8(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, s);
+}
+}  // namespace
+
+int trs_rs8_encode(const trs_rs8 *rs, uint8_t *const *shards, const size_t *lens, size_t n,
+                   void *stream) {
+  if (!rs || (n && (!shards || !lens))) return TRS_ERR_INVALID_ARGUMENT;
+  if (n == 0) return TRS_TOO_FEW_SHARDS;
+  int st = check_all8(rs->n, lens, n);
+  if (st) return st;
+  std::vector<const uint8_t *> ptrs(shards, shards + n);
+  return encode8_ptrs(rs, ptrs, lens[0], as_stream(stream));
+}
+
+int trs_rs8_encode_sep(const trs_rs8 *rs, const uint8_t *const *data, const size_t *dl,
+                       size_t nd, uint8_t *const *parity, const size_t *pl, size_t np,
+                       void *stream) {
+  if (!rs || (nd && (!data || !dl)) || (np && (!parity || !pl))) return TRS_ERR_INVALID_ARGUMENT;
+  // src/reedsolomon.rs:289-308
+  if (nd != rs->k) return nd < rs->k ? TRS_TOO_FEW_DATA_SHARDS : TRS_TOO_MANY_DATA_SHARDS;
+  if (np != rs->m) return np < rs->m ? TRS_TOO_FEW_PARITY_SHARDS : TRS_TOO_MANY_PARITY_SHARDS;
+  const size_t len = dl[0];
+  for (size_t i = 0; i < nd; ++i)
+    if (dl[i] != len) return TRS_INCORRECT_SHARD_SIZE;
+  for (size_t i = 0; i < np; ++i)
+    if (pl[i] != len) return TRS_INCORRECT_SHARD_SIZE;
+  if (len == 0) return TRS_OK;
+  std::vector<const uint8_t *> ptrs(data, data + nd);
+  ptrs.insert(ptrs.end(), parity, parity + np);
+  return encode8_ptrs(rs, ptrs, len, as_stream(stream));
+}
+
+int trs_rs8_encode_rows_into(const trs_rs8 *rs, const uint8_t *data, size_t data_len,
+                             size_t row_len, uint8_t *parity, size_t parity_len, void *stream) {
+  if (!rs) return TRS_ERR_INVALID_ARGUMENT;
+  // src/reedsolomon.rs:673-680
+  if (data_len != rs->k * row_len) return TRS_INCORRECT_SHARD_SIZE;
+  if (parity_len != rs->m * row_len) return TRS_INCORRECT_SHARD_SIZE;
+  if (row_len == 0) return TRS_OK;
+  if (!data || !parity) return TRS_ERR_INVALID_ARGUMENT;
+  std::vector<const uint8_t *> ptrs;
+  for (size_t i = 0; i < rs->k; ++i) ptrs.push_back(data + i * row_len);
+  for (size_t i = 0; i < rs->m; ++i) ptrs.push_back(parity + i * row_len);
+  return encode8_ptrs(rs, ptrs, row_len, as_stream(stream));
+}
+
+int trs_rs8_verify(const trs_rs8 *rs, const uint8_t *const *shards, const size_t *lens, size_t n,
+                   int *ok, void *stream) {
+  if (!rs || !ok || (n && (!shards || !lens))) return TRS_ERR_INVALID_ARGUMENT;
+  if (n == 0) return TRS_TOO_FEW_SHARDS;
+  int st = check_all8(rs->n, lens, n);
+  if (st) return st;
+  std::vector<const uint8_t *> ptrs(shards, shards + n);
+  for (auto q : ptrs)
+    if (!q) return TRS_ERR_INVALID_ARGUMENT;
+  uint32_t *flag = nullptr;
+  if ((st = verify_flag(&flag))) return st;
+  hipStream_t s = as_stream(stream);
+  TRS_TRY(hipMemsetAsync(flag, 0, sizeof(uint32_t), s));
+  if ((st = run_gf8(ptrs, rs->k, rs->m, rs->parity, lens[0], kVerify, flag, s))) return st;
+  return read_flag(flag, s, ok);
+}
+
+namespace {
+// reconstruct_internal / PreparedDecoder::reconstruct_internal
+// (src/reedsolomon.rs:615-641, :771-796).  `plan` non-null = prepared.
+int reconstruct8(const trs_rs8 *rs, size_t k, size_t total, std::shared_ptr<Plan> plan,
+                 uint8_t *const *shards, const size_t *lens, const uint8_t *present, size_t n,
+                 int data_only, hipStream_t s) {
+  if (n && (!shards || !lens || !present)) return TRS_ERR_INVALID_ARGUMENT;
+  if (n != total) return n < total ? TRS_TOO_FEW_SHARDS : TRS_TOO_MANY_SHARDS;
+  Mask mask;
+  size_t count = 0, len = 0;
+  int st = scan_shards(lens, present, n, false, mask, count, len);
+  if (st) return st;
+  if (plan) {
+    if (mask != plan->presence) return TRS_INVALID_SHARD_FLAGS;
+    if (count == total) return TRS_OK;
+    if (len == 0) return TRS_TOO_FEW_SHARDS_PRESENT;
+  } else {
+    if (count == total) return TRS_OK;
+    if (count < k) return TRS_TOO_FEW_SHARDS_PRESENT;
+    if ((st = plan8_for(rs, mask, &plan))) return st;
+  }
+  std::vector<const uint8_t *> ptrs;
+  size_t nout = 0;
+  if ((st = gather(*plan, shards, lens, k, len, data_only != 0, ptrs, nout))) return st;
+  return run_gf8(ptrs, k, nout, plan->coef, len, kWrite, nullptr, s);
+}
+}  // namespace
+
+int trs_rs8_reconstruct(const trs_rs8 *rs, uint8_t *const *shards, const size_t *lens,
+                        const uint8_t *present, size_t n, int data_only, void *stream) {
+  if (!rs) return TRS_ERR_INVALID_ARGUMENT;
+  return reconstruct8(rs, rs->k, rs->n, nullptr, shards, lens, present, n, data_only,
+                      as_stream(stream));
+}
+
+int trs_rs8_reconstruct_rows(const trs_rs8 *rs, uint8_t *rows, size_t rows_len, size_t row_len,
+                             const uint8_t *present, size_t n_present, void *stream) {
+  if (!rs || (n_present && !present)) return TRS_ERR_INVALID_ARGUMENT;
+  // src/reedsolomon.rs:520-528
+  if (n_present != rs->n) return TRS_INVALID_SHARD_FLAGS;
+  if (row_len == 0) return TRS_EMPTY_SHARD;
+  if (rows_len != rs->n * row_len) return TRS_INCORRECT_SHARD_SIZE;
+  if (!rows) return TRS_ERR_INVALID_ARGUMENT;
+  std::vector<uint8_t *> shards(rs->n);
+  std::vector<size_t> lens(rs->n, row_len);
+  for (size_t i = 0; i < rs->n; ++i) shards[i] = rows + i * row_len;
+  return reconstruct8(rs, rs->k, rs->n, nullptr, shards.data(), lens.data(), present, rs->n, 0,
+                      as_stream(stream));
+}
+
+int trs_rs8_prepare_decode(const trs_rs8 *rs, const uint8_t *present, size_t n, trs_dec8 **out) {
+  if (!rs || !out || (n && !present)) return TRS_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  // src/reedsolomon.rs:485-500
+  if (n != rs->n) return TRS_INVALID_SHARD_FLAGS;
+  Mask m = make_mask(present, n);
+  size_t count = 0;
+  for (size_t i = 0; i < n; ++i) count += present[i] != 0;
+  if (count < rs->k) return TRS_TOO_FEW_SHARDS_PRESENT;
+  std::shared_ptr<Plan> plan;
+  int st = plan8_for(rs, m, &plan);
+  if (st) return st;
+  auto d = std::make_unique<trs_dec8>();
+  d->k = rs->k;
+  d->n = rs->n;
+  d->plan = plan;
+  *out = d.release();
+  return TRS_OK;
+}
+
+int trs_dec8_reconstruct(const trs_dec8 *dec, uint8_t *const *shards, const size_t *lens,
+                         const uint8_t *present, size_t n, int data_only, void *stream) {
+  if (!dec) return TRS_ERR_INVALID_ARGUMENT;
+  return reconstruct8(nullptr, dec->k, dec->n, dec->plan, shards, lens, present, n, data_only,
+                      as_stream(stream));
+}
+
+void trs_dec8_free(trs_dec8 *dec) { delete dec; }
+
+// ---------------------------------------------------------------- RS16
+
+int trs_rs16_new(size_t k, size_t m, trs_rs16 **out) {
+  if (!out) return TRS_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  // src/reedsolomon16.rs:282-290
+  if (k == 0) return TRS_TOO_FEW_DATA_SHARDS;
+  if (m == 0) return TRS_TOO_FEW_PARITY_SHARDS;
+  if (k + m > 65536) return TRS_TOO_MANY_SHARDS;
+  auto rs = std::make_unique<trs_rs16>();
+  rs->k = k;
+  rs->m = m;
+  rs->n = k + m;
+  if (!build_generator<F16>(k, k + m, rs->G)) return TRS_SINGULAR_MATRIX;
+  const int st = upload_coef16(rs->parity, rs->G.row(k), m, k);
+  if (st) return st;
+  *out = rs.release();
+  return TRS_OK;
+}
+
+void trs_rs16_free(trs_rs16 *rs) { delete rs; }
+size_t trs_rs16_data_shard_count(const trs_rs16 *rs) { return rs ? rs->k : 0; }
+size_t trs_rs16_parity_shard_count(const trs_rs16 *rs) { return rs ? rs->m : 0; }
+size_t trs_rs16_total_shard_count(const trs_rs16 *rs) { return rs ? rs->n : 0; }
+const char *trs_rs16_encode_route(const trs_rs16 *rs) {
+  (void)rs;
+  return "matrix";
+}
+
+int trs_rs16_generator(const trs_rs16 *rs, uint16_t *out, size_t out_len) {
+  if (!rs || !out || out_len != rs->n * rs->k) return TRS_ERR_INVALID_ARGUMENT;
+  std::copy(rs->G.a.begin(), rs->G.a.end(), out);
+  return TRS_OK;
+}
+
+namespace {
+// note_shard_len over a shard list (src/reedsolomon16.rs:758-772, :704-711).
+int check_shards16(const size_t *lens, size_t n, size_t *len_out) {
+  size_t len = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const size_t l = lens[i];
+    if (l == 0) return TRS_EMPTY_SHARD;
+    if (l & 1) return TRS_INCORRECT_SHARD_SIZE;
+    if (len && l != len) return TRS_INCORRECT_SHARD_SIZE;
+    len = l;
+  }
+  if (!len) return TRS_EMPTY_SHARD;
+  *len_out = len;
+  return TRS_OK;
+}
+}  // namespace
+
+int trs_rs16_encode(const trs_rs16 *rs, uint8_t *const *shards, const size_t *lens, size_t n,
+                    void *stream) {
+  if (!rs || (n && (!shards || !lens))) return TRS_ERR_INVALID_ARGUMENT;
+  if (n < rs->n) return TRS_TOO_FEW_SHARDS;
+  if (n > rs->n) return TRS_TOO_MANY_SHARDS;
+  size_t len = 0;
+  int st = check_shards16(lens, n, &len);
+  if (st) return st;
+  std::vector<const uint8_t *> ptrs(shards, shards + n);
+  for (auto q : ptrs)
+    if (!q) return TRS_ERR_INVALID_ARGUMENT;
+  return run_gf16(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, as_stream(stream));
+}
+
+int trs_rs16_encode_sep(const trs_rs16 *rs, const uint8_t *const *data, const size_t *dl,
+                        size_t nd, uint8_t *const *parity, const size_t *pl, size_t np,
+                        void *stream) {
+  if (!rs || (nd && (!data || !dl)) || (np && (!parity || !pl))) return TRS_ERR_INVALID_ARGUMENT;
+  // src/reedsolomon16.rs:427-446
+  if (nd != rs->k) return nd < rs->k ? TRS_TOO_FEW_DATA_SHARDS : TRS_TOO_MANY_DATA_SHARDS;
+  if (np != rs->m) return np < rs->m ? TRS_TOO_FEW_PARITY_SHARDS : TRS_TOO_MANY_PARITY_SHARDS;
+  std::vector<size_t> lens(dl, dl + nd);
+  lens.insert(lens.end(), pl, pl + np);
+  size_t len = 0;
+  int st = check_shards16(lens.data(), lens.size(), &len);
+  if (st) return st;
+  std::vector<const uint8_t *> ptrs(data, data + nd);
+  ptrs.insert(ptrs.end(), parity, parity + np);
+  for (auto q : ptrs)
+    if (!q) return TRS_ERR_INVALID_ARGUMENT;
+  return run_gf16(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, as_stream(stream));
+}
+
+int trs_rs16_encode_sep_matrix(const trs_rs16 *rs, const uint8_t *const *data,
+                               const size_t *dl, size_t nd, uint8_t *const *parity,
+                               const size_t *pl, size_t np, void *stream) {
+  if (!rs || (nd && (!data || !dl)) || (np && (!parity || !pl))) return TRS_ERR_INVALID_ARGUMENT;
+  // src/reedsolomon16.rs:329-335 checks lengths only (counts via the apply).
+  std::vector<size_t> lens(dl, dl + nd);
+  lens.insert(lens.end(), pl, pl + np);
+  size_t len = 0;
+  int st = check_shards16(lens.data(), lens.size(), &len);
+  if (st) return st;
+  if (nd != rs->k || np != rs->m) return TRS_ERR_INVALID_ARGUMENT;
+  std::vector<const uint8_t *> ptrs(data, data + nd);
+  ptrs.insert(ptrs.end(), parity, parity + np);
+  return run_gf16(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, as_stream(stream));
+}
+
+int trs_rs16_verify(const trs_rs16 *rs, const uint8_t *const *shards, const size_t *lens,
+                    size_t n, int *ok, void *stream) {
+  if (!rs || !ok || (n && (!shards || !lens))) return TRS_ERR_INVALID_ARGUMENT;
+  if (n < rs->n) return TRS_TOO_FEW_SHARDS;
+  if (n > rs->n) return TRS_TOO_MANY_SHARDS;
+  size_t len = 0;
+  int st = check_shards16(lens, n, &len);
+  if (st) return st;
+  std::vector<const uint8_t *> ptrs(shards, shards + n);
+  for (auto q : ptrs)
+    if (!q) return TRS_ERR_INVALID_ARGUMENT;
+  uint32_t *flag = nullptr;
+  if ((st = verify_flag(&flag))) return st;
+  hipStream_t s = as_stream(stream);
+  TRS_TRY(hipMemsetAsync(flag, 0, sizeof(uint32_t), s));
+  if ((st = run_gf16(ptrs, rs->k, rs->m, rs->parity, len, kVerify, flag, s))) return st;
+  return read_flag(flag, s, ok);
+}
+
+namespace {
+int reconstruct16(const trs_rs16 *rs, size_t k, size_t total, std::shared_ptr<Plan> plan,
+                  uint8_t *const *shards, const size_t *lens, const uint8_t *present, size_t n,
+                  int data_only, hipStream_t s) {
+  if (n && (!shards || !lens || !present)) return TRS_ERR_INVALID_ARGUMENT;
+  if (n != total) return n < total ? TRS_TOO_FEW_SHARDS : TRS_TOO_MANY_SHARDS;
+  Mask mask;
+  size_t count = 0, len = 0;
+  int st = scan_shards(lens, present, n, true, mask, count, len);
+  if (st) return st;
+  if (plan) {
+    if (mask != plan->presence) return TRS_INVALID_SHARD_FLAGS;
+    if (count == total) return TRS_OK;
+    if (len == 0) return TRS_TOO_FEW_SHARDS_PRESENT;
+  } else {
+    if (count == total) return TRS_OK;
+    if (count < k) return TRS_TOO_FEW_SHARDS_PRESENT;
+    if ((st = plan16_for(rs, mask, &plan))) return st;
+  }
+  std::vector<const uint8_t *> ptrs;
+  size_t nout = 0;
+  if ((st = gather(*plan, shards, lens, k, len, data_only != 0, ptrs, nout))) return st;
+  return run_gf16(ptrs, k, nout, plan->coef, len, kWrite, nullptr, s);
+}
+}  // namespace
+
+int trs_rs16_reconstruct(const trs_rs16 *rs, uint8_t *const *shards, const size_t *lens,
+                         const uint8_t *present, size_t n, int data_only, void *stream) {
+  if (!rs) return TRS_ERR_INVALID_ARGUMENT;
+  return reconstruct16(rs, rs->k, rs->n, nullptr, shards, lens, present, n, data_only,
+                       as_stream(stream));
+}
+
+int trs_rs16_reconstruct_rows(const trs_rs16 *rs, uint8_t *rows, size_t rows_len, size_t row_len,
+                              const uint8_t *present, size_t n_present, void *stream) {
+  if (!rs || (n_present && !present)) return TRS_ERR_INVALID_ARGUMENT;
+  // src/reedsolomon16.rs:552-560
+  if (n_present != rs->n) return TRS_INVALID_SHARD_FLAGS;
+  if (row_len == 0 || (row_len & 1)) return TRS_INCORRECT_SHARD_SIZE;
+  if (rows_len != rs->n * row_len) return TRS_INCORRECT_SHARD_SIZE;
+  if (!rows) return TRS_ERR_INVALID_ARGUMENT;
+  std::vector<uint8_t *> shards(rs->n);
+  std::vector<size_t> lens(rs->n, row_len);
+  for (size_t i = 0; i < rs->n; ++i) shards[i] = rows + i * row_len;
+  return reconstruct16(rs, rs->k, rs->n, nullptr, shards.data(), lens.data(), present, rs->n, 0,
+                       as_stream(stream));
+}
+
+int trs_rs16_prepare_decode(const trs_rs16 *rs, const uint8_t *present, size_t n,
+                            trs_dec16 **out) {
+  if (!rs || !out || (n && !present)) return TRS_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (n != rs->n) return TRS_INVALID_SHARD_FLAGS;
+  Mask m = make_mask(present, n);
+  size_t count = 0;
+  for (size_t i = 0; i < n; ++i) count += present[i] != 0;
+  if (count < rs->k) return TRS_TOO_FEW_SHARDS_PRESENT;
+  std::shared_ptr<Plan> plan;
+  int st = plan16_for(rs, m, &plan);
+  if (st) return st;
+  auto d = std::make_unique<trs_dec16>();
+  d->k = rs->k;
+  d->n = rs->n;
+  d->plan = plan;
+  *out = d.release();
+  return TRS_OK;
+}
+
+int trs_dec16_reconstruct(const trs_dec16 *dec, uint8_t *const *shards, const size_t *lens,
+                          const uint8_t *present, size_t n, int data_only, void *stream) {
+  if (!dec) return TRS_ERR_INVALID_ARGUMENT;
+  return reconstruct16(nullptr, dec->k, dec->n, dec->plan, shards, lens, present, n, data_only,
+                       as_stream(stream));
+}
+
+void trs_dec16_free(trs_dec16 *dec) { delete dec; }
+
+}  // extern "C"
+
+// ============================================================================
+// batched GF(2^8) work list
+// ============================================================================
+
+struct trs_batch8 {
+  struct Req {
+    std::shared_ptr<Plan> plan;  // keeps a reconstruct plan's coefficients alive
+    const DevCoef *coef = nullptr;
+    std::vector<const uint8_t *> ptrs;
+    uint64_t len = 0;
+    uint32_t k = 0, nout = 0;
+  };
+  std::vector<Req> reqs;
+  DevBuf jobs, tiles, ptrs;
+  uint64_t total_tiles = 0;
+  uint32_t grid = 0;
+  bool prepared = false;
+};
+
+extern "C" {
+
+int trs_batch8_new(trs_batch8 **out) {
+  if (!out) return TRS_ERR_INVALID_ARGUMENT;
+  *out = new trs_batch8();
+  return TRS_OK;
+}
+
+void trs_batch8_free(trs_batch8 *b) { delete b; }
+
+void trs_batch8_clear(trs_batch8 *b) {
+  if (!b) return;
+  b->reqs.clear();
+  b->prepared = false;
+  b->total_tiles = 0;
+}
+
+size_t trs_batch8_size(const trs_batch8 *b) { return b ? b->reqs.size() : 0; }
+
+uint64_t trs_batch8_bytes(const trs_batch8 *b) {
+  if (!b) return 0;
+  uint64_t t = 0;
+  for (auto &r : b->reqs) t += uint64_t(r.k + r.nout) * r.len;
+  return t;
+}
+
+int trs_batch8_add_encode(trs_batch8 *b, const trs_rs8 *rs, uint8_t *const *shards,
+                          const size_t *lens, size_t n) {
+  if (!b || !rs || (n && (!shards || !lens))) return TRS_ERR_INVALID_ARGUMENT;
+  if (n == 0) return TRS_TOO_FEW_SHARDS;
+  int st = check_all8(rs->n, lens, n);
+  if (st) return st;
+  trs_batch8::Req r;
+  r.coef = &rs->parity;
+  r.ptrs.assign(shards, shards + n);
+  for (auto q : r.ptrs)
+    if (!q) return TRS_ERR_INVALID_ARGUMENT;
+  r.len = lens[0];
+  r.k = uint32_t(rs->k);
+  r.nout = uint32_t(rs->m);
+  b->reqs.push_back(std::move(r));
+  b->prepared = false;
+  return TRS_OK;
+}
+
+int trs_batch8_add_reconstruct(trs_batch8 *b, const trs_rs8 *rs, uint8_t *const *shards,
+                               const size_t *lens, const uint8_t *present, size_t n,
+                               int data_only) {
+  if (!b || !rs || (n && (!shards || !lens || !present))) return TRS_ERR_INVALID_ARGUMENT;
+  if (n != rs->n) return n < rs->n ? TRS_TOO_FEW_SHARDS : TRS_TOO_MANY_SHARDS;
+  Mask mask;
+  size_t count = 0, len = 0;
+  int st = scan_shards(lens, present, n, false, mask, count, len);
+  if (st) return st;
+  if (count == rs->n) return TRS_OK;
+  if (count < rs->k) return TRS_TOO_FEW_SHARDS_PRESENT;
+  std::shared_ptr<Plan> plan;
+  if ((st = plan8_for(rs, mask, &plan))) return st;
+  trs_batch8::Req r;
+  size_t nout = 0;
+  if ((st = gather(*plan, shards, lens, rs->k, len, data_only != 0, r.ptrs, nout))) return st;
+  if (nout == 0) return TRS_OK;
+  r.plan = plan;
+  r.coef = &plan->coef;
+  r.len = len;
+  r.k = uint32_t(rs->k);
+  r.nout = uint32_t(nout);
+  b->reqs.push_back(std::move(r));
+  b->prepared = false;
+  return TRS_OK;
+}
+
+int trs_batch8_prepare(trs_batch8 *b) {
+  if (!b) return TRS_ERR_INVALID_ARGUMENT;
+  size_t nptr = 0;
+  for (auto &r : b->reqs) nptr += r.ptrs.size();
+  TRS_TRY(b->ptrs.ensure(nptr * sizeof(void *) + 8));
+  TRS_TRY(b->jobs.ensure(b->reqs.size() * sizeof(BatchJob) + 8));
+  TRS_TRY(b->tiles.ensure((b->reqs.size() + 1) * sizeof(uint64_t)));
+  std::vector<const uint8_t *> hp;
+  hp.reserve(nptr);
+  std::vector<BatchJob> jobs(b->reqs.size());
+  std::vector<uint64_t> tb(b->reqs.size() + 1, 0);
+  const auto *dptr = static_cast<const uint8_t *const *>(b->ptrs.p);
+  uint64_t tile = 0;
+  for (size_t j = 0; j < b->reqs.size(); ++j) {
+    const auto &r = b->reqs[j];
+    BatchJob &J = jobs[j];
+    J.ptrs = dptr + hp.size();
+    bool al = true;
+    for (auto q : r.ptrs) {
+      hp.push_back(q);
+      al = al && aligned16(q);
+    }
+    J.coef = r.coef->ptr();
+    J.len = r.len;
+    J.k = r.k;
+    J.nout = r.nout;
+    J.ostride = r.coef->ostride;
+    J.vec_ok = al ? 1u : 0u;
+    J.out_tiles = (r.nout + kOutStride - 1) / kOutStride;
+    J.tile_begin = tile;
+    tb[j] = tile;
+    tile += ((r.len + kChunkBytes - 1) / kChunkBytes) * J.out_tiles;
+  }
+  tb[b->reqs.size()] = tile;
+  b->total_tiles = tile;
+  if (!hp.empty())
+    TRS_TRY(hipMemcpy(b->ptrs.p, hp.data(), hp.size() * sizeof(void *), hipMemcpyHostToDevice));
+  if (!jobs.empty())
+    TRS_TRY(hipMemcpy(b->jobs.p, jobs.data(), jobs.size() * sizeof(BatchJob),
+                      hipMemcpyHostToDevice));
+  TRS_TRY(hipMemcpy(b->tiles.p, tb.data(), tb.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  uint64_t grid = uint64_t(cus) * 8;
+  if (const char *e = std::getenv("TRS_BATCH_GRID_PER_CU")) grid = uint64_t(cus) * std::atoi(e);
+  if (grid > tile) grid = tile;
+  if (grid == 0) grid = 1;
+  b->grid = uint32_t(grid);
+  b->prepared = true;
+  return TRS_OK;
+}
+
+int trs_batch8_run(trs_batch8 *b, void *stream) {
+  if (!b) return TRS_ERR_INVALID_ARGUMENT;
+  if (!b->prepared) {
+    int st = trs_batch8_prepare(b);
+    if (st) return st;
+  }
+  if (b->total_tiles == 0) return TRS_OK;
+  return hip_status(launch_batch8(static_cast<const BatchJob *>(b->jobs.p),
+                                  static_cast<const uint64_t *>(b->tiles.p),
+                                  uint32_t(b->reqs.size()), b->total_tiles, b->grid,
+                                  as_stream(stream)));
+}
+
+}  // extern "C"

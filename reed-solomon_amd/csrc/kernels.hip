@@ -1,0 +1,323 @@
+// HIP kernels for gfx950 (MI355X): fused GF(2^8) matrix apply, tower
+// (GF((2^8)^2)) Karatsuba apply over split planes, the batched work-list
+// kernel, and the seeded fill used by benches and tests.
+//
+// Arithmetic.  Multiplying a packed word of 4 bytes by 2 in GF(2^8) (poly
+// 0x11d) is xtime4: shift every byte left, then fold 0x1d into the bytes whose
+// top bit fell out.  A coefficient c is applied as the XOR of the powers
+// x*2^b for the set bits b of c; the coefficient is uniform across the
+// workgroup, so its bits become scalar masks and each term is one
+// `acc ^= p & mask` (a single v_bitop3 on gfx950).  The powers of an input
+// word are computed once and reused by every output row.  This is exact
+// GF(2^8) arithmetic: it matches galois::mul (src/galois.rs:74) bit for bit.
+//
+// Layout.  Every lane owns 16 consecutive bytes of every shard (one 16-byte
+// global load per input), so a 256-lane workgroup streams a 4 KiB column
+// chunk of all inputs once, coalesced, and keeps up to 16 output rows in
+// VGPRs.  blockIdx.y tiles larger output counts.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace trs {
+
+namespace {
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// The reduction bytes (0 or 0x1d per byte) come from one packed 16-bit
+// multiply (full rate), not a 32-bit v_mul_lo (quarter rate).
+__device__ __forceinline__ uint32_t xtime4(uint32_t x) {
+  const uint32_t hi = (x >> 7) & 0x01010101u;
+  const u16x2 red = __builtin_bit_cast(u16x2, hi) * (u16x2){0x1d, 0x1d};
+  return ((x << 1) & 0xfefefefeu) ^ __builtin_bit_cast(uint32_t, red);
+}
+
+// Loads 16 bytes at p; `full` lanes use one 16-byte load, the others (tail
+// or unaligned shards) assemble the `rem` valid bytes and zero the rest.
+__device__ __forceinline__ void load_lane(const uint8_t *p, bool full, uint64_t rem,
+                                          uint32_t x[4]) {
+  if (full) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(p);
+    x[0] = v.x;
+    x[1] = v.y;
+    x[2] = v.z;
+    x[3] = v.w;
+  } else {
+    x[0] = x[1] = x[2] = x[3] = 0;
+#pragma unroll
+    for (int b = 0; b < 16; ++b)
+      if (uint64_t(b) < rem) x[b >> 2] |= uint32_t(p[b]) << (8 * (b & 3));
+  }
+}
+
+__device__ __forceinline__ void store_lane(uint8_t *p, bool full, uint64_t rem,
+                                           const uint32_t x[4]) {
+  if (full) {
+    *reinterpret_cast<uint4 *>(p) = make_uint4(x[0], x[1], x[2], x[3]);
+  } else {
+#pragma unroll
+    for (int b = 0; b < 16; ++b)
+      if (uint64_t(b) < rem) p[b] = uint8_t(x[b >> 2] >> (8 * (b & 3)));
+  }
+}
+
+// acc[j] ^= c_j * x for the NT coefficients at cw (transposed, padded rows:
+// byte j of the NT/4 words at cw is output j's coefficient).
+template <int NT>
+__device__ __forceinline__ void mac_rows(uint32_t (&acc)[NT][4], const uint32_t x[4],
+                                         const uint32_t *cw) {
+  uint32_t p[8][4];
+#pragma unroll
+  for (int w = 0; w < 4; ++w) p[0][w] = x[w];
+#pragma unroll
+  for (int b = 1; b < 8; ++b)
+#pragma unroll
+    for (int w = 0; w < 4; ++w) p[b][w] = xtime4(p[b - 1][w]);
+#pragma unroll
+  for (int q = 0; q < NT / 4; ++q) {
+    const uint32_t word = cw[q];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const uint32_t c = (word >> (8 * s)) & 0xffu;
+      if (c) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          const uint32_t msk = 0u - ((c >> b) & 1u);
+#pragma unroll
+          for (int w = 0; w < 4; ++w) acc[q * 4 + s][w] ^= p[b][w] & msk;
+        }
+      }
+    }
+  }
+}
+
+// One lane's share of a GF(2^8) apply tile: outputs jbase..jbase+nt over the
+// 16 bytes at `off`.  ptr_at(i) returns shard pointer i (inputs first).
+template <int NT, uint32_t MODE, typename PtrAt>
+__device__ __forceinline__ void gf8_tile(PtrAt ptr_at, const uint32_t *coef, uint64_t len,
+                                         uint32_t k, uint32_t nout, uint32_t ostride,
+                                         bool vec_ok, uint32_t jbase, uint64_t off,
+                                         uint32_t *flag) {
+  const uint64_t rem = len - off;
+  const bool full = vec_ok && rem >= 16;
+  const uint32_t nt = min(uint32_t(NT), nout - jbase);
+  uint32_t acc[NT][4];
+#pragma unroll
+  for (int j = 0; j < NT; ++j)
+#pragma unroll
+    for (int w = 0; w < 4; ++w) acc[j][w] = 0;
+
+  for (uint32_t i = 0; i < k; ++i) {
+    uint32_t x[4];
+    load_lane(ptr_at(i) + off, full, rem, x);
+    mac_rows<NT>(acc, x, coef + ((size_t)i * ostride + jbase) / 4);
+  }
+
+  uint32_t bad = 0;
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    if (uint32_t(j) < nt) {
+      uint8_t *o = const_cast<uint8_t *>(ptr_at(k + jbase + j)) + off;
+      if (MODE == kWrite) {
+        store_lane(o, full, rem, acc[j]);
+      } else {
+        uint32_t y[4];
+        load_lane(o, full, rem, y);
+        if (MODE == kVerify) {
+#pragma unroll
+          for (int w = 0; w < 4; ++w) bad |= y[w] ^ acc[j][w];
+        } else {
+#pragma unroll
+          for (int w = 0; w < 4; ++w) y[w] ^= acc[j][w];
+          store_lane(o, full, rem, y);
+        }
+      }
+    }
+  }
+  if (MODE == kVerify && bad) *flag = 1u;
+}
+
+template <int NT, uint32_t MODE>
+__global__ __launch_bounds__(kBlockThreads) void k_gf8_apply(const Gf8Call a) {
+  const uint64_t off = (uint64_t(blockIdx.x) * kBlockThreads + threadIdx.x) * kLaneBytes;
+  if (off >= a.len) return;
+  gf8_tile<NT, MODE>([&](uint32_t idx) { return a.ptr[idx]; }, a.coef, a.len, a.k, a.nout,
+                     a.ostride, a.vec_ok != 0, blockIdx.y * NT, off, a.flag);
+}
+
+// Batched work list: tiles are (job, 4 KiB column chunk, 16-row output tile),
+// numbered job by job; a fixed grid of workgroups strides over them.
+__global__ __launch_bounds__(kBlockThreads) void k_batch8(const BatchJob *jobs,
+                                                         const uint64_t *tile_begin,
+                                                         uint32_t njobs, uint64_t total) {
+  for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
+    uint32_t lo = 0, hi = njobs;  // tile_begin[lo] <= t < tile_begin[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (tile_begin[mid] <= t)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const BatchJob &J = jobs[lo];
+    const uint64_t local = t - J.tile_begin;
+    const uint64_t chunk = local / J.out_tiles;
+    const uint32_t ot = uint32_t(local - chunk * J.out_tiles);
+    const uint64_t off = chunk * kChunkBytes + uint64_t(threadIdx.x) * kLaneBytes;
+    if (off < J.len) {
+      const uint8_t *const *ptrs = J.ptrs;
+      gf8_tile<kOutStride, kWrite>([&](uint32_t idx) { return ptrs[idx]; }, J.coef, J.len, J.k,
+                                   J.nout, J.ostride, J.vec_ok != 0, ot * kOutStride, off,
+                                   nullptr);
+    }
+  }
+}
+
+// Tower apply over split planes with the Karatsuba triple
+// [c_lo, c_hi^c_lo, c_hi*L] (src/fft16.rs:176-180): per input symbol
+// (a_hi, a_lo), t0 = c_lo*a_lo, t1 = (c_hi^c_lo)*(a_hi^a_lo), t2 = (c_hi*L)*a_hi,
+// out_lo ^= t0^t2, out_hi ^= t0^t1 (src/gf/tower_x86.rs:80-93).  The three
+// product sums are accumulated separately and combined at the end.
+template <int NT, uint32_t MODE>
+__global__ __launch_bounds__(kBlockThreads) void k_gf16_apply(const Gf16Call a) {
+  const uint64_t off = (uint64_t(blockIdx.x) * kBlockThreads + threadIdx.x) * kLaneBytes;
+  if (off >= a.plane) return;
+  const uint64_t rem = a.plane - off;
+  const bool full = a.vec_ok && rem >= 16;
+  const uint32_t jbase = blockIdx.y * NT;
+  const uint32_t nt = min(uint32_t(NT), a.nout - jbase);
+  const size_t csz = (size_t)a.k * a.ostride / 4;
+  uint32_t A[NT][4], B[NT][4], C[NT][4];
+#pragma unroll
+  for (int j = 0; j < NT; ++j)
+#pragma unroll
+    for (int w = 0; w < 4; ++w) A[j][w] = B[j][w] = C[j][w] = 0;
+
+  for (uint32_t i = 0; i < a.k; ++i) {
+    const uint8_t *s = a.ptrs[i];
+    uint32_t lo[4], hi[4], sum[4];
+    load_lane(s + off, full, rem, lo);
+    load_lane(s + a.plane + off, full, rem, hi);
+    const uint32_t *cw = a.coef + ((size_t)i * a.ostride + jbase) / 4;
+    mac_rows<NT>(A, lo, cw);
+    mac_rows<NT>(C, hi, cw + 2 * csz);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) sum[w] = lo[w] ^ hi[w];
+    mac_rows<NT>(B, sum, cw + csz);
+  }
+
+  uint32_t bad = 0;
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    if (uint32_t(j) < nt) {
+      uint8_t *o = const_cast<uint8_t *>(a.ptrs[a.k + jbase + j]);
+      uint32_t olo[4], ohi[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        olo[w] = A[j][w] ^ C[j][w];
+        ohi[w] = A[j][w] ^ B[j][w];
+      }
+      if (MODE == kWrite) {
+        store_lane(o + off, full, rem, olo);
+        store_lane(o + a.plane + off, full, rem, ohi);
+      } else {
+        uint32_t ylo[4], yhi[4];
+        load_lane(o + off, full, rem, ylo);
+        load_lane(o + a.plane + off, full, rem, yhi);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) bad |= (ylo[w] ^ olo[w]) | (yhi[w] ^ ohi[w]);
+      }
+    }
+  }
+  if (MODE == kVerify && bad) *a.flag = 1u;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kBlockThreads) void k_fill(uint8_t *dst, uint64_t len,
+                                                       uint64_t seed) {
+  const uint64_t words = (len + 7) / 8;
+  const bool aligned = (reinterpret_cast<uintptr_t>(dst) & 7) == 0;
+  for (uint64_t w = uint64_t(blockIdx.x) * kBlockThreads + threadIdx.x; w < words;
+       w += uint64_t(gridDim.x) * kBlockThreads) {
+    const uint64_t v = mix64(seed + (w + 1) * 0x9E3779B97F4A7C15ull);
+    const uint64_t base = w * 8;
+    if (aligned && base + 8 <= len) {
+      *reinterpret_cast<uint64_t *>(dst + base) = v;
+    } else {
+      for (int b = 0; b < 8; ++b)
+        if (base + b < len) dst[base + b] = uint8_t(v >> (8 * b));
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_gf8_apply(const Gf8Call &call, uint32_t mode, hipStream_t stream) {
+  if (call.len == 0 || call.nout == 0) return hipSuccess;
+  const uint64_t lanes = (call.len + kLaneBytes - 1) / kLaneBytes;
+  const uint32_t gx = uint32_t((lanes + kBlockThreads - 1) / kBlockThreads);
+  const int nt = call.nout <= 4 ? 4 : call.nout <= 8 ? 8 : 16;
+  const dim3 grid(gx, (call.nout + nt - 1) / nt);
+  const dim3 block(kBlockThreads);
+#define TRS_GF8(NT, MODE) hipLaunchKernelGGL((k_gf8_apply<NT, MODE>), grid, block, 0, stream, call)
+  if (mode == kWrite) {
+    if (nt == 4) TRS_GF8(4, kWrite);
+    else if (nt == 8) TRS_GF8(8, kWrite);
+    else TRS_GF8(16, kWrite);
+  } else if (mode == kVerify) {
+    if (nt == 4) TRS_GF8(4, kVerify);
+    else if (nt == 8) TRS_GF8(8, kVerify);
+    else TRS_GF8(16, kVerify);
+  } else {
+    if (nt == 4) TRS_GF8(4, kXorInto);
+    else if (nt == 8) TRS_GF8(8, kXorInto);
+    else TRS_GF8(16, kXorInto);
+  }
+#undef TRS_GF8
+  return hipGetLastError();
+}
+
+hipError_t launch_gf16_apply(const Gf16Call &call, hipStream_t stream) {
+  if (call.plane == 0 || call.nout == 0) return hipSuccess;
+  const uint64_t lanes = (call.plane + kLaneBytes - 1) / kLaneBytes;
+  const uint32_t gx = uint32_t((lanes + kBlockThreads - 1) / kBlockThreads);
+  const int nt = call.nout <= 4 ? 4 : 8;
+  const dim3 grid(gx, (call.nout + nt - 1) / nt);
+  const dim3 block(kBlockThreads);
+#define TRS_GF16(NT, MODE) hipLaunchKernelGGL((k_gf16_apply<NT, MODE>), grid, block, 0, stream, call)
+  if (call.mode == kVerify) {
+    if (nt == 4) TRS_GF16(4, kVerify);
+    else TRS_GF16(8, kVerify);
+  } else {
+    if (nt == 4) TRS_GF16(4, kWrite);
+    else TRS_GF16(8, kWrite);
+  }
+#undef TRS_GF16
+  return hipGetLastError();
+}
+
+hipError_t launch_batch8(const BatchJob *jobs, const uint64_t *tile_begin, uint32_t njobs,
+                         uint64_t total_tiles, uint32_t grid, hipStream_t stream) {
+  if (total_tiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch8, dim3(grid), dim3(kBlockThreads), 0, stream, jobs, tile_begin,
+                     njobs, total_tiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(uint8_t *dst, uint64_t len, uint64_t seed, hipStream_t stream) {
+  if (len == 0) return hipSuccess;
+  const uint64_t words = (len + 7) / 8;
+  uint64_t blocks = (words + kBlockThreads - 1) / kBlockThreads;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_fill, dim3(uint32_t(blocks)), dim3(kBlockThreads), 0, stream, dst, len,
+                     seed);
+  return hipGetLastError();
+}
+
+}  // namespace trs

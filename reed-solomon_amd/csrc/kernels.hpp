@@ -1,0 +1,73 @@
+// Launch descriptors shared by the host codec (codec.cpp) and the HIP
+// kernels (kernels.hip).  Plain structs only: the host fills them, the
+// launchers pass them to the kernels by value.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include <hip/hip_runtime_api.h>
+
+namespace trs {
+
+// Pointers carried inline in the kernel arguments of one GF(2^8) apply.
+// RS8 stripes have at most 256 shards, so inputs + outputs always fit.
+constexpr int kInlinePtrs = 256;
+
+// Output rows per workgroup tile; coefficient rows are padded to this.
+constexpr int kOutStride = 16;
+
+// Bytes of every shard handled by one lane of one workgroup.
+constexpr int kLaneBytes = 16;
+constexpr int kBlockThreads = 256;
+constexpr uint64_t kChunkBytes = uint64_t(kLaneBytes) * kBlockThreads;  // 4 KiB
+
+enum ApplyMode : uint32_t {
+  kWrite = 0,   // out = M * in
+  kVerify = 1,  // flag |= (out != M * in)
+  kXorInto = 2  // out ^= M * in
+};
+
+// One GF(2^8) matrix apply: nout outputs from k inputs over `len` bytes.
+// coef is the transposed, padded coefficient matrix in device memory:
+// byte (i * ostride + j) is row j's coefficient for input i.
+struct Gf8Call {
+  const uint8_t *ptr[kInlinePtrs];  // [0,k) inputs, [k,k+nout) outputs
+  const uint32_t *coef;
+  uint64_t len;
+  uint32_t k, nout, ostride, vec_ok;
+  uint32_t *flag;  // verify mismatch flag (kVerify only)
+};
+
+// One tower (GF((2^8)^2)) matrix apply over split-plane shards.  ptrs is a
+// device array [k inputs, nout outputs]; every shard is `plane` low bytes
+// then `plane` high bytes.  coef holds three transposed padded byte matrices
+// (Karatsuba products c_lo, c_hi^c_lo, c_hi*L), each k * ostride bytes.
+struct Gf16Call {
+  const uint8_t *const *ptrs;
+  const uint32_t *coef;
+  uint64_t plane;
+  uint32_t k, nout, ostride, vec_ok;
+  uint32_t mode;
+  uint32_t *flag;
+};
+
+// One entry of a batched GF(2^8) work list (device memory).
+struct BatchJob {
+  const uint8_t *const *ptrs;  // device array [k inputs, nout outputs]
+  const uint32_t *coef;        // transposed padded coefficients (as Gf8Call)
+  uint64_t len;
+  uint64_t tile_begin;         // first global tile of this job
+  uint32_t k, nout, ostride, vec_ok;
+  uint32_t out_tiles;          // ceil(nout / kOutStride)
+  uint32_t pad;
+};
+
+// Host-side launchers (kernels.hip).  All return hipError_t of the launch.
+hipError_t launch_gf8_apply(const Gf8Call &call, uint32_t mode, hipStream_t stream);
+hipError_t launch_gf16_apply(const Gf16Call &call, hipStream_t stream);
+hipError_t launch_batch8(const BatchJob *jobs, const uint64_t *tile_begin, uint32_t njobs,
+                         uint64_t total_tiles, uint32_t grid, hipStream_t stream);
+hipError_t launch_fill(uint8_t *dst, uint64_t len, uint64_t seed, hipStream_t stream);
+
+}  // namespace trs

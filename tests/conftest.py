@@ -14,3 +14,27 @@ for p in (ROOT, os.path.join(ROOT, "oracle")):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (ROCm device) to run")
+
+
+def _ensure_built():
+    lib = os.path.join(ROOT, "reed-solomon_amd", "libtrs_gpu.so")
+    orc = os.path.join(ROOT, "oracle", "liboracle.so")
+    import subprocess
+
+    if not os.path.exists(lib):
+        subprocess.run(["make", "-j8", "-C", os.path.join(ROOT, "reed-solomon_amd", "csrc")],
+                       check=True)
+    if not os.path.exists(orc):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True)
+
+
+_ensure_built()
+
+import pytest  # noqa: E402
+
+
+@pytest.fixture(scope="session")
+def trs():
+    import __graft_entry__
+
+    return __graft_entry__.load()

@@ -1,0 +1,292 @@
+"""GPU parity for ReedSolomon (GF(2^8)) through the C ABI, against the oracle.
+
+Mirrors the reference's codec tests (src/reedsolomon.rs:917-1207) with the
+CPU oracle as the expected side, plus the error order of the boundary and the
+full-size (10,4) x 256 x 64 KiB workload (round trip + oracle parity)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from util import deterministic_data, rows_to_dev, seeded, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+LENGTHS = [1, 15, 16, 17, 31, 32, 63, 100, 1024, 10000, 65536]
+
+
+def encoded(trs, k, m, length, seed=0x1234_5678):
+    rs = trs.ReedSolomon(k, m)
+    data = deterministic_data(k * length, seed).reshape(k, length) if length * k <= 200_000 \
+        else seeded(k, length, seed)
+    shards = rows_to_dev(np.concatenate([data, np.zeros((m, length), np.uint8)]))
+    rs.encode(shards)
+    return rs, shards, data
+
+
+@pytest.mark.parametrize("k,m", [(7, 13), (10, 10), (20, 10), (4, 2), (6, 4), (17, 17), (10, 4)])
+def test_encode_matches_oracle(trs, k, m):
+    # src/reedsolomon.rs:944-970 (routes agree with encode_scalar across lengths)
+    G = O.build_matrix(8, k, k + m)
+    assert (trs.ReedSolomon(k, m).generator() == G).all()
+    for length in LENGTHS:
+        rs, shards, data = encoded(trs, k, m, length, 0x9E37_79B9)
+        want = O.rs8_encode(G, k, data)
+        assert (to_host(shards[k:]) == want).all(), (k, m, length)
+        assert (to_host(shards[:k]) == data).all()
+
+
+def test_wide_shapes_and_output_tiles(trs):
+    # k + m up to 256; more than 16 outputs exercises the output-tile grid
+    for k, m in [(1, 1), (3, 40), (128, 128), (255, 1), (1, 255)]:
+        G = O.build_matrix(8, k, k + m)
+        for length in (1, 33, 4096 + 5):
+            data = seeded(k, length, 77 + k)
+            rs = trs.ReedSolomon(k, m)
+            shards = rows_to_dev(np.concatenate([data, np.zeros((m, length), np.uint8)]))
+            rs.encode(shards)
+            assert (to_host(shards[k:]) == O.rs8_encode(G, k, data)).all(), (k, m, length)
+
+
+def test_encode_rows_matches_per_shard_encode(trs):
+    # src/reedsolomon.rs:917-942
+    k, m, row_len = 6, 4, 293
+    rs, shards, data = encoded(trs, k, m, row_len)
+    expected = np.concatenate(to_host(shards[k:]))
+    flat = to_dev(data.reshape(-1))
+    assert (rs.encode_rows(flat, row_len).cpu().numpy() == expected).all()
+    buf = to_dev(np.zeros(m * row_len, np.uint8))
+    rs.encode_rows_into(flat, row_len, buf)
+    assert (buf.cpu().numpy() == expected).all()
+    with pytest.raises(trs.Error) as e:
+        rs.encode_rows_into(flat, row_len + 1, buf)
+    assert e.value.name == "IncorrectShardSize"
+
+
+def test_encode_sep_matches_encode(trs):
+    # src/reedsolomon.rs:974-993
+    for k, m in [(7, 13), (10, 10), (32, 32), (4, 2), (18, 6)]:
+        for length in (16, 100, 987, 1024):
+            rs, shards, data = encoded(trs, k, m, length)
+            parity = rows_to_dev(np.zeros((m, length), np.uint8))
+            rs.encode_sep(rows_to_dev(data), parity)
+            assert (to_host(parity) == to_host(shards[k:])).all()
+
+
+def test_encode_fused_alias(trs):
+    rs, shards, data = encoded(trs, 10, 10, 1000)
+    again = rows_to_dev(np.concatenate([data, np.zeros((10, 1000), np.uint8)]))
+    rs.encode_fused(again)
+    assert (to_host(again) == to_host(shards)).all()
+
+
+def _erase_and_view(shards, erased):
+    view = []
+    for i, s in enumerate(shards):
+        if i in erased:
+            s.zero_()
+        view.append((s, i not in erased))
+    return view
+
+
+@pytest.mark.parametrize("k,m", [(4, 2), (7, 13), (10, 10), (6, 4), (10, 4), (32, 32)])
+@pytest.mark.parametrize("length", [15, 100, 1024])
+def test_reconstruct_patterns(trs, k, m, length):
+    # src/reedsolomon.rs:1048-1084, each pattern twice (second hits the plan cache)
+    rs, shards, data = encoded(trs, k, m, length)
+    original = to_host(shards)
+    patterns = [[0], [k - 1], [k], list(range(min(m, k))), list(range(k, k + m)),
+                [(e * 2) % (k + m) for e in range(m)]]
+    for erased in patterns:
+        for _ in range(2):
+            cur = rows_to_dev(original)
+            rs.reconstruct(_erase_and_view(cur, set(erased)))
+            assert (to_host(cur) == original).all(), (k, m, length, erased)
+
+
+def test_round_trip_reconstruct(trs):
+    # src/reedsolomon.rs:1023-1045
+    rs = trs.ReedSolomon(4, 4)
+    data = np.stack([np.full(100, j + 1, np.uint8) for j in range(4)])
+    shards = rows_to_dev(np.concatenate([data, np.zeros((4, 100), np.uint8)]))
+    rs.encode(shards)
+    original = to_host(shards)
+    rs.reconstruct(_erase_and_view(shards, {0, 2, 5}))
+    assert (to_host(shards) == original).all()
+
+
+def test_option_shards_are_allocated(trs):
+    # Option<T> containers: None is allocated zeroed and filled (src/reedsolomon.rs:79-102)
+    k, m, length = 5, 3, 300
+    rs, shards, data = encoded(trs, k, m, length)
+    original = to_host(shards)
+    opt = list(shards)
+    opt[1] = None
+    opt[6] = None
+    rs.reconstruct(opt)
+    assert all(t is not None for t in opt)
+    assert (to_host(opt) == original).all()
+    opt = list(rows_to_dev(original))
+    opt[0] = None
+    opt[7] = None
+    rs.reconstruct_data(opt)
+    assert (opt[0].cpu().numpy() == original[0]).all() and opt[7] is None
+
+
+def test_prepared_decoder(trs):
+    # src/reedsolomon.rs:1087-1129
+    k, m, length = 7, 13, 200
+    rs, shards, data = encoded(trs, k, m, length)
+    original = to_host(shards)
+    erased = {1, 3, 8, 15}
+    dec = rs.prepare_decode([i not in erased for i in range(k + m)])
+    for _ in range(3):
+        cur = rows_to_dev(original)
+        dec.reconstruct(_erase_and_view(cur, erased))
+        assert (to_host(cur) == original).all()
+    cur = rows_to_dev(original)
+    with pytest.raises(trs.Error) as e:
+        dec.reconstruct(_erase_and_view(cur, {0}))
+    assert e.value.name == "InvalidShardFlags"
+
+
+def test_prepared_data_only(trs):
+    # src/reedsolomon.rs:1132-1160
+    k, m, length = 6, 4, 128
+    rs, shards, data = encoded(trs, k, m, length)
+    original = to_host(shards)
+    erased = {2, k + 1}
+    dec = rs.prepare_decode([i not in erased for i in range(k + m)])
+    cur = rows_to_dev(original)
+    dec.reconstruct_data(_erase_and_view(cur, erased))
+    got = to_host(cur)
+    assert (got[2] == original[2]).all()
+    assert (got[k + 1] == 0).all(), "missing parity must stay untouched"
+
+
+def test_reconstruct_rows_roundtrip(trs):
+    # src/reedsolomon.rs:1163-1185 (row_len 257: unaligned rows take the byte path)
+    k, m, row_len = 10, 10, 257
+    rs, shards, data = encoded(trs, k, m, row_len)
+    pristine = to_host(shards).reshape(-1)
+    rows = pristine.copy()
+    erased = [0, 4, 12, 19]
+    present = [True] * (k + m)
+    for e in erased:
+        present[e] = False
+        rows[e * row_len:(e + 1) * row_len] = 0
+    drows = to_dev(rows)
+    rs.reconstruct_rows(drows, row_len, present)
+    assert (drows.cpu().numpy() == pristine).all()
+    with pytest.raises(trs.Error) as e:
+        rs.reconstruct_rows(drows, row_len, present[:-1])
+    assert e.value.name == "InvalidShardFlags"
+    with pytest.raises(trs.Error) as e:
+        rs.reconstruct_rows(drows, 0, present)
+    assert e.value.name == "EmptyShard"
+    with pytest.raises(trs.Error) as e:
+        rs.reconstruct_rows(drows, row_len - 1, present)
+    assert e.value.name == "IncorrectShardSize"
+
+
+def test_verify_blocks(trs):
+    # src/reedsolomon.rs:1188-1207
+    k, m = 4, 3
+    for length in (100, 32 * 1024, 32 * 1024 + 17):
+        rs, shards, data = encoded(trs, k, m, length)
+        assert rs.verify(shards)
+        host = to_host(shards)
+        for pos in (0, length // 2, length - 1):
+            bad = host.copy()
+            bad[k, pos] ^= 1
+            assert not rs.verify(rows_to_dev(bad)), (length, pos)
+
+
+def test_error_order(trs):
+    # src/reedsolomon.rs:701-722 (encode) and :615-641, :801-821 (reconstruct)
+    k, m, length = 4, 2, 64
+    rs, shards, data = encoded(trs, k, m, length)
+
+    def err(fn, *a):
+        with pytest.raises(trs.Error) as e:
+            fn(*a)
+        return e.value.name
+
+    assert err(rs.encode, shards[:-1]) == "TooFewShards"
+    assert err(rs.encode, shards + [shards[0]]) == "TooManyShards"
+    empty = [to_dev(np.zeros(0, np.uint8))] + shards[1:]
+    assert err(rs.encode, empty) == "EmptyShard"
+    ragged = shards[:-1] + [to_dev(np.zeros(length + 1, np.uint8))]
+    assert err(rs.encode, ragged) == "IncorrectShardSize"
+    assert err(rs.verify, ragged) == "IncorrectShardSize"
+    assert err(rs.encode_sep, shards[:k - 1], shards[k:]) == "TooFewDataShards"
+    assert err(rs.encode_sep, shards[:k] + [shards[0]], shards[k:]) == "TooManyDataShards"
+    assert err(rs.encode_sep, shards[:k], shards[k:k + 1]) == "TooFewParityShards"
+    assert err(rs.encode_sep, shards[:k], shards[k:] + [shards[0]]) == "TooManyParityShards"
+    assert err(rs.reconstruct, [(s, True) for s in shards[:-1]]) == "TooFewShards"
+    few = [(s, i < k - 1) for i, s in enumerate(shards)]
+    assert err(rs.reconstruct, few) == "TooFewShardsPresent"
+    wrong_missing = [(s, i != 0) for i, s in enumerate(shards)]
+    wrong_missing[0] = (to_dev(np.zeros(length - 1, np.uint8)), False)
+    assert err(rs.reconstruct, wrong_missing) == "IncorrectShardSize"
+    assert err(rs.prepare_decode, [True] * (k + m - 1)) == "InvalidShardFlags"
+    assert err(rs.prepare_decode, [True] * (k - 1) + [False] * (m + 1)) == "TooFewShardsPresent"
+    # all present: a no-op
+    rs.reconstruct([(s, True) for s in shards])
+
+
+def test_mul_slice_matches_scalar(trs):
+    # src/gf/scalar.rs:7-23 via src/gf/mod.rs:69-85
+    for length in (1, 7, 16, 4097, 100_003):
+        inp = O.fill(length, length)
+        base = O.fill(length + 1, length)
+        for c in (0, 1, 2, 0x1d, 0x8e, 255):
+            out = to_dev(base)
+            trs.mul_slice(out, to_dev(inp), c)
+            want = base.copy()
+            O.mul_slice(want, inp, c, False)
+            assert (out.cpu().numpy() == want).all()
+            out = to_dev(base)
+            trs.mul_slice_xor(out, to_dev(inp), c)
+            want = base.copy()
+            O.mul_slice(want, inp, c, True)
+            assert (out.cpu().numpy() == want).all()
+    with pytest.raises(AssertionError):
+        trs.mul_slice(to_dev(np.zeros(3, np.uint8)), to_dev(np.zeros(4, np.uint8)), 3)
+
+
+def test_fill_random_matches_oracle_stream(trs):
+    import torch
+
+    for length in (1, 8, 13, 4096, 100_001):
+        t = torch.empty(length, dtype=torch.uint8, device="cuda")
+        trs.fill_random(t, 0x7A9E5EED00000001 + length)
+        assert (t.cpu().numpy() == O.fill(0x7A9E5EED00000001 + length, length)).all()
+
+
+def test_full_size_10_4_workload(trs):
+    """256 stripes x (10,4) x 64 KiB, seeded: parity == oracle for every stripe,
+    then 4 uniformly chosen erasures per stripe reconstruct to the original."""
+    import torch
+
+    k, m, L, S = 10, 4, 65536, 256
+    rs = trs.ReedSolomon(k, m)
+    G = O.build_matrix(8, k, k + m)
+    buf = torch.empty(S * (k + m) * L, dtype=torch.uint8, device="cuda")
+    rng = np.random.default_rng(0x7A9E5EED00000000)
+    for s in range(S):
+        stripe = buf[s * (k + m) * L:(s + 1) * (k + m) * L]
+        trs.fill_random(stripe[:k * L], 0x7A9E5EED00000000 + s)
+        rs.encode([stripe[i * L:(i + 1) * L] for i in range(k + m)])
+    host = buf.cpu().numpy().reshape(S, k + m, L)
+    for s in range(S):
+        data = O.fill(0x7A9E5EED00000000 + s, k * L).reshape(k, L)
+        assert (host[s, :k] == data).all()
+        assert (host[s, k:] == O.rs8_encode(G, k, data)).all(), s
+    original = buf.clone()
+    for s in range(S):
+        erased = set(rng.choice(k + m, 4, replace=False).tolist())
+        stripe = buf[s * (k + m) * L:(s + 1) * (k + m) * L]
+        shards = [stripe[i * L:(i + 1) * L] for i in range(k + m)]
+        rs.reconstruct(_erase_and_view(shards, erased))
+    torch.cuda.synchronize()
+    assert torch.equal(buf, original)
