@@ -14,6 +14,7 @@ output, reference_kats.json, is committed; tests never read the reference.
 """
 from __future__ import annotations
 
+import hashlib
 import json
 import os
 import re
@@ -80,8 +81,15 @@ def extract(root: str) -> dict:
     for k, m, regs, ops, mults in re.findall(
         r"// shape \((\d+),(\d+)\): (\d+) registers, (\d+) ops, (\d+) multiplies", progs
     ):
+        # The op list of the shape's macro, one "op a b [c]" line per token
+        # group, hashed: the reference's builder output, byte for byte.
+        body = progs[progs.index(f"macro_rules! fft_program_{k}_{m} "):]
+        body = body[: body.index("pub(crate) use")]
+        lines = [" ".join(t.split()) for t in re.findall(r"^\s+((?:ld|mx|xo|mc|cp|st)[^;]*);",
+                                                         body, re.M)]
+        digest = hashlib.sha256("\n".join(lines).encode()).hexdigest()
         shapes.append({"k": int(k), "m": int(m), "registers": int(regs), "ops": int(ops),
-                       "multiplies": int(mults)})
+                       "multiplies": int(mults), "op_lines": len(lines), "op_sha256": digest})
     out["fft_generated_programs"] = {"source": "src/fft_programs.rs:19,133,254,451,622,787",
                                      "shapes": shapes}
     return out
