@@ -90,6 +90,13 @@ int trs_rs8_encode(const trs_rs8 *rs, uint8_t *const *shards, const size_t *lens
 int trs_rs8_encode_sep(const trs_rs8 *rs, const uint8_t *const *data, const size_t *data_lens,
                        size_t n_data, uint8_t *const *parity, const size_t *parity_lens,
                        size_t n_parity, void *stream);
+/* Encode through a named route (0 = fused table kernel, 1 = bit-sliced
+ * constant matrix, 2 = bit-sliced LCH FFT); the bench / test hook that
+ * ReedSolomon::encode_forced is in the reference (src/reedsolomon.rs:365-398).
+ * TRS_ERR_UNSUPPORTED when the shape has no program for that route. */
+int trs_rs8_encode_sep_route(const trs_rs8 *rs, int route, const uint8_t *const *data,
+                             const size_t *data_lens, size_t n_data, uint8_t *const *parity,
+                             const size_t *parity_lens, size_t n_parity, void *stream);
 /* ReedSolomon::encode_rows_into, src/reedsolomon.rs:667-682: data holds k
  * rows of row_len bytes back to back, parity m rows. */
 int trs_rs8_encode_rows_into(const trs_rs8 *rs, const uint8_t *data, size_t data_len,

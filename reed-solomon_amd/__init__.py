@@ -307,6 +307,17 @@ class ReedSolomon(_Codec):
     def encode_route(self, shard_len: int) -> str:
         return lib.trs_rs8_encode_route(self._h, shard_len).decode()
 
+    ROUTES = {"table": 0, "bitsliced-matrix": 1, "bitsliced-fft": 2}
+
+    def encode_sep_route(self, route: str, data: Sequence, parity: Sequence) -> None:
+        """Encode through a named device route (the reference's encode_forced
+        bench hook, src/reedsolomon.rs:365-398).  Raises Error(Device(-4)) when
+        the shape has no program for that route."""
+        dp, dl = _ptrs(data)
+        pp, pl = _ptrs(parity)
+        _check(lib.trs_rs8_encode_sep_route(self._h, self.ROUTES[route], dp, dl, len(data), pp,
+                                            pl, len(parity), _stream()))
+
     def encode_fused(self, shards: Sequence) -> None:
         """Alias of encode (src/reedsolomon.rs:354-359)."""
         self.encode(shards)

@@ -9,6 +9,12 @@ from __future__ import annotations
 import ctypes
 import os
 
+# torch ships its own libamdhip64 (same SONAME, libamdhip64.so.7).  Loading
+# torch first makes libtrs_gpu.so bind to that already-loaded HIP runtime;
+# loading ours first would put two HIP runtimes in the process and torch would
+# then see no GPU.
+import torch  # noqa: F401
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtrs_gpu.so")
 HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "trs_gpu.h")
@@ -58,6 +64,8 @@ for fam, elem in (("rs8", c_u8p), ("rs16", c_u16p)):
 
 _sig("trs_rs8_encode_route", ctypes.c_char_p, c_vp, c_size)
 _sig("trs_rs16_encode_route", ctypes.c_char_p, c_vp)
+_sig("trs_rs8_encode_sep_route", c_int, c_vp, c_int, c_ptrs, c_sizep, c_size, c_ptrs, c_sizep,
+     c_size, c_vp)
 _sig("trs_rs8_encode_rows_into", c_int, c_vp, c_vp, c_size, c_size, c_vp, c_size, c_vp)
 _sig("trs_rs16_encode_sep_matrix", c_int, c_vp, c_ptrs, c_sizep, c_size, c_ptrs, c_sizep, c_size,
      c_vp)

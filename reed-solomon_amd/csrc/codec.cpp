@@ -296,6 +296,7 @@ using namespace trs;
 
 struct trs_rs8 {
   size_t k = 0, m = 0, n = 0;
+  int route = kRouteTable;  // encode route: bit-sliced FFT > bit-sliced matrix > table kernel
   Matrix<F8> G;
   DevCoef parity;
   PlanCache plans;
@@ -526,6 +527,10 @@ int trs_rs8_new(size_t k, size_t m, trs_rs8 **out) {
   if (!build_generator<F8>(k, k + m, rs->G)) return TRS_SINGULAR_MATRIX;
   const int st = upload_coef8(rs->parity, rs->G.row(k), m, k);
   if (st) return st;
+  if (bs_encode_available(kRouteBsFft, uint32_t(k), uint32_t(m)))
+    rs->route = kRouteBsFft;
+  else if (bs_encode_available(kRouteBsMatrix, uint32_t(k), uint32_t(m)))
+    rs->route = kRouteBsMatrix;
   *out = rs.release();
   return TRS_OK;
 }
@@ -536,9 +541,13 @@ size_t trs_rs8_parity_shard_count(const trs_rs8 *rs) { return rs ? rs->m : 0; }
 size_t trs_rs8_total_shard_count(const trs_rs8 *rs) { return rs ? rs->n : 0; }
 
 const char *trs_rs8_encode_route(const trs_rs8 *rs, size_t shard_len) {
-  (void)rs;
   (void)shard_len;
-  return "gpu-fused-matrix";
+  if (!rs) return "none";
+  switch (rs->route) {
+    case kRouteBsFft: return "gpu-bitsliced-fft";
+    case kRouteBsMatrix: return "gpu-bitsliced-matrix";
+    default: return "gpu-fused-matrix";
+  }
 }
 
 int trs_rs8_generator(const trs_rs8 *rs, uint8_t *out, size_t out_len) {
@@ -560,10 +569,23 @@ int check_all8(size_t total, const size_t *lens, size_t n) {
 }
 
 int encode8_ptrs(const trs_rs8 *rs, std::vector<const uint8_t *> &ptrs, size_t len,
-                 hipStream_t s) {
+                 hipStream_t s, int route) {
   for (auto q : ptrs)
     if (!q) return TRS_ERR_INVALID_ARGUMENT;
-  return run_gf8(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, s);
+  if (route == kRouteTable || len == 0)
+    return run_gf8(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, s);
+  if (!bs_encode_available(route, uint32_t(rs->k), uint32_t(rs->m))) return TRS_ERR_UNSUPPORTED;
+  Gf8Call c{};
+  bool al = true;
+  for (size_t i = 0; i < ptrs.size(); ++i) {
+    c.ptr[i] = ptrs[i];
+    al = al && aligned16(ptrs[i]);
+  }
+  c.len = len;
+  c.k = uint32_t(rs->k);
+  c.nout = uint32_t(rs->m);
+  c.vec_ok = al ? 1u : 0u;
+  return hip_status(launch_bs_encode(route, c, s));
 }
 }  // namespace
 
@@ -574,7 +596,7 @@ int trs_rs8_encode(const trs_rs8 *rs, uint8_t *const *shards, const size_t *lens
   int st = check_all8(rs->n, lens, n);
   if (st) return st;
   std::vector<const uint8_t *> ptrs(shards, shards + n);
-  return encode8_ptrs(rs, ptrs, lens[0], as_stream(stream));
+  return encode8_ptrs(rs, ptrs, lens[0], as_stream(stream), rs->route);
 }
 
 int trs_rs8_encode_sep(const trs_rs8 *rs, const uint8_t *const *data, const size_t *dl,
@@ -592,7 +614,25 @@ int trs_rs8_encode_sep(const trs_rs8 *rs, const uint8_t *const *data, const size
   if (len == 0) return TRS_OK;
   std::vector<const uint8_t *> ptrs(data, data + nd);
   ptrs.insert(ptrs.end(), parity, parity + np);
-  return encode8_ptrs(rs, ptrs, len, as_stream(stream));
+  return encode8_ptrs(rs, ptrs, len, as_stream(stream), rs->route);
+}
+
+int trs_rs8_encode_sep_route(const trs_rs8 *rs, int route, const uint8_t *const *data,
+                             const size_t *dl, size_t nd, uint8_t *const *parity, const size_t *pl,
+                             size_t np, void *stream) {
+  if (!rs || (nd && (!data || !dl)) || (np && (!parity || !pl))) return TRS_ERR_INVALID_ARGUMENT;
+  if (route < kRouteTable || route > kRouteBsFft) return TRS_ERR_INVALID_ARGUMENT;
+  if (nd != rs->k) return nd < rs->k ? TRS_TOO_FEW_DATA_SHARDS : TRS_TOO_MANY_DATA_SHARDS;
+  if (np != rs->m) return np < rs->m ? TRS_TOO_FEW_PARITY_SHARDS : TRS_TOO_MANY_PARITY_SHARDS;
+  const size_t len = dl[0];
+  for (size_t i = 0; i < nd; ++i)
+    if (dl[i] != len) return TRS_INCORRECT_SHARD_SIZE;
+  for (size_t i = 0; i < np; ++i)
+    if (pl[i] != len) return TRS_INCORRECT_SHARD_SIZE;
+  if (len == 0) return TRS_OK;
+  std::vector<const uint8_t *> ptrs(data, data + nd);
+  ptrs.insert(ptrs.end(), parity, parity + np);
+  return encode8_ptrs(rs, ptrs, len, as_stream(stream), route);
 }
 
 int trs_rs8_encode_rows_into(const trs_rs8 *rs, const uint8_t *data, size_t data_len,
@@ -606,7 +646,10 @@ int trs_rs8_encode_rows_into(const trs_rs8 *rs, const uint8_t *data, size_t data
   std::vector<const uint8_t *> ptrs;
   for (size_t i = 0; i < rs->k; ++i) ptrs.push_back(data + i * row_len);
   for (size_t i = 0; i < rs->m; ++i) ptrs.push_back(parity + i * row_len);
-  return encode8_ptrs(rs, ptrs, row_len, as_stream(stream));
+  // The reference's rows API always takes its matrix route
+  // (src/reedsolomon.rs:684-697); every route here is byte-identical, so the
+  // rows take the codec's fastest one.
+  return encode8_ptrs(rs, ptrs, row_len, as_stream(stream), rs->route);
 }
 
 int trs_rs8_verify(const trs_rs8 *rs, const uint8_t *const *shards, const size_t *lens, size_t n,

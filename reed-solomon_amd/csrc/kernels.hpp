@@ -70,4 +70,10 @@ hipError_t launch_batch8(const BatchJob *jobs, const uint64_t *tile_begin, uint3
                          uint64_t total_tiles, uint32_t grid, hipStream_t stream);
 hipError_t launch_fill(uint8_t *dst, uint64_t len, uint64_t seed, hipStream_t stream);
 
+// Bit-sliced encode with compile-time coefficients (gen_bitsliced.hip,
+// generated by gf8_codegen.py).  route: kRouteBsMatrix or kRouteBsFft.
+enum EncodeRoute : int { kRouteTable = 0, kRouteBsMatrix = 1, kRouteBsFft = 2 };
+hipError_t launch_bs_encode(int route, const Gf8Call &call, hipStream_t stream);
+bool bs_encode_available(int route, uint32_t k, uint32_t m);
+
 }  // namespace trs

@@ -1,0 +1,212 @@
+"""Generates the bit-sliced GF(2^8) encode kernels (build step).
+
+Bit-sliced form: a lane loads 32 bytes of a shard (two 16-byte loads) and
+transposes them into 8 planes, plane b holding bit b of all 32 bytes.
+Multiplying by a constant c is then the 8x8 GF(2) matrix of "x -> c*x"
+(the same matrix GFNI's affine instruction takes, src/gf/gfni.rs:41-64):
+output plane r is the XOR of the input planes b whose product c*2^b has bit r
+set.  With c known at compile time every multiply becomes a short list of
+plane XORs (3-input XORs on gfx950), and field additions are plane XORs.
+
+Two programs per shape are emitted, both byte-identical to the generator
+matrix encode:
+  * fft  -- the LCH additive-FFT op list from fftgen.build_encode_program
+            (the reference's generated programs, src/fft_programs.rs);
+  * mat  -- the systematic generator applied row by row with constant
+            coefficients (src/reedsolomon.rs:154-184).
+
+Usage: python gf8_codegen.py OUT.hip
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import fftgen  # noqa: E402
+
+MUL = fftgen.MUL
+
+# Shapes with generated kernels: the six generated FFT shapes and (10,4).
+SHAPES = [(7, 13), (10, 10), (14, 14), (16, 16), (18, 6), (32, 32), (10, 4)]
+FFT_SHAPES = set(fftgen.GENERATED_SHAPES)
+# Constant-matrix programs keep all k input planes live across every output;
+# past ~20 inputs they exceed the register file, so only these get one.
+MAT_SHAPES = {(7, 13), (10, 10), (18, 6), (10, 4)}
+
+
+def bit_rows(c: int):
+    """rows[r] = input planes b feeding output plane r of x -> c*x."""
+    return [[b for b in range(8) if (MUL[c][1 << b] >> r) & 1] for r in range(8)]
+
+
+def _gf8_generator(k: int, n: int):
+    """Systematic generator V(n,k) * inv(V_top) over GF(2^8) (src/reedsolomon.rs:154-159)."""
+    inv8 = [0] * 256
+    for a in range(1, 256):
+        for b in range(1, 256):
+            if MUL[a][b] == 1:
+                inv8[a] = b
+                break
+    V = [[0] * k for _ in range(n)]
+    for r in range(n):
+        p = 1
+        for c in range(k):
+            V[r][c] = p
+            p = MUL[p][r]
+    # invert top k x k
+    A = [V[r][:] + [1 if i == r else 0 for i in range(k)] for r in range(k)]
+    for c in range(k):
+        piv = next(r for r in range(c, k) if A[r][c])
+        A[c], A[piv] = A[piv], A[c]
+        s = inv8[A[c][c]]
+        A[c] = [MUL[s][x] for x in A[c]]
+        for r in range(k):
+            if r != c and A[r][c]:
+                f = A[r][c]
+                A[r] = [x ^ MUL[f][y] for x, y in zip(A[r], A[c])]
+    inv = [row[k:] for row in A]
+    G = [[0] * k for _ in range(n)]
+    for r in range(n):
+        for c in range(k):
+            acc = 0
+            for i in range(k):
+                acc ^= MUL[V[r][i]][inv[i][c]]
+            G[r][c] = acc
+    return G
+
+
+def matrix_program(k: int, m: int):
+    """Row-by-row constant-coefficient program: ops plus interleaved stores."""
+    G = _gf8_generator(k, k + m)
+    ops = []
+    acc = k
+    for j in range(m):
+        first = True
+        for i in range(k):
+            c = G[k + j][i]
+            if c == 0:
+                continue
+            ops.append(("mc" if first else "mx", acc, i, c))
+            first = False
+        if first:
+            ops.append(("zero", acc))
+        ops.append(("st", j, acc))
+    return ops, k + 1
+
+
+def fft_program(k: int, m: int):
+    p = fftgen.build_encode_program(k, m)
+    ops = list(p.ops) + [("st", j, r) for j, r in enumerate(p.parity_regs)]
+    return ops, p.register_count
+
+
+def emit_body(k: int, ops, nreg: int) -> str:
+    out = []
+    out.append(f"  uint32_t R[{nreg}][8];")
+    for s in range(k):
+        out.append(f"  load_planes(in[{s}] + off, full, rem, R[{s}]);")
+    for op in ops:
+        kind = op[0]
+        if kind == "st":
+            _, j, r = op
+            out.append(f"  store_planes(out[{j}] + off, full, rem, R[{r}]);")
+        elif kind == "zero":
+            out.append(f"  for (int r = 0; r < 8; ++r) R[{op[1]}][r] = 0;")
+        elif kind in ("xo", "cp"):
+            _, d, s = op
+            sym = "^=" if kind == "xo" else "="
+            for r in range(8):
+                out.append(f"  R[{d}][{r}] {sym} R[{s}][{r}];")
+        else:
+            _, d, s, c = op
+            rows = bit_rows(c)
+            for r in range(8):
+                terms = " ^ ".join(f"R[{s}][{b}]" for b in rows[r])
+                if kind == "mx":
+                    if terms:
+                        out.append(f"  R[{d}][{r}] ^= {terms};")
+                else:
+                    out.append(f"  R[{d}][{r}] = {terms if terms else '0u'};")
+    return "\n".join(out)
+
+
+def xor_cost(ops) -> int:
+    """VALU estimate per 32-byte lane chunk: 3-input XORs, excluding transposes."""
+    n = 0
+    for op in ops:
+        if op[0] in ("xo",):
+            n += 8
+        elif op[0] in ("mx", "mc"):
+            for row in bit_rows(op[3]):
+                t = len(row) + (1 if op[0] == "mx" else 0)
+                n += max(0, (t - 1 + 1) // 2)
+    return n
+
+
+def generate() -> str:
+    parts = [
+        "// GENERATED by reed-solomon_amd/gf8_codegen.py -- do not edit.",
+        "// Bit-sliced GF(2^8) encode programs with compile-time coefficients.",
+        '#include "bitslice.hpp"',
+        "",
+        "namespace trs {",
+        "namespace {",
+    ]
+    table = []
+    for (k, m) in SHAPES:
+        routes = []
+        if (k, m) in MAT_SHAPES:
+            routes.append(("mat", matrix_program(k, m)))
+        if (k, m) in FFT_SHAPES:
+            routes.append(("fft", fft_program(k, m)))
+        for route, (ops, nreg) in routes:
+            name = f"bs_{route}_{k}_{m}"
+            parts.append(f"// ({k},{m}) {route}: {nreg} registers, ~{xor_cost(ops)} plane XORs "
+                         "per 32-byte lane chunk")
+            parts.append(f"__device__ __forceinline__ void {name}(const uint8_t *const *in, "
+                         "uint8_t *const *out, uint64_t off, bool full, uint64_t rem) {")
+            parts.append(emit_body(k, ops, nreg))
+            parts.append("}")
+            parts.append(f"__global__ __launch_bounds__(kBlockThreads) void k_{name}(const Gf8Call a) {{")
+            parts.append("  const uint64_t off = (uint64_t(blockIdx.x) * kBlockThreads + threadIdx.x) * 32;")
+            parts.append("  if (off >= a.len) return;")
+            parts.append("  const uint64_t rem = a.len - off;")
+            parts.append(f"  {name}(a.ptr, const_cast<uint8_t *const *>(a.ptr + {k}), off, "
+                         "a.vec_ok && rem >= 32, rem);")
+            parts.append("}")
+            parts.append("")
+            table.append((route, k, m, name))
+    parts.append("}  // namespace")
+    parts.append("")
+    parts.append("// route: 1 = matrix, 2 = fft.  Returns hipErrorNotSupported for shapes")
+    parts.append("// without a generated program.")
+    parts.append("hipError_t launch_bs_encode(int route, const Gf8Call &a, hipStream_t stream) {")
+    parts.append("  if (a.len == 0) return hipSuccess;")
+    parts.append("  const uint64_t lanes = (a.len + 31) / 32;")
+    parts.append("  const dim3 grid(uint32_t((lanes + kBlockThreads - 1) / kBlockThreads));")
+    for route, k, m, name in table:
+        rid = 1 if route == "mat" else 2
+        parts.append(f"  if (route == {rid} && a.k == {k} && a.nout == {m}) {{")
+        parts.append(f"    hipLaunchKernelGGL(k_{name}, grid, dim3(kBlockThreads), 0, stream, a);")
+        parts.append("    return hipGetLastError();")
+        parts.append("  }")
+    parts.append("  return hipErrorNotSupported;")
+    parts.append("}")
+    parts.append("")
+    parts.append("bool bs_encode_available(int route, uint32_t k, uint32_t m) {")
+    for route, k, m, name in table:
+        rid = 1 if route == "mat" else 2
+        parts.append(f"  if (route == {rid} && k == {k} && m == {m}) return true;")
+    parts.append("  return false;")
+    parts.append("}")
+    parts.append("")
+    parts.append("}  // namespace trs")
+    return "\n".join(parts) + "\n"
+
+
+if __name__ == "__main__":
+    text = generate()
+    with open(sys.argv[1], "w", encoding="utf-8") as f:
+        f.write(text)
