@@ -181,8 +181,11 @@ class _Codec:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            getattr(lib, f"trs_{self._fam}_free")(h)
+        if h and lib is not None:
+            try:
+                getattr(lib, f"trs_{self._fam}_free")(h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     def data_shard_count(self) -> int:
@@ -261,8 +264,11 @@ class _Decoder:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            getattr(lib, f"trs_{self._fam}_free")(h)
+        if h and lib is not None:
+            try:
+                getattr(lib, f"trs_{self._fam}_free")(h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     def _reconstruct(self, shards: list, data_only: bool) -> None:
@@ -383,8 +389,11 @@ class Batch:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            lib.trs_batch8_free(h)
+        if h and lib is not None:
+            try:
+                lib.trs_batch8_free(h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     def clear(self) -> None:

@@ -958,14 +958,20 @@ struct trs_batch8 {
     std::vector<const uint8_t *> ptrs;
     uint64_t len = 0;
     uint32_t k = 0, nout = 0;
+    uint32_t prog = 0;  // generated bit-sliced program (encode of generated shapes), 0 = table
+  };
+  // One device work list per kernel: table-kernel jobs and generated-program jobs.
+  struct List {
+    DevBuf jobs, tiles;
+    uint32_t njobs = 0;
+    uint64_t total_tiles = 0;
+    uint32_t grid = 0;
   };
   std::vector<Req> reqs;
-  DevBuf jobs, tiles, ptrs;
-  uint64_t total_tiles = 0;
-  uint32_t grid = 0;
+  DevBuf ptrs;
+  List table, bs;
   bool prepared = false;
 };
-
 extern "C" {
 
 int trs_batch8_new(trs_batch8 **out) {
@@ -980,7 +986,8 @@ void trs_batch8_clear(trs_batch8 *b) {
   if (!b) return;
   b->reqs.clear();
   b->prepared = false;
-  b->total_tiles = 0;
+  b->table.njobs = b->bs.njobs = 0;
+  b->table.total_tiles = b->bs.total_tiles = 0;
 }
 
 size_t trs_batch8_size(const trs_batch8 *b) { return b ? b->reqs.size() : 0; }
@@ -1006,6 +1013,7 @@ int trs_batch8_add_encode(trs_batch8 *b, const trs_rs8 *rs, uint8_t *const *shar
   r.len = lens[0];
   r.k = uint32_t(rs->k);
   r.nout = uint32_t(rs->m);
+  if (rs->route != kRouteTable) r.prog = bs_program_id(rs->route, r.k, r.nout);
   b->reqs.push_back(std::move(r));
   b->prepared = false;
   return TRS_OK;
@@ -1043,17 +1051,14 @@ int trs_batch8_prepare(trs_batch8 *b) {
   size_t nptr = 0;
   for (auto &r : b->reqs) nptr += r.ptrs.size();
   TRS_TRY(b->ptrs.ensure(nptr * sizeof(void *) + 8));
-  TRS_TRY(b->jobs.ensure(b->reqs.size() * sizeof(BatchJob) + 8));
-  TRS_TRY(b->tiles.ensure((b->reqs.size() + 1) * sizeof(uint64_t)));
   std::vector<const uint8_t *> hp;
   hp.reserve(nptr);
-  std::vector<BatchJob> jobs(b->reqs.size());
-  std::vector<uint64_t> tb(b->reqs.size() + 1, 0);
+  std::vector<BatchJob> jt, jb;
+  std::vector<uint64_t> tt, tb;
+  uint64_t tile_t = 0, tile_b = 0;
   const auto *dptr = static_cast<const uint8_t *const *>(b->ptrs.p);
-  uint64_t tile = 0;
-  for (size_t j = 0; j < b->reqs.size(); ++j) {
-    const auto &r = b->reqs[j];
-    BatchJob &J = jobs[j];
+  for (const auto &r : b->reqs) {
+    BatchJob J{};
     J.ptrs = dptr + hp.size();
     bool al = true;
     for (auto q : r.ptrs) {
@@ -1067,26 +1072,47 @@ int trs_batch8_prepare(trs_batch8 *b) {
     J.ostride = r.coef->ostride;
     J.vec_ok = al ? 1u : 0u;
     J.out_tiles = (r.nout + kOutStride - 1) / kOutStride;
-    J.tile_begin = tile;
-    tb[j] = tile;
-    tile += ((r.len + kChunkBytes - 1) / kChunkBytes) * J.out_tiles;
+    J.prog = r.prog;
+    if (r.prog) {
+      J.tile_begin = tile_b;
+      tb.push_back(tile_b);
+      tile_b += (r.len + kBsChunkBytes - 1) / kBsChunkBytes;
+      jb.push_back(J);
+    } else {
+      J.tile_begin = tile_t;
+      tt.push_back(tile_t);
+      tile_t += ((r.len + kChunkBytes - 1) / kChunkBytes) * J.out_tiles;
+      jt.push_back(J);
+    }
   }
-  tb[b->reqs.size()] = tile;
-  b->total_tiles = tile;
   if (!hp.empty())
     TRS_TRY(hipMemcpy(b->ptrs.p, hp.data(), hp.size() * sizeof(void *), hipMemcpyHostToDevice));
-  if (!jobs.empty())
-    TRS_TRY(hipMemcpy(b->jobs.p, jobs.data(), jobs.size() * sizeof(BatchJob),
-                      hipMemcpyHostToDevice));
-  TRS_TRY(hipMemcpy(b->tiles.p, tb.data(), tb.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  uint64_t grid = uint64_t(cus) * 8;
-  if (const char *e = std::getenv("TRS_BATCH_GRID_PER_CU")) grid = uint64_t(cus) * std::atoi(e);
-  if (grid > tile) grid = tile;
-  if (grid == 0) grid = 1;
-  b->grid = uint32_t(grid);
+  auto upload = [&](trs_batch8::List &L, std::vector<BatchJob> &jobs, std::vector<uint64_t> &tiles,
+                    uint64_t total, uint64_t per_cu) -> int {
+    tiles.push_back(total);
+    TRS_TRY(L.jobs.ensure(jobs.size() * sizeof(BatchJob) + 8));
+    TRS_TRY(L.tiles.ensure(tiles.size() * sizeof(uint64_t)));
+    if (!jobs.empty())
+      TRS_TRY(hipMemcpy(L.jobs.p, jobs.data(), jobs.size() * sizeof(BatchJob),
+                        hipMemcpyHostToDevice));
+    TRS_TRY(hipMemcpy(L.tiles.p, tiles.data(), tiles.size() * sizeof(uint64_t),
+                      hipMemcpyHostToDevice));
+    L.njobs = uint32_t(jobs.size());
+    L.total_tiles = total;
+    uint64_t grid = uint64_t(cus) * per_cu;
+    if (grid > total) grid = total;
+    L.grid = uint32_t(grid ? grid : 1);
+    return TRS_OK;
+  };
+  uint64_t per_cu_table = 8, per_cu_bs = 4;
+  if (const char *e = std::getenv("TRS_BATCH_GRID_PER_CU")) per_cu_table = std::atoi(e);
+  if (const char *e = std::getenv("TRS_BATCH_BS_GRID_PER_CU")) per_cu_bs = std::atoi(e);
+  int st = upload(b->table, jt, tt, tile_t, per_cu_table);
+  if (st) return st;
+  if ((st = upload(b->bs, jb, tb, tile_b, per_cu_bs))) return st;
   b->prepared = true;
   return TRS_OK;
 }
@@ -1097,11 +1123,18 @@ int trs_batch8_run(trs_batch8 *b, void *stream) {
     int st = trs_batch8_prepare(b);
     if (st) return st;
   }
-  if (b->total_tiles == 0) return TRS_OK;
-  return hip_status(launch_batch8(static_cast<const BatchJob *>(b->jobs.p),
-                                  static_cast<const uint64_t *>(b->tiles.p),
-                                  uint32_t(b->reqs.size()), b->total_tiles, b->grid,
-                                  as_stream(stream)));
+  hipStream_t s = as_stream(stream);
+  if (b->bs.total_tiles) {
+    const int st = hip_status(launch_batch_bs(static_cast<const BatchJob *>(b->bs.jobs.p),
+                                              static_cast<const uint64_t *>(b->bs.tiles.p),
+                                              b->bs.njobs, b->bs.total_tiles, b->bs.grid, s));
+    if (st) return st;
+  }
+  if (b->table.total_tiles)
+    return hip_status(launch_batch8(static_cast<const BatchJob *>(b->table.jobs.p),
+                                    static_cast<const uint64_t *>(b->table.tiles.p),
+                                    b->table.njobs, b->table.total_tiles, b->table.grid, s));
+  return TRS_OK;
 }
 
 }  // extern "C"

@@ -60,7 +60,7 @@ struct BatchJob {
   uint64_t tile_begin;         // first global tile of this job
   uint32_t k, nout, ostride, vec_ok;
   uint32_t out_tiles;          // ceil(nout / kOutStride)
-  uint32_t pad;
+  uint32_t prog;               // generated bit-sliced program id (k_batch_bs), 0 = none
 };
 
 // Host-side launchers (kernels.hip).  All return hipError_t of the launch.
@@ -75,5 +75,10 @@ hipError_t launch_fill(uint8_t *dst, uint64_t len, uint64_t seed, hipStream_t st
 enum EncodeRoute : int { kRouteTable = 0, kRouteBsMatrix = 1, kRouteBsFft = 2 };
 hipError_t launch_bs_encode(int route, const Gf8Call &call, hipStream_t stream);
 bool bs_encode_available(int route, uint32_t k, uint32_t m);
+uint32_t bs_program_id(int route, uint32_t k, uint32_t m);
+// Persistent work-list kernel over jobs with a generated program (8 KiB tiles).
+constexpr uint64_t kBsChunkBytes = 8192;
+hipError_t launch_batch_bs(const BatchJob *jobs, const uint64_t *tile_begin, uint32_t njobs,
+                           uint64_t total_tiles, uint32_t grid, hipStream_t stream);
 
 }  // namespace trs

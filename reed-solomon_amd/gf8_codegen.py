@@ -178,7 +178,48 @@ def generate() -> str:
             parts.append("}")
             parts.append("")
             table.append((route, k, m, name))
+    # Persistent work-list kernel over jobs that name a generated program:
+    # tiles are (job, 8 KiB column chunk), numbered job by job.
+    parts.append("__global__ __launch_bounds__(kBlockThreads) void k_batch_bs(const BatchJob *jobs,")
+    parts.append("    const uint64_t *tile_begin, uint32_t njobs, uint64_t total) {")
+    parts.append("  for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {")
+    parts.append("    uint32_t lo = 0, hi = njobs;")
+    parts.append("    while (hi - lo > 1) {")
+    parts.append("      const uint32_t mid = (lo + hi) >> 1;")
+    parts.append("      if (tile_begin[mid] <= t) lo = mid; else hi = mid;")
+    parts.append("    }")
+    parts.append("    const BatchJob &J = jobs[lo];")
+    parts.append("    const uint64_t off = (t - J.tile_begin) * 8192 + uint64_t(threadIdx.x) * 32;")
+    parts.append("    if (off >= J.len) continue;")
+    parts.append("    const uint64_t rem = J.len - off;")
+    parts.append("    const bool full = J.vec_ok && rem >= 32;")
+    parts.append("    const uint8_t *const *in = J.ptrs;")
+    parts.append("    uint8_t *const *out = const_cast<uint8_t *const *>(J.ptrs + J.k);")
+    parts.append("    switch (J.prog) {")
+    for pid, (route, k, m, name) in enumerate(table, start=1):
+        parts.append(f"      case {pid}: {name}(in, out, off, full, rem); break;")
+    parts.append("      default: break;")
+    parts.append("    }")
+    parts.append("  }")
+    parts.append("}")
+    parts.append("")
     parts.append("}  // namespace")
+    parts.append("")
+    parts.append("// Program id of a generated (route, k, m) program for BatchJob::prog, 0 if none.")
+    parts.append("uint32_t bs_program_id(int route, uint32_t k, uint32_t m) {")
+    for pid, (route, k, m, name) in enumerate(table, start=1):
+        rid = 1 if route == "mat" else 2
+        parts.append(f"  if (route == {rid} && k == {k} && m == {m}) return {pid};")
+    parts.append("  return 0;")
+    parts.append("}")
+    parts.append("")
+    parts.append("hipError_t launch_batch_bs(const BatchJob *jobs, const uint64_t *tile_begin, "
+                 "uint32_t njobs, uint64_t total_tiles, uint32_t grid, hipStream_t stream) {")
+    parts.append("  if (total_tiles == 0) return hipSuccess;")
+    parts.append("  hipLaunchKernelGGL(k_batch_bs, dim3(grid), dim3(kBlockThreads), 0, stream, jobs,")
+    parts.append("                     tile_begin, njobs, total_tiles);")
+    parts.append("  return hipGetLastError();")
+    parts.append("}")
     parts.append("")
     parts.append("// route: 1 = matrix, 2 = fft.  Returns hipErrorNotSupported for shapes")
     parts.append("// without a generated program.")
