@@ -324,6 +324,13 @@ class ReedSolomon(_Codec):
         _check(lib.trs_rs8_encode_sep_route(self._h, self.ROUTES[route], dp, dl, len(data), pp,
                                             pl, len(parity), _stream()))
 
+    APPLY_ROUTES = {"auto": 0, "table": 1, "mfma": 2}
+
+    def set_apply_route(self, route: str) -> None:
+        """Device route for reconstruct applies: "auto", "table" or "mfma"
+        (int8 MFMA over the bit-expanded plan, k <= 32).  All byte-identical."""
+        _check(lib.trs_rs8_set_apply_route(self._h, self.APPLY_ROUTES[route]))
+
     def encode_fused(self, shards: Sequence) -> None:
         """Alias of encode (src/reedsolomon.rs:354-359)."""
         self.encode(shards)

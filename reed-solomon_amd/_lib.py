@@ -66,6 +66,7 @@ _sig("trs_rs8_encode_route", ctypes.c_char_p, c_vp, c_size)
 _sig("trs_rs16_encode_route", ctypes.c_char_p, c_vp)
 _sig("trs_rs8_encode_sep_route", c_int, c_vp, c_int, c_ptrs, c_sizep, c_size, c_ptrs, c_sizep,
      c_size, c_vp)
+_sig("trs_rs8_set_apply_route", c_int, c_vp, c_int)
 _sig("trs_rs8_encode_rows_into", c_int, c_vp, c_vp, c_size, c_size, c_vp, c_size, c_vp)
 _sig("trs_rs16_encode_sep_matrix", c_int, c_vp, c_ptrs, c_sizep, c_size, c_ptrs, c_sizep, c_size,
      c_vp)

@@ -156,6 +156,36 @@ int upload_coef16(DevCoef &dc, const uint16_t *rows, size_t nout, size_t k) {
   return TRS_OK;
 }
 
+// A fragments of the int8-MFMA apply (mfma.hip) for an nout x k GF(2^8)
+// matrix: [pass][K step][M tile][lane][16 bytes].  Lane l, element e of M tile
+// mt holds A[row][kk] with row = l%32 and kk = 16*(l/32) + e, where row maps to
+// output bit (o, r) through the accumulator layout (register v of lane half h
+// holds row 8*(v/4) + 4*h + v%4; registers 0..7 / 8..15 are the 8 bits of
+// outputs 2h / 2h+1 of the tile) and kk to input bit (i, b) = (4s + kk/8, kk%8).
+std::vector<uint8_t> build_mfma_frags(const uint8_t *rows, size_t nout, size_t k) {
+  const size_t ks = mfma_steps(uint32_t(k));
+  const size_t passes = (nout + 7) / 8;
+  std::vector<uint8_t> f(passes * ks * 2 * 64 * 16, 0);
+  const Gf8 &g = gf8();
+  for (size_t p = 0; p < passes; ++p)
+    for (size_t s = 0; s < ks; ++s)
+      for (size_t mt = 0; mt < 2; ++mt)
+        for (size_t l = 0; l < 64; ++l) {
+          const size_t row = l % 32;
+          const size_t half = (row / 4) % 2;
+          const size_t v = 4 * (row / 8) + row % 4;
+          const size_t o = p * 8 + mt * 4 + 2 * half + v / 8;
+          const size_t r = v % 8;
+          uint8_t *dst = &f[(((p * ks + s) * 2 + mt) * 64 + l) * 16];
+          for (size_t e = 0; e < 16; ++e) {
+            const size_t in = 4 * s + 2 * (l / 32) + e / 8;
+            const size_t b = e % 8;
+            if (o < nout && in < k) dst[e] = (g.mul[rows[o * k + in]][1u << b] >> r) & 1u;
+          }
+        }
+  return f;
+}
+
 using Mask = std::vector<uint64_t>;
 
 Mask make_mask(const uint8_t *present, size_t n) {
@@ -175,7 +205,16 @@ struct Plan {
   std::vector<uint32_t> missing_parity;  // ascending
   DevCoef coef;                          // [missing data rows; missing parity rows] x k
   std::vector<uint8_t> rows8;            // host copy (GF(2^8) plans)
+  DevBuf frag;                           // MFMA A fragments of the same rows (k <= 32)
 };
+
+int upload_frags(DevBuf &dst, const uint8_t *rows, size_t nout, size_t k) {
+  if (!mfma_available(uint32_t(k)) || nout == 0) return TRS_OK;
+  const auto f = build_mfma_frags(rows, nout, k);
+  TRS_TRY(dst.alloc(f.size()));
+  TRS_TRY(hipMemcpy(dst.p, f.data(), f.size(), hipMemcpyHostToDevice));
+  return TRS_OK;
+}
 
 constexpr size_t kPlanCacheLimit = 64;  // src/reedsolomon.rs:24
 
@@ -297,6 +336,7 @@ using namespace trs;
 struct trs_rs8 {
   size_t k = 0, m = 0, n = 0;
   int route = kRouteTable;  // encode route: bit-sliced FFT > bit-sliced matrix > table kernel
+  std::atomic<int> apply_route{0};  // reconstruct apply: 0 auto, 1 table kernel, 2 int8 MFMA
   Matrix<F8> G;
   DevCoef parity;
   PlanCache plans;
@@ -304,6 +344,7 @@ struct trs_rs8 {
 
 struct trs_dec8 {
   size_t k = 0, n = 0;
+  int apply_route = 0;  // the codec's apply route when the decoder was prepared
   std::shared_ptr<Plan> plan;
 };
 
@@ -331,9 +372,10 @@ int plan8_for(const trs_rs8 *rs, const Mask &m, std::shared_ptr<Plan> *out) {
   p->presence = m;
   split_indices(m, rs->k, rs->n, *p);
   if (!fused_rows(rs->G, rs->k, *p, p->rows8)) return TRS_SINGULAR_MATRIX;
-  const int st = upload_coef8(p->coef, p->rows8.data(),
-                              p->missing_data.size() + p->missing_parity.size(), rs->k);
+  const size_t nrows = p->missing_data.size() + p->missing_parity.size();
+  int st = upload_coef8(p->coef, p->rows8.data(), nrows, rs->k);
   if (st) return st;
+  if ((st = upload_frags(p->frag, p->rows8.data(), nrows, rs->k))) return st;
   *out = rs->plans.insert(m, p);
   return TRS_OK;
 }
@@ -672,9 +714,36 @@ int trs_rs8_verify(const trs_rs8 *rs, const uint8_t *const *shards, const size_t
 namespace {
 // reconstruct_internal / PreparedDecoder::reconstruct_internal
 // (src/reedsolomon.rs:615-641, :771-796).  `plan` non-null = prepared.
+// Route of a reconstruct apply.  Measured on MI355X the int8-MFMA kernel
+// does not yet beat the table kernel at any C2 shape (it is VALU-bound on
+// bit expansion and packing), so "auto" keeps the table kernel and MFMA runs
+// only when asked for.
+bool use_mfma(int route, size_t k, size_t nout) {
+  (void)nout;
+  return route == 2 && mfma_available(uint32_t(k));
+}
+
+int run_mfma(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout, const void *frag,
+             size_t len, hipStream_t stream) {
+  if (nout == 0 || len == 0) return TRS_OK;
+  if (k + nout > size_t(kInlinePtrs)) return TRS_ERR_INVALID_ARGUMENT;
+  Gf8MfmaCall c{};
+  bool al = true;
+  for (size_t i = 0; i < k + nout; ++i) {
+    c.ptr[i] = ptrs[i];
+    al = al && (reinterpret_cast<uintptr_t>(ptrs[i]) & 3) == 0;
+  }
+  c.frag = frag;
+  c.len = len;
+  c.k = uint32_t(k);
+  c.nout = uint32_t(nout);
+  c.vec_ok = al ? 1u : 0u;
+  return hip_status(launch_gf8_mfma(c, stream));
+}
+
 int reconstruct8(const trs_rs8 *rs, size_t k, size_t total, std::shared_ptr<Plan> plan,
                  uint8_t *const *shards, const size_t *lens, const uint8_t *present, size_t n,
-                 int data_only, hipStream_t s) {
+                 int data_only, hipStream_t s, int apply_route) {
   if (n && (!shards || !lens || !present)) return TRS_ERR_INVALID_ARGUMENT;
   if (n != total) return n < total ? TRS_TOO_FEW_SHARDS : TRS_TOO_MANY_SHARDS;
   Mask mask;
@@ -693,15 +762,24 @@ int reconstruct8(const trs_rs8 *rs, size_t k, size_t total, std::shared_ptr<Plan
   std::vector<const uint8_t *> ptrs;
   size_t nout = 0;
   if ((st = gather(*plan, shards, lens, k, len, data_only != 0, ptrs, nout))) return st;
+  if (use_mfma(apply_route, k, nout) && plan->frag.p)
+    return run_mfma(ptrs, k, nout, plan->frag.p, len, s);
   return run_gf8(ptrs, k, nout, plan->coef, len, kWrite, nullptr, s);
 }
 }  // namespace
+
+int trs_rs8_set_apply_route(trs_rs8 *rs, int route) {
+  if (!rs || route < 0 || route > 2) return TRS_ERR_INVALID_ARGUMENT;
+  if (route == 2 && !mfma_available(uint32_t(rs->k))) return TRS_ERR_UNSUPPORTED;
+  rs->apply_route.store(route);
+  return TRS_OK;
+}
 
 int trs_rs8_reconstruct(const trs_rs8 *rs, uint8_t *const *shards, const size_t *lens,
                         const uint8_t *present, size_t n, int data_only, void *stream) {
   if (!rs) return TRS_ERR_INVALID_ARGUMENT;
   return reconstruct8(rs, rs->k, rs->n, nullptr, shards, lens, present, n, data_only,
-                      as_stream(stream));
+                      as_stream(stream), rs->apply_route.load());
 }
 
 int trs_rs8_reconstruct_rows(const trs_rs8 *rs, uint8_t *rows, size_t rows_len, size_t row_len,
@@ -716,7 +794,7 @@ int trs_rs8_reconstruct_rows(const trs_rs8 *rs, uint8_t *rows, size_t rows_len, 
   std::vector<size_t> lens(rs->n, row_len);
   for (size_t i = 0; i < rs->n; ++i) shards[i] = rows + i * row_len;
   return reconstruct8(rs, rs->k, rs->n, nullptr, shards.data(), lens.data(), present, rs->n, 0,
-                      as_stream(stream));
+                      as_stream(stream), rs->apply_route.load());
 }
 
 int trs_rs8_prepare_decode(const trs_rs8 *rs, const uint8_t *present, size_t n, trs_dec8 **out) {
@@ -734,6 +812,7 @@ int trs_rs8_prepare_decode(const trs_rs8 *rs, const uint8_t *present, size_t n, 
   auto d = std::make_unique<trs_dec8>();
   d->k = rs->k;
   d->n = rs->n;
+  d->apply_route = rs->apply_route.load();
   d->plan = plan;
   *out = d.release();
   return TRS_OK;
@@ -743,7 +822,7 @@ int trs_dec8_reconstruct(const trs_dec8 *dec, uint8_t *const *shards, const size
                          const uint8_t *present, size_t n, int data_only, void *stream) {
   if (!dec) return TRS_ERR_INVALID_ARGUMENT;
   return reconstruct8(nullptr, dec->k, dec->n, dec->plan, shards, lens, present, n, data_only,
-                      as_stream(stream));
+                      as_stream(stream), dec->apply_route);
 }
 
 void trs_dec8_free(trs_dec8 *dec) { delete dec; }
@@ -959,6 +1038,7 @@ struct trs_batch8 {
     uint64_t len = 0;
     uint32_t k = 0, nout = 0;
     uint32_t prog = 0;  // generated bit-sliced program (encode of generated shapes), 0 = table
+    const void *frag = nullptr;  // MFMA A fragments: the job runs on the int8-MFMA kernel
   };
   // One device work list per kernel: table-kernel jobs and generated-program jobs.
   struct List {
@@ -969,7 +1049,7 @@ struct trs_batch8 {
   };
   std::vector<Req> reqs;
   DevBuf ptrs;
-  List table, bs;
+  List table, bs, mfma;
   bool prepared = false;
 };
 extern "C" {
@@ -986,8 +1066,8 @@ void trs_batch8_clear(trs_batch8 *b) {
   if (!b) return;
   b->reqs.clear();
   b->prepared = false;
-  b->table.njobs = b->bs.njobs = 0;
-  b->table.total_tiles = b->bs.total_tiles = 0;
+  b->table.njobs = b->bs.njobs = b->mfma.njobs = 0;
+  b->table.total_tiles = b->bs.total_tiles = b->mfma.total_tiles = 0;
 }
 
 size_t trs_batch8_size(const trs_batch8 *b) { return b ? b->reqs.size() : 0; }
@@ -1041,6 +1121,11 @@ int trs_batch8_add_reconstruct(trs_batch8 *b, const trs_rs8 *rs, uint8_t *const 
   r.len = len;
   r.k = uint32_t(rs->k);
   r.nout = uint32_t(nout);
+  if (use_mfma(rs->apply_route.load(), rs->k, nout) && plan->frag.p) {
+    bool al = true;
+    for (auto q : r.ptrs) al = al && (reinterpret_cast<uintptr_t>(q) & 3) == 0;
+    if (al) r.frag = plan->frag.p;
+  }
   b->reqs.push_back(std::move(r));
   b->prepared = false;
   return TRS_OK;
@@ -1053,9 +1138,9 @@ int trs_batch8_prepare(trs_batch8 *b) {
   TRS_TRY(b->ptrs.ensure(nptr * sizeof(void *) + 8));
   std::vector<const uint8_t *> hp;
   hp.reserve(nptr);
-  std::vector<BatchJob> jt, jb;
-  std::vector<uint64_t> tt, tb;
-  uint64_t tile_t = 0, tile_b = 0;
+  std::vector<BatchJob> jt, jb, jm;
+  std::vector<uint64_t> tt, tb, tm;
+  uint64_t tile_t = 0, tile_b = 0, tile_m = 0;
   const auto *dptr = static_cast<const uint8_t *const *>(b->ptrs.p);
   for (const auto &r : b->reqs) {
     BatchJob J{};
@@ -1073,7 +1158,13 @@ int trs_batch8_prepare(trs_batch8 *b) {
     J.vec_ok = al ? 1u : 0u;
     J.out_tiles = (r.nout + kOutStride - 1) / kOutStride;
     J.prog = r.prog;
-    if (r.prog) {
+    if (r.frag) {
+      J.coef = static_cast<const uint32_t *>(r.frag);
+      J.tile_begin = tile_m;
+      tm.push_back(tile_m);
+      tile_m += ((r.len + mfma_group_cols() - 1) / mfma_group_cols()) * ((r.nout + 7) / 8);
+      jm.push_back(J);
+    } else if (r.prog) {
       J.tile_begin = tile_b;
       tb.push_back(tile_b);
       tile_b += (r.len + kBsChunkBytes - 1) / kBsChunkBytes;
@@ -1113,6 +1204,9 @@ int trs_batch8_prepare(trs_batch8 *b) {
   int st = upload(b->table, jt, tt, tile_t, per_cu_table);
   if (st) return st;
   if ((st = upload(b->bs, jb, tb, tile_b, per_cu_bs))) return st;
+  uint64_t per_cu_mfma = 2;
+  if (const char *e = std::getenv("TRS_BATCH_MFMA_GRID_PER_CU")) per_cu_mfma = std::atoi(e);
+  if ((st = upload(b->mfma, jm, tm, tile_m, per_cu_mfma))) return st;
   b->prepared = true;
   return TRS_OK;
 }
@@ -1128,6 +1222,13 @@ int trs_batch8_run(trs_batch8 *b, void *stream) {
     const int st = hip_status(launch_batch_bs(static_cast<const BatchJob *>(b->bs.jobs.p),
                                               static_cast<const uint64_t *>(b->bs.tiles.p),
                                               b->bs.njobs, b->bs.total_tiles, b->bs.grid, s));
+    if (st) return st;
+  }
+  if (b->mfma.total_tiles) {
+    const int st = hip_status(launch_batch_mfma(static_cast<const BatchJob *>(b->mfma.jobs.p),
+                                                static_cast<const uint64_t *>(b->mfma.tiles.p),
+                                                b->mfma.njobs, b->mfma.total_tiles, b->mfma.grid,
+                                                s));
     if (st) return st;
   }
   if (b->table.total_tiles)

@@ -147,20 +147,19 @@ __global__ __launch_bounds__(kBlockThreads) void k_gf8_apply(const Gf8Call a) {
 }
 
 // Batched work list: tiles are (job, 4 KiB column chunk, 16-row output tile),
-// numbered job by job; a fixed grid of workgroups strides over them.
+// numbered job by job.  Workgroup b takes the contiguous tile range
+// [b*T/G, (b+1)*T/G): one binary search for its first job, then it walks the
+// jobs forward as its range crosses them.
 __global__ __launch_bounds__(kBlockThreads) void k_batch8(const BatchJob *jobs,
                                                          const uint64_t *tile_begin,
                                                          uint32_t njobs, uint64_t total) {
-  for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
-    uint32_t lo = 0, hi = njobs;  // tile_begin[lo] <= t < tile_begin[hi]
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (tile_begin[mid] <= t)
-        lo = mid;
-      else
-        hi = mid;
-    }
-    const BatchJob &J = jobs[lo];
+  const uint64_t t0 = total * blockIdx.x / gridDim.x;
+  const uint64_t t1 = total * (blockIdx.x + 1) / gridDim.x;
+  if (t0 >= t1) return;
+  uint32_t j = first_job(tile_begin, njobs, t0);
+  for (uint64_t t = t0; t < t1; ++t) {
+    while (j + 1 < njobs && tile_begin[j + 1] <= t) ++j;
+    const BatchJob &J = jobs[j];
     const uint64_t local = t - J.tile_begin;
     const uint64_t chunk = local / J.out_tiles;
     const uint32_t ot = uint32_t(local - chunk * J.out_tiles);

@@ -52,10 +52,22 @@ struct Gf16Call {
   uint32_t *flag;
 };
 
+// One GF(2^8) apply on int8 MFMA (mfma.hip): frag holds the bit-expanded
+// coefficient matrix as per-lane A fragments, [pass][K step][2 M tiles][64
+// lanes][16 bytes]; pass p covers outputs 8p..8p+7, K step s inputs 4s..4s+3.
+// vec_ok: every pointer 4-byte aligned.
+struct Gf8MfmaCall {
+  const uint8_t *ptr[kInlinePtrs];  // [0,k) inputs, [k,k+nout) outputs
+  const void *frag;
+  uint64_t len;
+  uint32_t k, nout, vec_ok, pad;
+};
+
 // One entry of a batched GF(2^8) work list (device memory).
 struct BatchJob {
   const uint8_t *const *ptrs;  // device array [k inputs, nout outputs]
-  const uint32_t *coef;        // transposed padded coefficients (as Gf8Call)
+  const uint32_t *coef;        // transposed padded coefficients (as Gf8Call); MFMA jobs:
+                               // the A fragments of mfma.hip
   uint64_t len;
   uint64_t tile_begin;         // first global tile of this job
   uint32_t k, nout, ostride, vec_ok;
@@ -63,12 +75,37 @@ struct BatchJob {
   uint32_t prog;               // generated bit-sliced program id (k_batch_bs), 0 = none
 };
 
+#ifdef __HIPCC__
+// Largest job j with tile_begin[j] <= t (tile_begin ascending, tile_begin[0] = 0).
+__device__ __forceinline__ uint32_t first_job(const uint64_t *tile_begin, uint32_t njobs,
+                                              uint64_t t) {
+  uint32_t lo = 0, hi = njobs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (tile_begin[mid] <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+#endif
+
 // Host-side launchers (kernels.hip).  All return hipError_t of the launch.
 hipError_t launch_gf8_apply(const Gf8Call &call, uint32_t mode, hipStream_t stream);
 hipError_t launch_gf16_apply(const Gf16Call &call, hipStream_t stream);
 hipError_t launch_batch8(const BatchJob *jobs, const uint64_t *tile_begin, uint32_t njobs,
                          uint64_t total_tiles, uint32_t grid, hipStream_t stream);
 hipError_t launch_fill(uint8_t *dst, uint64_t len, uint64_t seed, hipStream_t stream);
+
+// int8-MFMA apply (mfma.hip); available for k <= 32.
+hipError_t launch_gf8_mfma(const Gf8MfmaCall &call, hipStream_t stream);
+bool mfma_available(uint32_t k);
+uint32_t mfma_steps(uint32_t k);
+uint64_t mfma_group_cols();  // columns per workgroup tile (4 waves x 512)
+// Persistent work-list kernel over MFMA jobs (BatchJob::coef = A fragments).
+hipError_t launch_batch_mfma(const BatchJob *jobs, const uint64_t *tile_begin, uint32_t njobs,
+                             uint64_t total_tiles, uint32_t grid, hipStream_t stream);
 
 // Bit-sliced encode with compile-time coefficients (gen_bitsliced.hip,
 // generated by gf8_codegen.py).  route: kRouteBsMatrix or kRouteBsFft.

@@ -182,13 +182,13 @@ def generate() -> str:
     # tiles are (job, 8 KiB column chunk), numbered job by job.
     parts.append("__global__ __launch_bounds__(kBlockThreads) void k_batch_bs(const BatchJob *jobs,")
     parts.append("    const uint64_t *tile_begin, uint32_t njobs, uint64_t total) {")
-    parts.append("  for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {")
-    parts.append("    uint32_t lo = 0, hi = njobs;")
-    parts.append("    while (hi - lo > 1) {")
-    parts.append("      const uint32_t mid = (lo + hi) >> 1;")
-    parts.append("      if (tile_begin[mid] <= t) lo = mid; else hi = mid;")
-    parts.append("    }")
-    parts.append("    const BatchJob &J = jobs[lo];")
+    parts.append("  const uint64_t t0 = total * blockIdx.x / gridDim.x;")
+    parts.append("  const uint64_t t1 = total * (blockIdx.x + 1) / gridDim.x;")
+    parts.append("  if (t0 >= t1) return;")
+    parts.append("  uint32_t j = first_job(tile_begin, njobs, t0);")
+    parts.append("  for (uint64_t t = t0; t < t1; ++t) {")
+    parts.append("    while (j + 1 < njobs && tile_begin[j + 1] <= t) ++j;")
+    parts.append("    const BatchJob &J = jobs[j];")
     parts.append("    const uint64_t off = (t - J.tile_begin) * 8192 + uint64_t(threadIdx.x) * 32;")
     parts.append("    if (off >= J.len) continue;")
     parts.append("    const uint64_t rem = J.len - off;")

@@ -12,11 +12,14 @@ pytestmark = pytest.mark.gpu
 SHAPES = [(7, 13), (10, 10), (14, 14), (16, 16), (18, 6), (32, 32), (10, 4)]
 
 
-def test_batch_matches_per_call(trs):
+@pytest.mark.parametrize("route", ["table", "mfma", "auto"])
+def test_batch_matches_per_call(trs, route):
     import torch
 
     rng = np.random.default_rng(11)
     codecs = {s: trs.ReedSolomon(*s) for s in SHAPES}
+    for c in codecs.values():
+        c.set_apply_route(route)
     batch = trs.Batch()
     cases = []
     queued = 0

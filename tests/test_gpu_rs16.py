@@ -159,3 +159,37 @@ def test_odd_length_rejected(trs):
     with pytest.raises(trs.Error) as e:
         rs.encode(shards)
     assert e.value.name == "IncorrectShardSize"
+
+
+def test_c3_shape_1024_1024(trs):
+    """ReedSolomon16 (1024,1024) with 4 KiB split-plane shards (2048 symbols):
+    the generator equals the oracle's, sampled parity rows equal the oracle's
+    symbol-wise tower product, and erasing 1024 uniformly chosen shards per
+    stripe reconstructs the original stripe."""
+    import torch
+
+    k = m = 1024
+    L = 4096
+    rs = trs.ReedSolomon16(k, m)
+    G = O.build_matrix(16, k, k + m)
+    assert (rs.generator() == G).all()
+    rng = np.random.default_rng(0xC3)
+    for stripe in range(2):
+        buf = torch.empty((k + m) * L, dtype=torch.uint8, device="cuda")
+        trs.fill_random(buf[:k * L], 0x7A9E5EED00000003 + stripe)
+        shards = [buf[i * L:(i + 1) * L] for i in range(k + m)]
+        rs.encode(shards)
+        host = buf.cpu().numpy().reshape(k + m, L)
+        rows = [0, 1, 255, 256, 511, 1023]
+        want = O.rs16_symbolwise(G[[k + r for r in rows]], list(host[:k]), L)
+        assert (host[[k + r for r in rows]] == want).all(), stripe
+        original = buf.clone()
+        erased = set(rng.choice(k + m, 1024, replace=False).tolist())
+        view = []
+        for i, s in enumerate(shards):
+            if i in erased:
+                s.zero_()
+            view.append((s, i not in erased))
+        rs.reconstruct(view)
+        torch.cuda.synchronize()
+        assert torch.equal(buf, original), stripe

@@ -311,3 +311,47 @@ def test_every_encode_route_matches_oracle(trs, k, m):
                 continue
             assert (to_host(parity) == want).all(), (k, m, length, route)
     assert rs.encode_route(65536).startswith("gpu-")
+
+
+@pytest.mark.parametrize("route", ["table", "mfma"])
+@pytest.mark.parametrize("k,m", [(10, 4), (7, 13), (10, 10), (32, 32), (17, 5), (3, 2), (29, 9)])
+def test_reconstruct_apply_routes(trs, k, m, route):
+    """The fused table kernel and the int8-MFMA kernel (bit-expanded plan)
+    rebuild every erased shard byte for byte, across tails, lengths that are
+    not multiples of 4 and erasure counts 1..m, plus data-only."""
+    rng = np.random.default_rng(k * 100 + m)
+    G = O.build_matrix(8, k, k + m)
+    rs = trs.ReedSolomon(k, m)
+    rs.set_apply_route(route)
+    for length in (1, 3, 4, 5, 127, 128, 129, 2048, 2049, 70003):
+        data = seeded(k, length, 99 + length)
+        original = np.concatenate([data, O.rs8_encode(G, k, data)])
+        e = int(rng.integers(1, m + 1))
+        erased = set(rng.choice(k + m, e, replace=False).tolist())
+        cur = rows_to_dev(original)
+        rs.reconstruct(_erase_and_view(cur, erased))
+        assert (to_host(cur) == original).all(), (k, m, length, sorted(erased), route)
+        cur = rows_to_dev(original)
+        rs.reconstruct_data(_erase_and_view(cur, erased))
+        got = to_host(cur)
+        for i in range(k + m):
+            want = original[i] if (i < k or i not in erased) else np.zeros(length, np.uint8)
+            assert (got[i] == want).all(), (k, m, length, i, route)
+
+
+def test_unaligned_rows_on_mfma_route(trs):
+    # rows at odd offsets exercise the byte path of the MFMA kernel
+    k, m, row_len = 10, 6, 1001
+    rs = trs.ReedSolomon(k, m)
+    rs.set_apply_route("mfma")
+    G = O.build_matrix(8, k, k + m)
+    data = seeded(k, row_len, 5)
+    pristine = np.concatenate([data, O.rs8_encode(G, k, data)]).reshape(-1)
+    rows = pristine.copy()
+    present = [True] * (k + m)
+    for e in (1, 2, 3, 11, 15):
+        present[e] = False
+        rows[e * row_len:(e + 1) * row_len] = 0
+    drows = to_dev(rows)
+    rs.reconstruct_rows(drows, row_len, present)
+    assert (drows.cpu().numpy() == pristine).all()
