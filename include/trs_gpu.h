@@ -202,6 +202,39 @@ int trs_batch8_run(trs_batch8 *b, void *stream);
 uint64_t trs_batch8_bytes(const trs_batch8 *b);
 size_t trs_batch8_size(const trs_batch8 *b);
 
+/* ================= autotuning (no reference counterpart) ================== */
+/* Times one apply candidate on synthetic device data: `nout` outputs from `k`
+ * inputs of `len` bytes over GF(2^8) (field 8) or the tower (field 16,
+ * split-plane, even len), random nonzero coefficients, through `route`
+ * (0 = table kernel, 1 = generated bit-sliced program -- GF(2^8) shapes with
+ * one only, 2 = int8 MFMA, k <= 32) in column strips of `strip` bytes (0 =
+ * whole shard; one launch per strip).  *ms_out = mean milliseconds of `reps`
+ * timed applies after one warm-up.  TRS_ERR_UNSUPPORTED for a route the shape
+ * has no kernel for.  The reference tunes by hand (STRIP_BYTES,
+ * src/reedsolomon16.rs:796-804; MULT_MARGIN, src/reedsolomon.rs:226-232). */
+int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len, int route, uint64_t strip,
+                   int reps, double *ms_out);
+
+/* FP32 MLP cost model (12 -> 64 -> 64 -> 1, ReLU), trained and evaluated by
+ * hand-written HIP kernels; it predicts log kernel time from shape / route /
+ * strip features so the autotuner measures only the top candidates.  Inputs
+ * are host float arrays: X is n x 12 row-major, T and Y are n.  Training is
+ * full-batch Adam on the mean squared error; gradients are summed in a fixed
+ * order (deterministic).  Parameter layout: W1[64][12], b1[64], W2[64][64],
+ * b2[64], W3[64], b3. */
+typedef struct trs_mlp trs_mlp;
+int trs_mlp_param_count(void);
+int trs_mlp_new(uint64_t seed, trs_mlp **out);
+void trs_mlp_free(trs_mlp *m);
+int trs_mlp_get_params(const trs_mlp *m, float *out, size_t n);
+int trs_mlp_set_params(trs_mlp *m, const float *in, size_t n);
+/* Loss and gradient of the current parameters (for gradient checks). */
+int trs_mlp_gradients(trs_mlp *m, const float *X, const float *T, size_t n, float *grad_out,
+                      float *loss_out);
+int trs_mlp_train(trs_mlp *m, const float *X, const float *T, size_t n, int steps, float lr,
+                  float *loss_out);
+int trs_mlp_predict(const trs_mlp *m, const float *X, size_t n, float *Y);
+
 /* ================= utilities (bench / test helpers) ======================= */
 /* Fill a device buffer with the seeded splitmix64 byte stream: byte j of
  * 8-byte word w is byte j (little-endian) of mix64(seed + (w+1)*0x9E3779B97F4A7C15). */

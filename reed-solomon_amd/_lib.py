@@ -85,6 +85,19 @@ _sig("trs_batch8_bytes", ctypes.c_uint64, c_vp)
 _sig("trs_batch8_size", c_size, c_vp)
 
 
+c_fp = ctypes.POINTER(ctypes.c_float)
+_sig("trs_tune_apply", c_int, c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, c_int,
+     ctypes.c_uint64, c_int, ctypes.POINTER(ctypes.c_double))
+_sig("trs_mlp_param_count", c_int)
+_sig("trs_mlp_new", c_int, ctypes.c_uint64, ctypes.POINTER(c_vp))
+_sig("trs_mlp_free", None, c_vp)
+_sig("trs_mlp_get_params", c_int, c_vp, c_fp, c_size)
+_sig("trs_mlp_set_params", c_int, c_vp, c_fp, c_size)
+_sig("trs_mlp_gradients", c_int, c_vp, c_fp, c_fp, c_size, c_fp, c_fp)
+_sig("trs_mlp_train", c_int, c_vp, c_fp, c_fp, c_size, c_int, ctypes.c_float, c_fp)
+_sig("trs_mlp_predict", c_int, c_vp, c_fp, c_size, c_fp)
+
+
 def declared_symbols() -> list:
     """Every function name include/trs_gpu.h declares."""
     import re

@@ -476,6 +476,8 @@ int run_gf16(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout,
   c.ptrs = static_cast<const uint8_t *const *>(table.d);
   c.coef = coef.ptr();
   c.plane = plane;
+  c.col0 = 0;
+  c.cols = plane;
   c.k = uint32_t(k);
   c.nout = uint32_t(nout);
   c.ostride = coef.ostride;
@@ -1183,13 +1185,18 @@ int trs_batch8_prepare(trs_batch8 *b) {
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   auto upload = [&](trs_batch8::List &L, std::vector<BatchJob> &jobs, std::vector<uint64_t> &tiles,
                     uint64_t total, uint64_t per_cu) -> int {
-    tiles.push_back(total);
+    // tile -> job map, one uint32 per tile
+    std::vector<uint32_t> tj(total ? total : 1, 0);
+    for (size_t j = 0; j < jobs.size(); ++j) {
+      const uint64_t end = j + 1 < jobs.size() ? tiles[j + 1] : total;
+      for (uint64_t t = tiles[j]; t < end; ++t) tj[t] = uint32_t(j);
+    }
     TRS_TRY(L.jobs.ensure(jobs.size() * sizeof(BatchJob) + 8));
-    TRS_TRY(L.tiles.ensure(tiles.size() * sizeof(uint64_t)));
+    TRS_TRY(L.tiles.ensure(tj.size() * sizeof(uint32_t)));
     if (!jobs.empty())
       TRS_TRY(hipMemcpy(L.jobs.p, jobs.data(), jobs.size() * sizeof(BatchJob),
                         hipMemcpyHostToDevice));
-    TRS_TRY(hipMemcpy(L.tiles.p, tiles.data(), tiles.size() * sizeof(uint64_t),
+    TRS_TRY(hipMemcpy(L.tiles.p, tj.data(), tj.size() * sizeof(uint32_t),
                       hipMemcpyHostToDevice));
     L.njobs = uint32_t(jobs.size());
     L.total_tiles = total;
@@ -1220,22 +1227,157 @@ int trs_batch8_run(trs_batch8 *b, void *stream) {
   hipStream_t s = as_stream(stream);
   if (b->bs.total_tiles) {
     const int st = hip_status(launch_batch_bs(static_cast<const BatchJob *>(b->bs.jobs.p),
-                                              static_cast<const uint64_t *>(b->bs.tiles.p),
-                                              b->bs.njobs, b->bs.total_tiles, b->bs.grid, s));
+                                              static_cast<const uint32_t *>(b->bs.tiles.p),
+                                              b->bs.total_tiles, b->bs.grid, s));
     if (st) return st;
   }
   if (b->mfma.total_tiles) {
     const int st = hip_status(launch_batch_mfma(static_cast<const BatchJob *>(b->mfma.jobs.p),
-                                                static_cast<const uint64_t *>(b->mfma.tiles.p),
-                                                b->mfma.njobs, b->mfma.total_tiles, b->mfma.grid,
-                                                s));
+                                                static_cast<const uint32_t *>(b->mfma.tiles.p),
+                                                b->mfma.total_tiles, b->mfma.grid, s));
     if (st) return st;
   }
   if (b->table.total_tiles)
     return hip_status(launch_batch8(static_cast<const BatchJob *>(b->table.jobs.p),
-                                    static_cast<const uint64_t *>(b->table.tiles.p),
-                                    b->table.njobs, b->table.total_tiles, b->table.grid, s));
+                                    static_cast<const uint32_t *>(b->table.tiles.p),
+                                    b->table.total_tiles, b->table.grid, s));
   return TRS_OK;
 }
 
 }  // extern "C"
+
+// ============================================================================
+// autotuning support: time one apply candidate on synthetic device data
+// ============================================================================
+
+namespace {
+
+struct TuneSpace {
+  std::mutex mu;
+  DevBuf data;
+  DevCoef coef;
+  DevBuf frag;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+TuneSpace &tune_space() {
+  static TuneSpace *t = new TuneSpace();  // kept for the life of the process
+  return *t;
+}
+
+uint64_t tune_rng(uint64_t &s) {
+  s += 0x9E3779B97F4A7C15ull;
+  uint64_t z = s;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+
+extern "C" int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len, int route,
+                              uint64_t strip, int reps, double *ms_out) {
+  if (!ms_out || k == 0 || nout == 0 || len == 0 || reps <= 0) return TRS_ERR_INVALID_ARGUMENT;
+  if (field != 8 && field != 16) return TRS_ERR_INVALID_ARGUMENT;
+  if (field == 16 && ((len & 1) || (strip & 1))) return TRS_ERR_INVALID_ARGUMENT;
+  if (field == 8 && k + nout > 256) return TRS_ERR_INVALID_ARGUMENT;
+  int bs_route = 0;
+  if (field == 8 && route == 1) {
+    bs_route = bs_encode_available(kRouteBsFft, k, nout)      ? kRouteBsFft
+               : bs_encode_available(kRouteBsMatrix, k, nout) ? kRouteBsMatrix
+                                                                : 0;
+    if (!bs_route) return TRS_ERR_UNSUPPORTED;
+  } else if (field == 8 && route == 2) {
+    if (!mfma_available(k)) return TRS_ERR_UNSUPPORTED;
+  } else if (route != 0) {
+    return TRS_ERR_UNSUPPORTED;
+  }
+  TuneSpace &T = tune_space();
+  std::lock_guard<std::mutex> lk(T.mu);
+  if (!T.stream) {
+    TRS_TRY(hipStreamCreateWithFlags(&T.stream, hipStreamNonBlocking));
+    TRS_TRY(hipEventCreate(&T.ev0));
+    TRS_TRY(hipEventCreate(&T.ev1));
+  }
+  const uint64_t bytes = uint64_t(k + nout) * len;
+  if (bytes > T.data.n) {
+    TRS_TRY(T.data.alloc(bytes));
+    TRS_TRY(launch_fill(static_cast<uint8_t *>(T.data.p), bytes, 0x7A9E5EEDull, T.stream));
+  }
+  // random nonzero coefficients (kernel time does not depend on their values)
+  uint64_t seed = (uint64_t(field) << 40) ^ (uint64_t(k) << 20) ^ nout;
+  int st = TRS_OK;
+  std::vector<uint8_t> rows8;
+  if (field == 8) {
+    rows8.resize(size_t(k) * nout);
+    for (auto &c : rows8) c = uint8_t(1 + tune_rng(seed) % 255);
+    if (route == 0) st = upload_coef8(T.coef, rows8.data(), nout, k);
+    if (!st && route == 2) st = upload_frags(T.frag, rows8.data(), nout, k);
+  } else {
+    std::vector<uint16_t> rows16(size_t(k) * nout);
+    for (auto &c : rows16) c = uint16_t(1 + tune_rng(seed) % 65535);
+    st = upload_coef16(T.coef, rows16.data(), nout, k);
+  }
+  if (st) return st;
+  uint8_t *base = static_cast<uint8_t *>(T.data.p);
+  const uint64_t sw = (strip == 0 || strip >= len) ? len : strip;
+  PtrTable table16;
+  if (field == 16) {
+    std::vector<const uint8_t *> p16;
+    for (uint32_t i = 0; i < k + nout; ++i) p16.push_back(base + uint64_t(i) * len);
+    if ((st = table16.upload(p16, T.stream))) return st;
+  }
+  auto run_once = [&]() -> int {
+    for (uint64_t off = 0; off < (field == 16 ? len / 2 : len); off += (field == 16 ? sw / 2 : sw)) {
+      const uint64_t span =
+          std::min<uint64_t>(field == 16 ? sw / 2 : sw, (field == 16 ? len / 2 : len) - off);
+      hipError_t e = hipSuccess;
+      if (field == 16) {
+        Gf16Call c{};
+        c.ptrs = static_cast<const uint8_t *const *>(table16.d);
+        c.coef = T.coef.ptr();
+        c.plane = len / 2;
+        c.col0 = off;
+        c.cols = span;
+        c.k = k;
+        c.nout = nout;
+        c.ostride = T.coef.ostride;
+        c.vec_ok = ((len / 2) % 16 == 0 && off % 16 == 0) ? 1u : 0u;
+        c.mode = kWrite;
+        e = launch_gf16_apply(c, T.stream);
+      } else if (route == 2) {
+        Gf8MfmaCall c{};
+        for (uint32_t i = 0; i < k + nout; ++i) c.ptr[i] = base + uint64_t(i) * len + off;
+        c.frag = T.frag.p;
+        c.len = span;
+        c.k = k;
+        c.nout = nout;
+        c.vec_ok = (len % 4 == 0 && off % 4 == 0) ? 1u : 0u;
+        e = launch_gf8_mfma(c, T.stream);
+      } else {
+        Gf8Call c{};
+        for (uint32_t i = 0; i < k + nout; ++i) c.ptr[i] = base + uint64_t(i) * len + off;
+        c.coef = T.coef.ptr();
+        c.len = span;
+        c.k = k;
+        c.nout = nout;
+        c.ostride = T.coef.ostride;
+        c.vec_ok = (len % 16 == 0 && off % 16 == 0) ? 1u : 0u;
+        e = route == 1 ? launch_bs_encode(bs_route, c, T.stream) : launch_gf8_apply(c, kWrite, T.stream);
+      }
+      if (e != hipSuccess) return hip_status(e);
+    }
+    return TRS_OK;
+  };
+  if ((st = run_once())) return st;  // warm-up
+  TRS_TRY(hipEventRecord(T.ev0, T.stream));
+  for (int r = 0; r < reps; ++r)
+    if ((st = run_once())) return st;
+  TRS_TRY(hipEventRecord(T.ev1, T.stream));
+  TRS_TRY(hipEventSynchronize(T.ev1));
+  float ms = 0;
+  TRS_TRY(hipEventElapsedTime(&ms, T.ev0, T.ev1));
+  *ms_out = double(ms) / reps;
+  return TRS_OK;
+}

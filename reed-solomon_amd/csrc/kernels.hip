@@ -147,19 +147,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_gf8_apply(const Gf8Call a) {
 }
 
 // Batched work list: tiles are (job, 4 KiB column chunk, 16-row output tile),
-// numbered job by job.  Workgroup b takes the contiguous tile range
-// [b*T/G, (b+1)*T/G): one binary search for its first job, then it walks the
-// jobs forward as its range crosses them.
+// numbered job by job; tile_job[t] names the job of tile t.  A fixed grid
+// strides over the tiles, which interleaves heavy and light jobs across
+// workgroups.
 __global__ __launch_bounds__(kBlockThreads) void k_batch8(const BatchJob *jobs,
-                                                         const uint64_t *tile_begin,
-                                                         uint32_t njobs, uint64_t total) {
-  const uint64_t t0 = total * blockIdx.x / gridDim.x;
-  const uint64_t t1 = total * (blockIdx.x + 1) / gridDim.x;
-  if (t0 >= t1) return;
-  uint32_t j = first_job(tile_begin, njobs, t0);
-  for (uint64_t t = t0; t < t1; ++t) {
-    while (j + 1 < njobs && tile_begin[j + 1] <= t) ++j;
-    const BatchJob &J = jobs[j];
+                                                         const uint32_t *tile_job,
+                                                         uint64_t total) {
+  for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
+    const BatchJob &J = jobs[tile_job[t]];
     const uint64_t local = t - J.tile_begin;
     const uint64_t chunk = local / J.out_tiles;
     const uint32_t ot = uint32_t(local - chunk * J.out_tiles);
@@ -180,9 +175,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch8(const BatchJob *jobs,
 // product sums are accumulated separately and combined at the end.
 template <int NT, uint32_t MODE>
 __global__ __launch_bounds__(kBlockThreads) void k_gf16_apply(const Gf16Call a) {
-  const uint64_t off = (uint64_t(blockIdx.x) * kBlockThreads + threadIdx.x) * kLaneBytes;
-  if (off >= a.plane) return;
-  const uint64_t rem = a.plane - off;
+  const uint64_t rel = (uint64_t(blockIdx.x) * kBlockThreads + threadIdx.x) * kLaneBytes;
+  if (rel >= a.cols) return;
+  const uint64_t off = a.col0 + rel;
+  const uint64_t rem = a.cols - rel;
   const bool full = a.vec_ok && rem >= 16;
   const uint32_t jbase = blockIdx.y * NT;
   const uint32_t nt = min(uint32_t(NT), a.nout - jbase);
@@ -283,8 +279,8 @@ hipError_t launch_gf8_apply(const Gf8Call &call, uint32_t mode, hipStream_t stre
 }
 
 hipError_t launch_gf16_apply(const Gf16Call &call, hipStream_t stream) {
-  if (call.plane == 0 || call.nout == 0) return hipSuccess;
-  const uint64_t lanes = (call.plane + kLaneBytes - 1) / kLaneBytes;
+  if (call.cols == 0 || call.nout == 0) return hipSuccess;
+  const uint64_t lanes = (call.cols + kLaneBytes - 1) / kLaneBytes;
   const uint32_t gx = uint32_t((lanes + kBlockThreads - 1) / kBlockThreads);
   const int nt = call.nout <= 4 ? 4 : 8;
   const dim3 grid(gx, (call.nout + nt - 1) / nt);
@@ -301,11 +297,11 @@ hipError_t launch_gf16_apply(const Gf16Call &call, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_batch8(const BatchJob *jobs, const uint64_t *tile_begin, uint32_t njobs,
-                         uint64_t total_tiles, uint32_t grid, hipStream_t stream) {
+hipError_t launch_batch8(const BatchJob *jobs, const uint32_t *tile_job, uint64_t total_tiles,
+                         uint32_t grid, hipStream_t stream) {
   if (total_tiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_batch8, dim3(grid), dim3(kBlockThreads), 0, stream, jobs, tile_begin,
-                     njobs, total_tiles);
+  hipLaunchKernelGGL(k_batch8, dim3(grid), dim3(kBlockThreads), 0, stream, jobs, tile_job,
+                     total_tiles);
   return hipGetLastError();
 }
 

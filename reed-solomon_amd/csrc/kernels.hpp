@@ -41,12 +41,13 @@ struct Gf8Call {
 
 // One tower (GF((2^8)^2)) matrix apply over split-plane shards.  ptrs is a
 // device array [k inputs, nout outputs]; every shard is `plane` low bytes
-// then `plane` high bytes.  coef holds three transposed padded byte matrices
+// then `plane` high bytes; symbols col0 .. col0+cols-1 are applied.  coef holds three transposed padded byte matrices
 // (Karatsuba products c_lo, c_hi^c_lo, c_hi*L), each k * ostride bytes.
 struct Gf16Call {
   const uint8_t *const *ptrs;
   const uint32_t *coef;
-  uint64_t plane;
+  uint64_t plane;       // bytes per plane (the high plane starts at this offset)
+  uint64_t col0, cols;  // symbol window [col0, col0 + cols) of the planes to apply
   uint32_t k, nout, ostride, vec_ok;
   uint32_t mode;
   uint32_t *flag;
@@ -75,27 +76,13 @@ struct BatchJob {
   uint32_t prog;               // generated bit-sliced program id (k_batch_bs), 0 = none
 };
 
-#ifdef __HIPCC__
-// Largest job j with tile_begin[j] <= t (tile_begin ascending, tile_begin[0] = 0).
-__device__ __forceinline__ uint32_t first_job(const uint64_t *tile_begin, uint32_t njobs,
-                                              uint64_t t) {
-  uint32_t lo = 0, hi = njobs;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (tile_begin[mid] <= t)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-#endif
 
 // Host-side launchers (kernels.hip).  All return hipError_t of the launch.
 hipError_t launch_gf8_apply(const Gf8Call &call, uint32_t mode, hipStream_t stream);
 hipError_t launch_gf16_apply(const Gf16Call &call, hipStream_t stream);
-hipError_t launch_batch8(const BatchJob *jobs, const uint64_t *tile_begin, uint32_t njobs,
-                         uint64_t total_tiles, uint32_t grid, hipStream_t stream);
+// Batched persistent kernels: tile_job[t] is the job index of tile t.
+hipError_t launch_batch8(const BatchJob *jobs, const uint32_t *tile_job, uint64_t total_tiles,
+                         uint32_t grid, hipStream_t stream);
 hipError_t launch_fill(uint8_t *dst, uint64_t len, uint64_t seed, hipStream_t stream);
 
 // int8-MFMA apply (mfma.hip); available for k <= 32.
@@ -104,7 +91,7 @@ bool mfma_available(uint32_t k);
 uint32_t mfma_steps(uint32_t k);
 uint64_t mfma_group_cols();  // columns per workgroup tile (4 waves x 512)
 // Persistent work-list kernel over MFMA jobs (BatchJob::coef = A fragments).
-hipError_t launch_batch_mfma(const BatchJob *jobs, const uint64_t *tile_begin, uint32_t njobs,
+hipError_t launch_batch_mfma(const BatchJob *jobs, const uint32_t *tile_job,
                              uint64_t total_tiles, uint32_t grid, hipStream_t stream);
 
 // Bit-sliced encode with compile-time coefficients (gen_bitsliced.hip,
@@ -115,7 +102,7 @@ bool bs_encode_available(int route, uint32_t k, uint32_t m);
 uint32_t bs_program_id(int route, uint32_t k, uint32_t m);
 // Persistent work-list kernel over jobs with a generated program (8 KiB tiles).
 constexpr uint64_t kBsChunkBytes = 8192;
-hipError_t launch_batch_bs(const BatchJob *jobs, const uint64_t *tile_begin, uint32_t njobs,
-                           uint64_t total_tiles, uint32_t grid, hipStream_t stream);
+hipError_t launch_batch_bs(const BatchJob *jobs, const uint32_t *tile_job, uint64_t total_tiles,
+                           uint32_t grid, hipStream_t stream);
 
 }  // namespace trs

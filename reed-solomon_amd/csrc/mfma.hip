@@ -214,17 +214,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_gf8_mfma(const Gf8MfmaCall a)
 }
 
 // Batched form: tiles are (job, pass, 2048-column group), numbered job by job;
-// workgroup b takes the contiguous range [b*T/G, (b+1)*T/G).  J.coef carries the A fragments.
+// tile_job[t] names the job of tile t and a fixed grid strides over them.
+// J.coef carries the A fragments.
 __global__ __launch_bounds__(kBlockThreads) void k_batch_mfma(const BatchJob *jobs,
-                                                             const uint64_t *tile_begin,
-                                                             uint32_t njobs, uint64_t total) {
-  const uint64_t t0 = total * blockIdx.x / gridDim.x;
-  const uint64_t t1 = total * (blockIdx.x + 1) / gridDim.x;
-  if (t0 >= t1) return;
-  uint32_t j = first_job(tile_begin, njobs, t0);
-  for (uint64_t t = t0; t < t1; ++t) {
-    while (j + 1 < njobs && tile_begin[j + 1] <= t) ++j;
-    const BatchJob &J = jobs[j];
+                                                             const uint32_t *tile_job,
+                                                             uint64_t total) {
+  for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
+    const BatchJob &J = jobs[tile_job[t]];
     const uint32_t passes = (J.nout + 7) / 8;
     const uint64_t local = t - J.tile_begin;
     const uint64_t group = local / passes;
@@ -272,11 +268,11 @@ bool mfma_available(uint32_t k) { return mfma_steps(k) <= 8 && k > 0; }
 
 uint64_t mfma_group_cols() { return kGroupCols; }
 
-hipError_t launch_batch_mfma(const BatchJob *jobs, const uint64_t *tile_begin, uint32_t njobs,
+hipError_t launch_batch_mfma(const BatchJob *jobs, const uint32_t *tile_job,
                              uint64_t total_tiles, uint32_t grid, hipStream_t stream) {
   if (total_tiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_batch_mfma, dim3(grid), dim3(kBlockThreads), 0, stream, jobs, tile_begin,
-                     njobs, total_tiles);
+  hipLaunchKernelGGL(k_batch_mfma, dim3(grid), dim3(kBlockThreads), 0, stream, jobs, tile_job,
+                     total_tiles);
   return hipGetLastError();
 }
 

@@ -179,16 +179,12 @@ def generate() -> str:
             parts.append("")
             table.append((route, k, m, name))
     # Persistent work-list kernel over jobs that name a generated program:
-    # tiles are (job, 8 KiB column chunk), numbered job by job.
+    # tiles are (job, 8 KiB column chunk), numbered job by job; tile_job[t]
+    # names the job of tile t.
     parts.append("__global__ __launch_bounds__(kBlockThreads) void k_batch_bs(const BatchJob *jobs,")
-    parts.append("    const uint64_t *tile_begin, uint32_t njobs, uint64_t total) {")
-    parts.append("  const uint64_t t0 = total * blockIdx.x / gridDim.x;")
-    parts.append("  const uint64_t t1 = total * (blockIdx.x + 1) / gridDim.x;")
-    parts.append("  if (t0 >= t1) return;")
-    parts.append("  uint32_t j = first_job(tile_begin, njobs, t0);")
-    parts.append("  for (uint64_t t = t0; t < t1; ++t) {")
-    parts.append("    while (j + 1 < njobs && tile_begin[j + 1] <= t) ++j;")
-    parts.append("    const BatchJob &J = jobs[j];")
+    parts.append("    const uint32_t *tile_job, uint64_t total) {")
+    parts.append("  for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {")
+    parts.append("    const BatchJob &J = jobs[tile_job[t]];")
     parts.append("    const uint64_t off = (t - J.tile_begin) * 8192 + uint64_t(threadIdx.x) * 32;")
     parts.append("    if (off >= J.len) continue;")
     parts.append("    const uint64_t rem = J.len - off;")
@@ -213,11 +209,11 @@ def generate() -> str:
     parts.append("  return 0;")
     parts.append("}")
     parts.append("")
-    parts.append("hipError_t launch_batch_bs(const BatchJob *jobs, const uint64_t *tile_begin, "
-                 "uint32_t njobs, uint64_t total_tiles, uint32_t grid, hipStream_t stream) {")
+    parts.append("hipError_t launch_batch_bs(const BatchJob *jobs, const uint32_t *tile_job, "
+                 "uint64_t total_tiles, uint32_t grid, hipStream_t stream) {")
     parts.append("  if (total_tiles == 0) return hipSuccess;")
     parts.append("  hipLaunchKernelGGL(k_batch_bs, dim3(grid), dim3(kBlockThreads), 0, stream, jobs,")
-    parts.append("                     tile_begin, njobs, total_tiles);")
+    parts.append("                     tile_job, total_tiles);")
     parts.append("  return hipGetLastError();")
     parts.append("}")
     parts.append("")
