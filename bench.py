@@ -56,6 +56,91 @@ def build_workload(stripes: int, rng: np.random.Generator):
     return meta
 
 
+SERVE_SHAPES = [(10, 4), (7, 13), (10, 10), (14, 14), (16, 16), (18, 6), (32, 32)]
+
+
+def served_leg(trs, torch, requests: int, rates, seed: int):
+    """Served mixed workload: Poisson arrivals, 70 % encode / 30 % reconstruct,
+    shapes of the C1/C2 configurations, 90 % GF(2^8) / 10 % GF(2^16), shard
+    length log-uniform [4 KiB, 4 MiB], erasures uniform in 1..m, device-resident
+    shards; continuous batching (4 slots, 64 GiB HBM budget) against the same
+    requests issued as plain per-request calls."""
+    rng = np.random.default_rng(seed)
+    c8 = {s: trs.ReedSolomon(*s) for s in SERVE_SHAPES}
+    c16 = {}
+    reqs = []
+    lo, hi = math.log(4096), math.log(4 << 20)
+    for i in range(requests):
+        gf16 = rng.random() < 0.10
+        k, m = SERVE_SHAPES[int(rng.integers(0, len(SERVE_SHAPES)))]
+        L = int(round(math.exp(rng.uniform(lo, hi)) / 64.0)) * 64
+        if gf16 and (k, m) not in c16:
+            c16[(k, m)] = trs.ReedSolomon16(k, m)
+        rs = c16[(k, m)] if gf16 else c8[(k, m)]
+        buf = torch.empty((k + m) * L, dtype=torch.uint8, device="cuda")
+        trs.fill_random(buf[:k * L], seed + i)
+        shards = [buf[j * L:(j + 1) * L] for j in range(k + m)]
+        rs.encode(shards)
+        if rng.random() < 0.7:
+            reqs.append(("enc", rs, shards, None, (k + m) * L))
+        else:
+            e = int(rng.integers(1, m + 1))
+            erased = set(rng.choice(k + m, e, replace=False).tolist())
+            reqs.append(("rec", rs, shards, [j not in erased for j in range(k + m)], (k + e) * L))
+    torch.cuda.synchronize()
+    total_bytes = sum(r[4] for r in reqs)
+
+    # plain per-request calls, one after another
+    stream = torch.cuda.current_stream()
+    lat = []
+    t0 = time.perf_counter()
+    for kind, rs, shards, present, _ in reqs:
+        a = time.perf_counter()
+        if kind == "enc":
+            rs.encode(shards)
+        else:
+            rs.reconstruct([(t, p) for t, p in zip(shards, present)])
+        stream.synchronize()
+        lat.append((time.perf_counter() - a) * 1e6)
+    plain_s = time.perf_counter() - t0
+    plain = {"gbs": round(total_bytes / plain_s / 1e9, 2),
+             "p50_us": round(float(np.percentile(lat, 50)), 1),
+             "p99_us": round(float(np.percentile(lat, 99)), 1)}
+
+    out = {"requests": requests, "bytes": total_bytes, "plain_per_request": plain, "rates": []}
+    srv = trs.Server(slots=4, hbm_budget_bytes=64 << 30, max_batch=256, window_us=50)
+    for rate in rates:
+        gaps = rng.exponential(1.0 / rate, len(reqs))
+        sched = np.cumsum(gaps)
+        tickets, submit_t = [], []
+        start = time.perf_counter()
+        for (kind, rs, shards, present, _), at in zip(reqs, sched):
+            while time.perf_counter() - start < at:
+                pass
+            submit_t.append(time.perf_counter())
+            if kind == "enc":
+                tickets.append(srv.submit_encode(rs, shards))
+            else:
+                tickets.append(srv.submit_reconstruct(rs, shards, present))
+        srv.drain()
+        lats, ends = [], []
+        for t, st in zip(tickets, submit_t):
+            l_us, _ = srv.take(t)
+            lats.append(l_us)
+            ends.append(st + l_us * 1e-6)
+        span = max(ends) - min(submit_t)
+        out["rates"].append({
+            "rate_req_s": rate,
+            "gbs": round(total_bytes / span / 1e9, 2),
+            "p50_us": round(float(np.percentile(lats, 50)), 1),
+            "p99_us": round(float(np.percentile(lats, 99)), 1),
+            "offered_s": round(float(sched[-1]), 4),
+            "span_s": round(span, 4),
+        })
+    srv.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +149,8 @@ def main() -> int:
     ap.add_argument("--stripes", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=800)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -239,6 +326,13 @@ def main() -> int:
                "sample": f"first {nst} stripes of the workload, encode + reconstruct, "
                          f"scalar C oracle (MUL-table loop), {spent:.1f} s"}
 
+    served = None
+    if not args.no_serve:
+        del buf, pristine
+        torch.cuda.empty_cache()
+        served = served_leg(trs, torch, args.serve_requests, [1000, 5000, 20000, 50000],
+                            SEED + 5)
+
     out = {
         "metric": "GB/s encode+reconstruct (byte-exact), batched vs per-call on 1 GPU; p99 latency",
         "value": round(value, 2),
@@ -286,6 +380,7 @@ def main() -> int:
                                        "p99": round(float(np.percentile(lat, 99)), 1)},
         },
         "batched_vs_per_call": round(value / percall_gbs, 2),
+        "served": served,
         "byte_exact": {"batched_reconstruct": rec_exact, "batched_encode": enc_exact,
                        "after_timed_steps": exact_after, "per_call": exact_percall,
                        "oracle_sampled_parity": host_ok},

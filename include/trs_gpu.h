@@ -202,6 +202,38 @@ int trs_batch8_run(trs_batch8 *b, void *stream);
 uint64_t trs_batch8_bytes(const trs_batch8 *b);
 size_t trs_batch8_size(const trs_batch8 *b);
 
+/* ================= serving loop (no reference counterpart) ================= */
+/* Continuous batching of encode / reconstruct requests over device-resident
+ * shards.  Submissions are validated like the single calls (same error codes)
+ * and return a ticket; a dispatcher packs pending requests, in arrival order,
+ * into the batch of a free slot (slot = stream + batch) while the bytes of
+ * in-flight work stay within hbm_budget_bytes (0 = unlimited), and launches
+ * it, so requests arriving while others run start as soon as a slot and budget
+ * free up.  window_us: how long the dispatcher waits for more arrivals before
+ * launching a partial batch.  Shard buffers must stay valid until the ticket
+ * completes. */
+typedef struct trs_server trs_server;
+int trs_server_new(uint32_t slots, uint64_t hbm_budget_bytes, uint32_t max_batch,
+                   uint32_t window_us, trs_server **out);
+void trs_server_free(trs_server *s);   /* drains queued and in-flight work first */
+int trs_server_submit_encode8(trs_server *s, const trs_rs8 *rs, uint8_t *const *shards,
+                              const size_t *lens, size_t n_shards, uint64_t *ticket);
+int trs_server_submit_reconstruct8(trs_server *s, const trs_rs8 *rs, uint8_t *const *shards,
+                                   const size_t *lens, const uint8_t *present, size_t n_shards,
+                                   int data_only, uint64_t *ticket);
+int trs_server_submit_encode16(trs_server *s, const trs_rs16 *rs, uint8_t *const *shards,
+                               const size_t *lens, size_t n_shards, uint64_t *ticket);
+int trs_server_submit_reconstruct16(trs_server *s, const trs_rs16 *rs, uint8_t *const *shards,
+                                    const size_t *lens, const uint8_t *present, size_t n_shards,
+                                    int data_only, uint64_t *ticket);
+int trs_server_poll(trs_server *s, uint64_t ticket, int *done, int *status);
+int trs_server_wait(trs_server *s, uint64_t ticket, int *status);
+int trs_server_drain(trs_server *s);
+/* Submit-to-completion latency (microseconds), algorithmic bytes and status of
+ * a finished ticket; the ticket is forgotten afterwards. */
+int trs_server_take(trs_server *s, uint64_t ticket, double *latency_us, uint64_t *bytes,
+                    int *status);
+
 /* ================= autotuning (no reference counterpart) ================== */
 /* Times one apply candidate on synthetic device data: `nout` outputs from `k`
  * inputs of `len` bytes over GF(2^8) (field 8) or the tower (field 16,

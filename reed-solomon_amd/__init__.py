@@ -41,6 +41,7 @@ __all__ = [
     "PreparedDecoder",
     "PreparedDecoder16",
     "Batch",
+    "Server",
     "mul_slice",
     "mul_slice_xor",
     "fill_random",
@@ -431,6 +432,79 @@ class Batch:
 
     def __len__(self) -> int:
         return lib.trs_batch8_size(self._h)
+
+
+class Server:
+    """Continuous-batching request loop over device-resident shards (no
+    reference counterpart).  submit_* validate like the single calls and return
+    a ticket; wait(ticket) blocks until that request is done; take(ticket)
+    returns (latency_us, bytes).  Tensors must stay alive until completion."""
+
+    def __init__(self, slots: int = 4, hbm_budget_bytes: int = 64 << 30, max_batch: int = 256,
+                 window_us: int = 50):
+        h = ctypes.c_void_p()
+        _check(lib.trs_server_new(slots, hbm_budget_bytes, max_batch, window_us,
+                                  ctypes.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def close(self) -> None:
+        h = getattr(self, "_h", None)
+        if h and lib is not None:
+            try:
+                lib.trs_server_free(h)
+            except Exception:
+                pass
+            self._h = None
+            self._keep = {}
+
+    def __del__(self):
+        self.close()
+
+    def _ticket(self, rc: int, t, keep) -> int:
+        _check(rc)
+        self._keep[t.value] = keep
+        return t.value
+
+    def submit_encode(self, rs, shards: Sequence) -> int:
+        p, l = _ptrs(shards)
+        t = ctypes.c_uint64()
+        fn = lib.trs_server_submit_encode16 if isinstance(rs, ReedSolomon16) else \
+            lib.trs_server_submit_encode8
+        return self._ticket(fn(self._h, rs._h, p, l, len(shards), ctypes.byref(t)), t,
+                            (rs, shards))
+
+    def submit_reconstruct(self, rs, shards: Sequence, present: Sequence[bool],
+                           data_only: bool = False) -> int:
+        p, l = _ptrs(shards)
+        t = ctypes.c_uint64()
+        fn = lib.trs_server_submit_reconstruct16 if isinstance(rs, ReedSolomon16) else \
+            lib.trs_server_submit_reconstruct8
+        return self._ticket(fn(self._h, rs._h, p, l, _flags(present), len(shards), int(data_only),
+                               ctypes.byref(t)), t, (rs, shards))
+
+    def wait(self, ticket: int) -> None:
+        st = ctypes.c_int(0)
+        _check(lib.trs_server_wait(self._h, ticket, ctypes.byref(st)))
+        _check(st.value)
+
+    def poll(self, ticket: int) -> bool:
+        done, st = ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib.trs_server_poll(self._h, ticket, ctypes.byref(done), ctypes.byref(st)))
+        if done.value:
+            _check(st.value)
+        return bool(done.value)
+
+    def drain(self) -> None:
+        _check(lib.trs_server_drain(self._h))
+
+    def take(self, ticket: int):
+        lat, nb, st = ctypes.c_double(0), ctypes.c_uint64(0), ctypes.c_int(0)
+        _check(lib.trs_server_take(self._h, ticket, ctypes.byref(lat), ctypes.byref(nb),
+                                   ctypes.byref(st)))
+        self._keep.pop(ticket, None)
+        _check(st.value)
+        return lat.value, nb.value
 
 
 def _slice(out, inp, c: int, xor: bool) -> None:

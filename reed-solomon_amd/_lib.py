@@ -98,6 +98,23 @@ _sig("trs_mlp_train", c_int, c_vp, c_fp, c_fp, c_size, c_int, ctypes.c_float, c_
 _sig("trs_mlp_predict", c_int, c_vp, c_fp, c_size, c_fp)
 
 
+c_u64p = ctypes.POINTER(ctypes.c_uint64)
+_sig("trs_server_new", c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+     ctypes.POINTER(c_vp))
+_sig("trs_server_free", None, c_vp)
+_sig("trs_server_submit_encode8", c_int, c_vp, c_vp, c_ptrs, c_sizep, c_size, c_u64p)
+_sig("trs_server_submit_reconstruct8", c_int, c_vp, c_vp, c_ptrs, c_sizep, c_u8p, c_size, c_int,
+     c_u64p)
+_sig("trs_server_submit_encode16", c_int, c_vp, c_vp, c_ptrs, c_sizep, c_size, c_u64p)
+_sig("trs_server_submit_reconstruct16", c_int, c_vp, c_vp, c_ptrs, c_sizep, c_u8p, c_size, c_int,
+     c_u64p)
+_sig("trs_server_poll", c_int, c_vp, ctypes.c_uint64, ctypes.POINTER(c_int), ctypes.POINTER(c_int))
+_sig("trs_server_wait", c_int, c_vp, ctypes.c_uint64, ctypes.POINTER(c_int))
+_sig("trs_server_drain", c_int, c_vp)
+_sig("trs_server_take", c_int, c_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double), c_u64p,
+     ctypes.POINTER(c_int))
+
+
 def declared_symbols() -> list:
     """Every function name include/trs_gpu.h declares."""
     import re

@@ -1381,3 +1381,375 @@ extern "C" int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len
   *ms_out = double(ms) / reps;
   return TRS_OK;
 }
+
+// ============================================================================
+// serving loop: continuous batching of encode / reconstruct requests
+// ============================================================================
+//
+// Callers submit requests from any thread; each is validated on submit
+// exactly like the single call (same error codes) and gets a ticket.  A
+// dispatcher thread packs pending requests into the work list of a free slot
+// (one slot = one stream + one batch), admitting requests in arrival order
+// while the bytes of in-flight work stay within the HBM budget, and launches
+// it.  Slots run concurrently, so requests that arrive while others are on
+// the GPU start as soon as a slot and budget free up.  A completion thread per
+// slot waits for its batch, stamps every request's completion time and frees
+// the slot.  GF(2^8) requests run in the slot's batched kernels; tower
+// requests run as per-call applies on the slot's stream.
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+#include <unordered_map>
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct SrvReq {
+  uint64_t ticket = 0;
+  bool gf16 = false;
+  trs_batch8::Req r8;                  // GF(2^8) request (batched)
+  std::shared_ptr<Plan> plan16;        // tower reconstruct plan (keeps coef alive)
+  const DevCoef *coef16 = nullptr;     // tower coefficients
+  std::vector<const uint8_t *> ptrs16;
+  uint64_t len16 = 0;
+  uint32_t k16 = 0, nout16 = 0;
+  uint64_t bytes = 0;                  // algorithmic bytes (k + nout) * len
+};
+
+struct TicketState {
+  int status = 0;
+  bool done = false;
+  Clock::time_point submit, finish;
+  uint64_t bytes = 0;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  trs_batch8 batch;
+  std::vector<SrvReq> reqs;
+  uint64_t bytes = 0;
+  bool busy = false;      // owned by a batch (set when the dispatcher takes the slot)
+  bool launched = false;  // the batch is on the stream; the completer may wait for it
+  int launch_status = 0;
+};
+
+}  // namespace
+
+struct trs_server {
+  uint64_t budget = 0;
+  size_t max_batch = 0;
+  uint32_t window_us = 0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<SrvReq> pending;
+  std::vector<std::unique_ptr<Slot>> slots;
+  std::unordered_map<uint64_t, TicketState> tickets;
+  uint64_t next_ticket = 1;
+  uint64_t inflight = 0;
+  bool stop = false;
+  std::thread dispatcher;
+  std::vector<std::thread> completers;
+  std::vector<std::condition_variable *> slot_cv;
+};
+
+namespace {
+
+void server_dispatch(trs_server *S) {
+  std::unique_lock<std::mutex> lk(S->mu);
+  while (true) {
+    S->cv_work.wait(lk, [&] {
+      if (S->stop) return true;
+      if (S->pending.empty()) return false;
+      for (auto &sl : S->slots)
+        if (!sl->busy) return true;
+      return false;
+    });
+    if (S->stop) return;
+    // short batching window: let a burst of arrivals join this batch
+    if (S->window_us && S->pending.size() < S->max_batch) {
+      lk.unlock();
+      std::this_thread::sleep_for(std::chrono::microseconds(S->window_us));
+      lk.lock();
+      if (S->stop) return;
+    }
+    Slot *slot = nullptr;
+    size_t si = 0;
+    for (size_t i = 0; i < S->slots.size(); ++i)
+      if (!S->slots[i]->busy) {
+        slot = S->slots[i].get();
+        si = i;
+        break;
+      }
+    if (!slot) continue;
+    // admission by HBM budget, in arrival order
+    slot->reqs.clear();
+    slot->bytes = 0;
+    while (!S->pending.empty() && slot->reqs.size() < S->max_batch) {
+      const uint64_t b = S->pending.front().bytes;
+      if (S->inflight + slot->bytes + b > S->budget && !(S->inflight == 0 && slot->reqs.empty()))
+        break;
+      slot->bytes += b;
+      slot->reqs.push_back(std::move(S->pending.front()));
+      S->pending.pop_front();
+    }
+    if (slot->reqs.empty()) {
+      // the next request does not fit beside in-flight work: wait for completions
+      S->cv_work.wait(lk, [&] {
+        return S->stop || S->pending.empty() || S->inflight == 0 ||
+               S->inflight + S->pending.front().bytes <= S->budget;
+      });
+      continue;
+    }
+    slot->busy = true;
+    S->inflight += slot->bytes;
+    lk.unlock();
+    // build and launch the slot's batch outside the lock
+    int st = TRS_OK;
+    trs_batch8_clear(&slot->batch);
+    for (auto &q : slot->reqs)
+      if (!q.gf16) slot->batch.reqs.push_back(q.r8);
+    if (!slot->batch.reqs.empty()) {
+      st = trs_batch8_prepare(&slot->batch);
+      if (!st) st = trs_batch8_run(&slot->batch, slot->stream);
+    }
+    for (auto &q : slot->reqs)
+      if (q.gf16 && !st) st = run_gf16(q.ptrs16, q.k16, q.nout16, *q.coef16, q.len16, kWrite, nullptr,
+                                      slot->stream);
+    if (!st) st = hip_status(hipEventRecord(slot->done, slot->stream));
+    lk.lock();
+    slot->launch_status = st;
+    slot->launched = true;
+    S->slot_cv[si]->notify_one();
+  }
+}
+
+void server_complete(trs_server *S, size_t si) {
+  Slot *slot = S->slots[si].get();
+  std::unique_lock<std::mutex> lk(S->mu);
+  while (true) {
+    S->slot_cv[si]->wait(lk, [&] { return S->stop || slot->launched; });
+    if (!slot->launched) return;  // stopping with nothing in flight
+    const int launch = slot->launch_status;
+    lk.unlock();
+    int st = launch;
+    if (!st) st = hip_status(hipEventSynchronize(slot->done));
+    const auto now = Clock::now();
+    lk.lock();
+    for (auto &q : slot->reqs) {
+      auto &t = S->tickets[q.ticket];
+      t.done = true;
+      t.status = st;
+      t.finish = now;
+    }
+    slot->reqs.clear();
+    S->inflight -= slot->bytes;
+    slot->bytes = 0;
+    slot->launched = false;
+    slot->busy = false;
+    S->cv_done.notify_all();
+    S->cv_work.notify_all();
+    if (S->stop) return;
+  }
+}
+
+int server_enqueue(trs_server *S, SrvReq &&q, uint64_t *ticket) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  q.ticket = S->next_ticket++;
+  TicketState t;
+  t.submit = Clock::now();
+  t.bytes = q.bytes;
+  if (q.bytes == 0) {  // nothing to do (all present / data-only with no missing data)
+    t.done = true;
+    t.finish = t.submit;
+  }
+  S->tickets[q.ticket] = t;
+  *ticket = q.ticket;
+  if (q.bytes) {
+    S->pending.push_back(std::move(q));
+    S->cv_work.notify_one();
+  }
+  return TRS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int trs_server_new(uint32_t slots, uint64_t hbm_budget_bytes, uint32_t max_batch,
+                   uint32_t window_us, trs_server **out) {
+  if (!out || slots == 0 || slots > 64 || max_batch == 0) return TRS_ERR_INVALID_ARGUMENT;
+  auto S = std::make_unique<trs_server>();
+  S->budget = hbm_budget_bytes ? hbm_budget_bytes : ~uint64_t(0);
+  S->max_batch = max_batch;
+  S->window_us = window_us;
+  for (uint32_t i = 0; i < slots; ++i) {
+    auto sl = std::make_unique<Slot>();
+    TRS_TRY(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
+    TRS_TRY(hipEventCreateWithFlags(&sl->done, hipEventDisableTiming));
+    S->slots.push_back(std::move(sl));
+    S->slot_cv.push_back(new std::condition_variable());
+  }
+  trs_server *raw = S.get();
+  raw->dispatcher = std::thread(server_dispatch, raw);
+  for (uint32_t i = 0; i < slots; ++i) raw->completers.emplace_back(server_complete, raw, i);
+  *out = S.release();
+  return TRS_OK;
+}
+
+void trs_server_free(trs_server *S) {
+  if (!S) return;
+  {
+    std::unique_lock<std::mutex> lk(S->mu);
+    // drain: wait for queued and in-flight work
+    S->cv_done.wait(lk, [&] {
+      if (!S->pending.empty()) return false;
+      for (auto &sl : S->slots)
+        if (sl->busy) return false;
+      return true;
+    });
+    S->stop = true;
+    S->cv_work.notify_all();
+    for (auto *cv : S->slot_cv) cv->notify_all();
+  }
+  S->dispatcher.join();
+  for (auto &t : S->completers) t.join();
+  for (auto &sl : S->slots) {
+    (void)hipEventDestroy(sl->done);
+    (void)hipStreamDestroy(sl->stream);
+  }
+  for (auto *cv : S->slot_cv) delete cv;
+  delete S;
+}
+
+int trs_server_submit_encode8(trs_server *S, const trs_rs8 *rs, uint8_t *const *shards,
+                              const size_t *lens, size_t n, uint64_t *ticket) {
+  if (!S || !ticket) return TRS_ERR_INVALID_ARGUMENT;
+  trs_batch8 tmp;
+  int st = trs_batch8_add_encode(&tmp, rs, shards, lens, n);
+  if (st) return st;
+  SrvReq q;
+  q.r8 = std::move(tmp.reqs.front());
+  q.bytes = uint64_t(q.r8.k + q.r8.nout) * q.r8.len;
+  return server_enqueue(S, std::move(q), ticket);
+}
+
+int trs_server_submit_reconstruct8(trs_server *S, const trs_rs8 *rs, uint8_t *const *shards,
+                                   const size_t *lens, const uint8_t *present, size_t n,
+                                   int data_only, uint64_t *ticket) {
+  if (!S || !ticket) return TRS_ERR_INVALID_ARGUMENT;
+  trs_batch8 tmp;
+  int st = trs_batch8_add_reconstruct(&tmp, rs, shards, lens, present, n, data_only);
+  if (st) return st;
+  SrvReq q;
+  if (!tmp.reqs.empty()) {
+    q.r8 = std::move(tmp.reqs.front());
+    q.bytes = uint64_t(q.r8.k + q.r8.nout) * q.r8.len;
+  }
+  return server_enqueue(S, std::move(q), ticket);
+}
+
+int trs_server_submit_encode16(trs_server *S, const trs_rs16 *rs, uint8_t *const *shards,
+                               const size_t *lens, size_t n, uint64_t *ticket) {
+  if (!S || !ticket || !rs || (n && (!shards || !lens))) return TRS_ERR_INVALID_ARGUMENT;
+  if (n < rs->n) return TRS_TOO_FEW_SHARDS;
+  if (n > rs->n) return TRS_TOO_MANY_SHARDS;
+  size_t len = 0;
+  int st = check_shards16(lens, n, &len);
+  if (st) return st;
+  SrvReq q;
+  q.gf16 = true;
+  q.ptrs16.assign(shards, shards + n);
+  for (auto p : q.ptrs16)
+    if (!p) return TRS_ERR_INVALID_ARGUMENT;
+  q.coef16 = &rs->parity;
+  q.len16 = len;
+  q.k16 = uint32_t(rs->k);
+  q.nout16 = uint32_t(rs->m);
+  q.bytes = uint64_t(rs->n) * len;
+  return server_enqueue(S, std::move(q), ticket);
+}
+
+int trs_server_submit_reconstruct16(trs_server *S, const trs_rs16 *rs, uint8_t *const *shards,
+                                    const size_t *lens, const uint8_t *present, size_t n,
+                                    int data_only, uint64_t *ticket) {
+  if (!S || !ticket || !rs || (n && (!shards || !lens || !present)))
+    return TRS_ERR_INVALID_ARGUMENT;
+  if (n != rs->n) return n < rs->n ? TRS_TOO_FEW_SHARDS : TRS_TOO_MANY_SHARDS;
+  Mask mask;
+  size_t count = 0, len = 0;
+  int st = scan_shards(lens, present, n, true, mask, count, len);
+  if (st) return st;
+  SrvReq q;
+  q.gf16 = true;
+  if (count != rs->n) {
+    if (count < rs->k) return TRS_TOO_FEW_SHARDS_PRESENT;
+    std::shared_ptr<Plan> plan;
+    if ((st = plan16_for(rs, mask, &plan))) return st;
+    size_t nout = 0;
+    if ((st = gather(*plan, shards, lens, rs->k, len, data_only != 0, q.ptrs16, nout))) return st;
+    if (nout) {
+      q.plan16 = plan;
+      q.coef16 = &plan->coef;
+      q.len16 = len;
+      q.k16 = uint32_t(rs->k);
+      q.nout16 = uint32_t(nout);
+      q.bytes = uint64_t(rs->k + nout) * len;
+    }
+  }
+  return server_enqueue(S, std::move(q), ticket);
+}
+
+int trs_server_wait(trs_server *S, uint64_t ticket, int *status) {
+  if (!S) return TRS_ERR_INVALID_ARGUMENT;
+  std::unique_lock<std::mutex> lk(S->mu);
+  auto it = S->tickets.find(ticket);
+  if (it == S->tickets.end()) return TRS_ERR_INVALID_ARGUMENT;
+  S->cv_done.wait(lk, [&] { return S->tickets[ticket].done; });
+  if (status) *status = S->tickets[ticket].status;
+  return TRS_OK;
+}
+
+int trs_server_poll(trs_server *S, uint64_t ticket, int *done, int *status) {
+  if (!S || !done) return TRS_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->tickets.find(ticket);
+  if (it == S->tickets.end()) return TRS_ERR_INVALID_ARGUMENT;
+  *done = it->second.done ? 1 : 0;
+  if (status) *status = it->second.status;
+  return TRS_OK;
+}
+
+int trs_server_drain(trs_server *S) {
+  if (!S) return TRS_ERR_INVALID_ARGUMENT;
+  std::unique_lock<std::mutex> lk(S->mu);
+  S->cv_done.wait(lk, [&] {
+    if (!S->pending.empty()) return false;
+    for (auto &sl : S->slots)
+      if (sl->busy) return false;
+    return true;
+  });
+  return TRS_OK;
+}
+
+// Latency (microseconds, submit to completion) and bytes of a finished ticket;
+// the entry is forgotten afterwards.
+int trs_server_take(trs_server *S, uint64_t ticket, double *latency_us, uint64_t *bytes,
+                    int *status) {
+  if (!S) return TRS_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->tickets.find(ticket);
+  if (it == S->tickets.end() || !it->second.done) return TRS_ERR_INVALID_ARGUMENT;
+  if (latency_us)
+    *latency_us =
+        std::chrono::duration<double, std::micro>(it->second.finish - it->second.submit).count();
+  if (bytes) *bytes = it->second.bytes;
+  if (status) *status = it->second.status;
+  S->tickets.erase(it);
+  return TRS_OK;
+}
+
+}  // extern "C"
