@@ -155,6 +155,37 @@ int trs_rs16_encode_sep(const trs_rs16 *rs, const uint8_t *const *data, const si
 int trs_rs16_encode_sep_matrix(const trs_rs16 *rs, const uint8_t *const *data,
                                const size_t *data_lens, size_t n_data, uint8_t *const *parity,
                                const size_t *parity_lens, size_t n_parity, void *stream);
+/* The tower-FFT route regardless of routing: the staged program of
+ * src/fft16.rs:357-376 run by the GPU interpreter (fft16::encode,
+ * src/fft16.rs:887-930, is the reference path it replaces).  Available for
+ * 2 <= k <= 1024 and k + m <= 4096 whose program fits one workgroup's LDS;
+ * otherwise TRS_ERR_UNSUPPORTED. */
+int trs_rs16_encode_sep_fft(const trs_rs16 *rs, const uint8_t *const *data,
+                            const size_t *data_lens, size_t n_data, uint8_t *const *parity,
+                            const size_t *parity_lens, size_t n_parity, void *stream);
+/* Host-only introspection of the staged program for shape (k, m) (the
+ * fields of StagedProgram16, src/fft16.rs:151-174, as the dump_stats test
+ * src/fft16.rs:1035-1074 prints them) and of its device lowering.  out[0..n):
+ * register_count, mult_count, glue ops, stages, zero registers, profitable
+ * (routing rule src/fft16.rs:85-106), device registers, device steps, LDS
+ * bytes per 32-column group.  TRS_ERR_UNSUPPORTED outside 2 <= k <= 1024,
+ * k + m <= 4096.  Touches no device. */
+int trs_fft16_program_info(size_t k, size_t m, size_t *out, size_t n);
+/* Host-only dump of the device lowering of shape (k, m)'s program, the data
+ * the GPU interpreter runs (no reference counterpart; tests replay it on the
+ * CPU).  counts[0..5): u32 words of steps (8 per step: kind, count, a, b, c,
+ * d, 0, 0), constant words, op words (2 per op), parity registers, and the
+ * register count.  Buffers too small for any count: only counts are filled. */
+int trs_fft16_lowered_program(size_t k, size_t m, uint32_t *steps, size_t steps_cap,
+                              uint32_t *consts, size_t consts_cap, uint32_t *ops, size_t ops_cap,
+                              uint32_t *parity, size_t parity_cap, size_t *counts);
+/* Encode of `stripes` stripes of the codec's shape in one call: shards holds
+ * stripes * (k + m) pointers (stripe s: data then parity), every shard `len`
+ * bytes.  route 0: the codec's route, 1: matrix, 2: tower FFT.  The reference
+ * encodes stripe by stripe (src/reedsolomon16.rs:412-418); batching only
+ * merges the launches. */
+int trs_rs16_encode_stripes(const trs_rs16 *rs, uint8_t *const *shards, size_t stripes,
+                            size_t len, int route, void *stream);
 /* ReedSolomon16::verify, src/reedsolomon16.rs:453-505. */
 int trs_rs16_verify(const trs_rs16 *rs, const uint8_t *const *shards, const size_t *lens,
                     size_t n_shards, int *ok, void *stream);

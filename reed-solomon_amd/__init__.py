@@ -374,11 +374,47 @@ class ReedSolomon16(_Codec):
         _check(lib.trs_rs16_encode_sep_matrix(self._h, dp, dl, len(data), pp, pl, len(parity),
                                               _stream()))
 
+    def encode_sep_fft(self, data: Sequence, parity: Sequence) -> None:
+        """The tower-FFT route regardless of routing (src/fft16.rs:887-930)."""
+        dp, dl = _ptrs(data)
+        pp, pl = _ptrs(parity)
+        _check(lib.trs_rs16_encode_sep_fft(self._h, dp, dl, len(data), pp, pl, len(parity),
+                                           _stream()))
+
+    ENCODE_ROUTES = ("auto", "matrix", "fft")
+
+    def encode_stripes(self, stripes: Sequence[Sequence], route: str = "auto") -> None:
+        """Encodes many stripes (each k data then m parity tensors of one
+        common even length) in one call; ``route`` as ENCODE_ROUTES."""
+        flat = [t for s in stripes for t in s]
+        n = self.total_shard_count()
+        if any(len(s) != n for s in stripes):
+            raise Error(-3)
+        length = flat[0].numel() if flat else 0
+        if any(t.numel() != length for t in flat):
+            raise Error(ERROR_NAMES.index("IncorrectShardSize"))
+        p, _ = _ptrs(flat)
+        _check(lib.trs_rs16_encode_stripes(self._h, p, len(stripes), length,
+                                           self.ENCODE_ROUTES.index(route), _stream()))
+
     def generator(self) -> np.ndarray:
         n, k = self.total_shard_count(), self.data_shard_count()
         out = np.zeros(n * k, np.uint16)
         _check(lib.trs_rs16_generator(self._h, out.ctypes.data_as(_lib.c_u16p), out.size))
         return out.reshape(n, k)
+
+
+def fft16_program_info(k: int, m: int) -> Optional[dict]:
+    """Host-side statistics of the staged tower-FFT program for (k, m) and of
+    its device lowering; None when the shape has no program."""
+    out = (ctypes.c_size_t * 9)()
+    rc = lib.trs_fft16_program_info(k, m, out, 9)
+    if rc == -4:
+        return None
+    _check(rc)
+    keys = ("register_count", "mult_count", "glue", "stages", "zero_regs", "profitable",
+            "device_regs", "device_steps", "group_bytes")
+    return {key: int(v) for key, v in zip(keys, out)}
 
 
 class Batch:

@@ -70,6 +70,13 @@ _sig("trs_rs8_set_apply_route", c_int, c_vp, c_int)
 _sig("trs_rs8_encode_rows_into", c_int, c_vp, c_vp, c_size, c_size, c_vp, c_size, c_vp)
 _sig("trs_rs16_encode_sep_matrix", c_int, c_vp, c_ptrs, c_sizep, c_size, c_ptrs, c_sizep, c_size,
      c_vp)
+_sig("trs_rs16_encode_sep_fft", c_int, c_vp, c_ptrs, c_sizep, c_size, c_ptrs, c_sizep, c_size,
+     c_vp)
+_sig("trs_rs16_encode_stripes", c_int, c_vp, c_ptrs, c_size, c_size, c_int, c_vp)
+_sig("trs_fft16_program_info", c_int, c_size, c_size, ctypes.POINTER(c_size), c_size)
+c_u32p = ctypes.POINTER(ctypes.c_uint32)
+_sig("trs_fft16_lowered_program", c_int, c_size, c_size, c_u32p, c_size, c_u32p, c_size, c_u32p,
+     c_size, c_u32p, c_size, ctypes.POINTER(c_size))
 for fam in ("dec8", "dec16"):
     _sig(f"trs_{fam}_reconstruct", c_int, c_vp, c_ptrs, c_sizep, c_u8p, c_size, c_int, c_vp)
     _sig(f"trs_{fam}_free", None, c_vp)

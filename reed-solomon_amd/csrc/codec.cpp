@@ -19,6 +19,7 @@
 #include <atomic>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <memory>
 #include <mutex>
 #include <shared_mutex>
@@ -26,6 +27,7 @@
 #include <vector>
 
 #include "../../include/trs_gpu.h"
+#include "fft16.hpp"
 #include "field.hpp"
 #include "kernels.hpp"
 
@@ -353,6 +355,16 @@ struct trs_rs16 {
   Matrix<F16> G;
   DevCoef parity;
   PlanCache plans;
+  // The staged tower-FFT program (src/reedsolomon16.rs:300-308), built for
+  // 2 <= k <= 1024; `staged` when the routing rule picks it.  Past k + m =
+  // 4096 a program has more than 2048 registers and never routes, so none is
+  // built.  The device form is uploaded on first use.
+  bool has_program = false;
+  bool staged = false;
+  Fft16Program program;
+  mutable std::once_flag fft_once;
+  mutable hipError_t fft_status = hipSuccess;
+  mutable Fft16DevicePtr fft;
 };
 
 struct trs_dec16 {
@@ -845,6 +857,11 @@ int trs_rs16_new(size_t k, size_t m, trs_rs16 **out) {
   if (!build_generator<F16>(k, k + m, rs->G)) return TRS_SINGULAR_MATRIX;
   const int st = upload_coef16(rs->parity, rs->G.row(k), m, k);
   if (st) return st;
+  if (k >= 2 && k <= kMaxFftDataShards && k + m <= 4096) {
+    rs->program = build_staged_program16(k, m);
+    rs->has_program = true;
+    rs->staged = fft16_profitable(rs->program, k, m);
+  }
   *out = rs.release();
   return TRS_OK;
 }
@@ -854,8 +871,7 @@ size_t trs_rs16_data_shard_count(const trs_rs16 *rs) { return rs ? rs->k : 0; }
 size_t trs_rs16_parity_shard_count(const trs_rs16 *rs) { return rs ? rs->m : 0; }
 size_t trs_rs16_total_shard_count(const trs_rs16 *rs) { return rs ? rs->n : 0; }
 const char *trs_rs16_encode_route(const trs_rs16 *rs) {
-  (void)rs;
-  return "matrix";
+  return rs && rs->staged ? "fft" : "matrix";
 }
 
 int trs_rs16_generator(const trs_rs16 *rs, uint16_t *out, size_t out_len) {
@@ -865,6 +881,41 @@ int trs_rs16_generator(const trs_rs16 *rs, uint16_t *out, size_t out_len) {
 }
 
 namespace {
+// The device form of the codec's program, uploaded once.
+int fft16_device_for(const trs_rs16 *rs, const Fft16Device **out) {
+  if (!rs->has_program) return TRS_ERR_UNSUPPORTED;
+  std::call_once(rs->fft_once, [rs] { rs->fft_status = fft16_device_new(rs->program, &rs->fft); });
+  if (rs->fft_status == hipErrorInvalidValue) return TRS_ERR_UNSUPPORTED;
+  if (rs->fft_status != hipSuccess) return hip_status(rs->fft_status);
+  *out = rs->fft.get();
+  return TRS_OK;
+}
+
+// Tower-FFT encode of `stripes` stripes; ptrs holds stripes * (k + m) shards.
+int run_fft16(const trs_rs16 *rs, const std::vector<const uint8_t *> &ptrs, size_t stripes,
+              size_t len, hipStream_t stream) {
+  const Fft16Device *d = nullptr;
+  int st = fft16_device_for(rs, &d);
+  if (st) return st;
+  if (len == 0 || stripes == 0) return TRS_OK;
+  bool al = true;
+  for (auto q : ptrs) al = al && aligned16(q);
+  PtrTable table;
+  if ((st = table.upload(ptrs, stream))) return st;
+  Fft16Call c{};
+  c.ptrs = static_cast<const uint8_t *const *>(table.d);
+  c.plane = len / 2;
+  c.stripes = uint32_t(stripes);
+  c.vec_ok = al ? 1u : 0u;
+  return hip_status(launch_fft16(*d, c, stream));
+}
+
+int encode16_ptrs(const trs_rs16 *rs, const std::vector<const uint8_t *> &ptrs, size_t len,
+                  hipStream_t stream) {
+  if (rs->staged) return run_fft16(rs, ptrs, 1, len, stream);
+  return run_gf16(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, stream);
+}
+
 // note_shard_len over a shard list (src/reedsolomon16.rs:758-772, :704-711).
 int check_shards16(const size_t *lens, size_t n, size_t *len_out) {
   size_t len = 0;
@@ -892,7 +943,7 @@ int trs_rs16_encode(const trs_rs16 *rs, uint8_t *const *shards, const size_t *le
   std::vector<const uint8_t *> ptrs(shards, shards + n);
   for (auto q : ptrs)
     if (!q) return TRS_ERR_INVALID_ARGUMENT;
-  return run_gf16(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, as_stream(stream));
+  return encode16_ptrs(rs, ptrs, len, as_stream(stream));
 }
 
 int trs_rs16_encode_sep(const trs_rs16 *rs, const uint8_t *const *data, const size_t *dl,
@@ -911,7 +962,7 @@ int trs_rs16_encode_sep(const trs_rs16 *rs, const uint8_t *const *data, const si
   ptrs.insert(ptrs.end(), parity, parity + np);
   for (auto q : ptrs)
     if (!q) return TRS_ERR_INVALID_ARGUMENT;
-  return run_gf16(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, as_stream(stream));
+  return encode16_ptrs(rs, ptrs, len, as_stream(stream));
 }
 
 int trs_rs16_encode_sep_matrix(const trs_rs16 *rs, const uint8_t *const *data,
@@ -928,6 +979,79 @@ int trs_rs16_encode_sep_matrix(const trs_rs16 *rs, const uint8_t *const *data,
   std::vector<const uint8_t *> ptrs(data, data + nd);
   ptrs.insert(ptrs.end(), parity, parity + np);
   return run_gf16(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, as_stream(stream));
+}
+
+int trs_rs16_encode_sep_fft(const trs_rs16 *rs, const uint8_t *const *data, const size_t *dl,
+                            size_t nd, uint8_t *const *parity, const size_t *pl, size_t np,
+                            void *stream) {
+  if (!rs || (nd && (!data || !dl)) || (np && (!parity || !pl))) return TRS_ERR_INVALID_ARGUMENT;
+  std::vector<size_t> lens(dl, dl + nd);
+  lens.insert(lens.end(), pl, pl + np);
+  size_t len = 0;
+  int st = check_shards16(lens.data(), lens.size(), &len);
+  if (st) return st;
+  if (nd != rs->k || np != rs->m) return TRS_ERR_INVALID_ARGUMENT;
+  std::vector<const uint8_t *> ptrs(data, data + nd);
+  ptrs.insert(ptrs.end(), parity, parity + np);
+  for (auto q : ptrs)
+    if (!q) return TRS_ERR_INVALID_ARGUMENT;
+  return run_fft16(rs, ptrs, 1, len, as_stream(stream));
+}
+
+int trs_fft16_program_info(size_t k, size_t m, size_t *out, size_t n) {
+  if (n && !out) return TRS_ERR_INVALID_ARGUMENT;
+  if (k < 2 || k > kMaxFftDataShards || m == 0 || k + m > 4096) return TRS_ERR_UNSUPPORTED;
+  const Fft16Program p = build_staged_program16(k, m);
+  size_t v[9] = {p.register_count, p.mult_count, p.glue.size(), p.stages.size(),
+                 p.zero_regs.size(), fft16_profitable(p, k, m) ? 1u : 0u, 0, 0, 0};
+  const Fft16Lowered L = fft16_lower(p);
+  v[6] = L.regs;
+  v[7] = L.steps.size();
+  v[8] = L.regs * 64;
+  for (size_t i = 0; i < n && i < 9; ++i) out[i] = v[i];
+  return TRS_OK;
+}
+
+int trs_fft16_lowered_program(size_t k, size_t m, uint32_t *steps, size_t steps_cap,
+                              uint32_t *consts, size_t consts_cap, uint32_t *ops, size_t ops_cap,
+                              uint32_t *parity, size_t parity_cap, size_t *counts) {
+  if (!counts) return TRS_ERR_INVALID_ARGUMENT;
+  if (k < 2 || k > kMaxFftDataShards || m == 0 || k + m > 4096) return TRS_ERR_UNSUPPORTED;
+  const Fft16Lowered L = fft16_lower(build_staged_program16(k, m));
+  counts[0] = L.steps.size() * 8;
+  counts[1] = L.consts.size();
+  counts[2] = L.ops.size();
+  counts[3] = L.parity.size();
+  counts[4] = L.regs;
+  if (steps_cap < counts[0] || consts_cap < counts[1] || ops_cap < counts[2] ||
+      parity_cap < counts[3])
+    return TRS_OK;  // sizes only
+  if (!steps || !consts || !ops || !parity) return TRS_ERR_INVALID_ARGUMENT;
+  std::memcpy(steps, L.steps.data(), counts[0] * sizeof(uint32_t));
+  std::copy(L.consts.begin(), L.consts.end(), consts);
+  std::copy(L.ops.begin(), L.ops.end(), ops);
+  std::copy(L.parity.begin(), L.parity.end(), parity);
+  return TRS_OK;
+}
+
+int trs_rs16_encode_stripes(const trs_rs16 *rs, uint8_t *const *shards, size_t stripes,
+                            size_t len, int route, void *stream) {
+  if (!rs || route < 0 || route > 2 || (stripes && !shards)) return TRS_ERR_INVALID_ARGUMENT;
+  if (stripes == 0) return TRS_OK;
+  if (len == 0) return TRS_EMPTY_SHARD;
+  if (len & 1) return TRS_INCORRECT_SHARD_SIZE;
+  std::vector<const uint8_t *> ptrs(shards, shards + stripes * rs->n);
+  for (auto q : ptrs)
+    if (!q) return TRS_ERR_INVALID_ARGUMENT;
+  hipStream_t s = as_stream(stream);
+  const bool fft = route == 2 || (route == 0 && rs->staged);
+  if (fft) return run_fft16(rs, ptrs, stripes, len, s);
+  for (size_t i = 0; i < stripes; ++i) {
+    std::vector<const uint8_t *> one(ptrs.begin() + i * rs->n, ptrs.begin() + (i + 1) * rs->n);
+    const int st = run_gf16(one, rs->k, rs->m, rs->parity, len, kWrite, nullptr, s);
+    if (st) return st;
+  }
+  return TRS_OK;
 }
 
 int trs_rs16_verify(const trs_rs16 *rs, const uint8_t *const *shards, const size_t *lens,

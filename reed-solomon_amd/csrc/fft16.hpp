@@ -1,0 +1,101 @@
+// Additive-FFT (Lin-Chung-Han) encode for the GF((2^8)^2) tower code.
+//
+// Host side: the staged program builder (a C++ restatement of
+// src/fft16.rs:182-621) and its lowering to the device step list.  Device
+// side (fft16.hip): an interpreter that keeps one workgroup's register file in
+// LDS, bit-sliced (16 planes of 32 symbol columns per register), and runs each
+// program step over all of its columns between barriers.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include <hip/hip_runtime_api.h>
+
+namespace trs {
+
+// src/fft16.rs:30-38
+constexpr size_t kMaxFftDataShards = 1024;
+constexpr size_t kMaxStagedRegisters16 = 2048;
+
+enum Fft16OpKind : uint8_t { kOpXor = 0, kOpCopy = 1, kOpMulXor = 2, kOpMulCopy = 3, kOpZero = 4 };
+
+struct Fft16Op {
+  uint8_t kind;
+  uint16_t dst, src, c;
+};
+
+// The lowered program (src/fft16.rs:138-174), same fields and meaning.
+struct Fft16Program {
+  struct Stage {
+    bool transform;
+    bool inverse;
+    uint32_t start, size, consts_start;  // transform
+    uint32_t glue_start, glue_len;       // glue run
+  };
+  size_t k = 0, m = 0;
+  std::vector<Stage> stages;
+  std::vector<uint16_t> consts;
+  std::vector<uint8_t> consts_skip;
+  std::vector<Fft16Op> glue;
+  std::vector<uint16_t> zero_regs, parity_regs;
+  size_t register_count = 0;
+  size_t mult_count = 0;
+};
+
+// src/fft16.rs:357-376 (cheaper of the two base strategies).  Requires
+// 1 <= k <= kMaxFftDataShards.
+Fft16Program build_staged_program16(size_t k, size_t m);
+// src/fft16.rs:85-106, with the multiply-count rule of x86 hosts with GFNI.
+bool fft16_profitable(const Fft16Program &p, size_t k, size_t m);
+
+// ---- device form ---------------------------------------------------------
+
+// One barrier-separated step.  kind 0: one level of a transform, `count`
+// butterflies at stride 2^b over registers [a, a + 2*count), block constants
+// at consts[c + block], d = inverse.  kind 1: `count` independent glue ops at
+// ops[a].
+struct Fft16Step {
+  uint32_t kind, count, a, b, c, d, pad0, pad1;
+};
+
+// Per-constant word: Karatsuba bytes [c_lo, c_hi ^ c_lo, c_hi * L] in bytes
+// 0..2 (src/fft16.rs:176-180), flags in byte 3.
+constexpr uint32_t kConstSkip = 1u << 24;  // block is a dead subtree
+constexpr uint32_t kConstZero = 2u << 24;  // c == 0: butterfly without multiply
+constexpr uint32_t kConstOne = 4u << 24;   // c == 1
+
+struct Fft16Device;  // device buffers of one lowered program
+void fft16_device_free(Fft16Device *d);
+struct Fft16DeviceDeleter {
+  void operator()(Fft16Device *d) const { fft16_device_free(d); }
+};
+using Fft16DevicePtr = std::unique_ptr<Fft16Device, Fft16DeviceDeleter>;
+
+struct Fft16Call {
+  const uint8_t *const *ptrs;  // device [stripes][k + m]: data then parity shards
+  uint64_t plane;              // symbols per shard (bytes per plane)
+  uint32_t stripes;
+  uint32_t vec_ok;             // every shard pointer 16-byte aligned
+};
+
+// Lowers p for the device (glue scheduled into independent sub-steps, whole
+// range copies coalesced) and uploads it.  hipErrorInvalidValue: the register
+// file does not fit one workgroup's LDS.
+hipError_t fft16_device_new(const Fft16Program &p, Fft16DevicePtr *out);
+// p lowered for the device (host only): glue scheduled into independent
+// sub-steps, whole-range copies coalesced, registers renumbered.  ops holds
+// two words per op: dst | src << 16, kind | Karatsuba bytes << 8.
+struct Fft16Lowered {
+  std::vector<Fft16Step> steps;
+  std::vector<uint32_t> consts;
+  std::vector<uint32_t> ops;
+  std::vector<uint32_t> parity;
+  size_t regs = 0, max_count = 0;
+};
+Fft16Lowered fft16_lower(const Fft16Program &p);
+hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t stream);
+
+}  // namespace trs
