@@ -141,6 +141,111 @@ def served_leg(trs, torch, requests: int, rates, seed: int):
     return out
 
 
+TOWER_SEED = 0x7A9E_5EED_0000_0003
+
+
+def tower_leg(trs, torch, O, stripes: int = 64):
+    """RS16 (1024,1024), 4 KiB split-plane shards (2048 symbols), `stripes`
+    stripes resident in HBM: encode through the tower-FFT route and through the
+    matrix route (one multi-stripe call each), then per stripe erase 1024
+    uniformly chosen shards and reconstruct through the matrix route with a
+    decode plan per pattern (first call builds it, the timed pass reuses it)."""
+    lib = trs._lib.lib
+    k = m = 1024
+    n, L = k + m, 4096
+    rs = trs.ReedSolomon16(k, m)
+    buf = torch.empty((stripes, n, L), dtype=torch.uint8, device="cuda")
+    for s in range(stripes):
+        trs.fill_random(buf[s, :k].reshape(-1), TOWER_SEED + s)
+    enc_bytes = stripes * n * L
+    sptr = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def timed(route, reps):
+        rs.encode_stripes(buf, route)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            rs.encode_stripes(buf, route)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    fft_ms = timed("fft", 20)
+    fft_parity = buf[:, k:].clone()
+    matrix_ms = timed("matrix", 2)
+    routes_equal = bool(torch.equal(fft_parity, buf[:, k:]))
+    del fft_parity
+    # oracle on sampled symbol columns of the first and last stripe
+    G = O.build_matrix(16, k, n)
+    cols = np.r_[0:24, 1000:1008, 2040:2048]
+    oracle_ok = True
+    for s in (0, stripes - 1):
+        host = buf[s].cpu().numpy()
+        sub = np.ascontiguousarray(np.concatenate([host[:k][:, cols], host[:k][:, L // 2 + cols]], 1))
+        want = O.rs16_symbolwise(G[k:], list(sub), sub.shape[1])
+        got = np.concatenate([host[k:][:, cols], host[k:][:, L // 2 + cols]], 1)
+        oracle_ok &= bool((got == want).all())
+    pristine = buf.clone()
+
+    rng = np.random.default_rng(TOWER_SEED)
+    base = buf.data_ptr()
+    lens = (ctypes.c_size_t * n)(*([L] * n))
+    calls = []
+    for s in range(stripes):
+        erased = rng.choice(n, 1024, replace=False)
+        flags = np.ones(n, np.uint8)
+        flags[erased] = 0
+        ptrs = (ctypes.c_void_p * n)(*[base + (s * n + i) * L for i in range(n)])
+        calls.append((ptrs, (ctypes.c_uint8 * n)(*flags.tolist()), erased))
+    rec_bytes = stripes * (k + 1024) * L
+
+    def erase_all():
+        for s, (_, _, erased) in enumerate(calls):
+            buf[s, torch.from_numpy(erased).to("cuda")] = 0
+
+    def reconstruct_all():
+        for ptrs, flags, _ in calls:
+            assert lib.trs_rs16_reconstruct(rs._h, ptrs, lens, flags, n, 0, sptr) == 0
+
+    erase_all()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reconstruct_all()  # builds and caches one plan per pattern
+    torch.cuda.synchronize()
+    first_s = time.perf_counter() - t0
+    first_exact = bool(torch.equal(buf, pristine))
+    reps = 2
+    ev = []
+    for _ in range(reps):
+        erase_all()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        reconstruct_all()
+        e1.record()
+        ev.append((e0, e1))
+    torch.cuda.synchronize()
+    rec_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    cached_exact = bool(torch.equal(buf, pristine))
+    del buf, pristine
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"RS16 (1024,1024), {stripes} stripes of 2048 x 4 KiB split-plane shards; "
+                    "encode via tower FFT and via matrix; reconstruct of 1024 uniformly erased "
+                    "shards per stripe (one pattern per stripe) via the matrix route",
+        "encode_bytes": enc_bytes,
+        "encode_fft": {"ms": round(fft_ms, 3), "gbs": round(enc_bytes / fft_ms / 1e6, 1)},
+        "encode_matrix": {"ms": round(matrix_ms, 3), "gbs": round(enc_bytes / matrix_ms / 1e6, 2)},
+        "fft_vs_matrix": round(matrix_ms / fft_ms, 1),
+        "reconstruct_bytes": rec_bytes,
+        "reconstruct_cached_plans": {"ms": round(rec_ms, 3),
+                                     "gbs": round(rec_bytes / rec_ms / 1e6, 2)},
+        "reconstruct_first_call_s": round(first_s, 3),
+        "byte_exact": {"fft_equals_matrix": routes_equal, "oracle_sampled_columns": oracle_ok,
+                       "reconstruct_first": first_exact, "reconstruct_cached": cached_exact},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +256,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=800)
+    ap.add_argument("--no-tower", action="store_true")
+    ap.add_argument("--tower-stripes", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -326,10 +433,14 @@ def main() -> int:
                "sample": f"first {nst} stripes of the workload, encode + reconstruct, "
                          f"scalar C oracle (MUL-table loop), {spent:.1f} s"}
 
+    del buf, pristine
+    torch.cuda.empty_cache()
+    tower = None
+    if not args.no_tower:
+        tower = tower_leg(trs, torch, O, args.tower_stripes)
+
     served = None
     if not args.no_serve:
-        del buf, pristine
-        torch.cuda.empty_cache()
         served = served_leg(trs, torch, args.serve_requests, [1000, 5000, 20000, 50000],
                             SEED + 5)
 
@@ -380,13 +491,17 @@ def main() -> int:
                                        "p99": round(float(np.percentile(lat, 99)), 1)},
         },
         "batched_vs_per_call": round(value / percall_gbs, 2),
+        "tower": tower,
         "served": served,
         "byte_exact": {"batched_reconstruct": rec_exact, "batched_encode": enc_exact,
                        "after_timed_steps": exact_after, "per_call": exact_percall,
                        "oracle_sampled_parity": host_ok},
     }
     print(json.dumps(out))
-    return 0 if (rec_exact and enc_exact and exact_after and exact_percall and host_ok) else 1
+    ok = rec_exact and enc_exact and exact_after and exact_percall and host_ok
+    if tower is not None:
+        ok = ok and all(tower["byte_exact"].values())
+    return 0 if ok else 1
 
 
 if __name__ == "__main__":

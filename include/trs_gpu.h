@@ -186,6 +186,15 @@ int trs_fft16_lowered_program(size_t k, size_t m, uint32_t *steps, size_t steps_
  * merges the launches. */
 int trs_rs16_encode_stripes(const trs_rs16 *rs, uint8_t *const *shards, size_t stripes,
                             size_t len, int route, void *stream);
+/* Host-only: the fused decode rows a reconstruct with flags `present` (n
+ * entries) applies -- rows for the missing data shards, then for the missing
+ * parity shards, k coefficients each (uint8 for field 8, uint16 for field 16),
+ * over the first k present shards.  These are the rows of the reference's
+ * DecodePlan (src/reedsolomon.rs:537-613, src/reedsolomon16.rs:595-640) with
+ * its parity pass (src/reedsolomon.rs:866-889) folded in.  *rows_out gets the
+ * row count; out may be NULL to query it. */
+int trs_decode_matrix(int field, size_t k, size_t n, const uint8_t *present, void *out,
+                      size_t out_len, size_t *rows_out);
 /* ReedSolomon16::verify, src/reedsolomon16.rs:453-505. */
 int trs_rs16_verify(const trs_rs16 *rs, const uint8_t *const *shards, const size_t *lens,
                     size_t n_shards, int *ok, void *stream);

@@ -383,11 +383,24 @@ class ReedSolomon16(_Codec):
 
     ENCODE_ROUTES = ("auto", "matrix", "fft")
 
-    def encode_stripes(self, stripes: Sequence[Sequence], route: str = "auto") -> None:
+    def encode_stripes(self, stripes, route: str = "auto") -> None:
         """Encodes many stripes (each k data then m parity tensors of one
-        common even length) in one call; ``route`` as ENCODE_ROUTES."""
-        flat = [t for s in stripes for t in s]
+        common even length) in one call; ``route`` as ENCODE_ROUTES.
+        ``stripes`` is a list of shard lists, or one uint8 device tensor of
+        shape (stripes, k + m, length) whose last dimension is contiguous."""
         n = self.total_shard_count()
+        if hasattr(stripes, "dim"):
+            t = stripes
+            if t.dim() != 3 or t.shape[1] != n or t.stride(2) != 1 or t.dtype.itemsize != 1:
+                raise Error(-3)
+            idx = np.arange(t.shape[0], dtype=np.uint64)[:, None] * np.uint64(t.stride(0)) + \
+                np.arange(n, dtype=np.uint64)[None, :] * np.uint64(t.stride(1))
+            ptrs = (idx + np.uint64(t.data_ptr())).reshape(-1)
+            arr = ptrs.ctypes.data_as(ctypes.POINTER(ctypes.c_void_p))
+            _check(lib.trs_rs16_encode_stripes(self._h, arr, t.shape[0], t.shape[2],
+                                               self.ENCODE_ROUTES.index(route), _stream()))
+            return
+        flat = [t for s in stripes for t in s]
         if any(len(s) != n for s in stripes):
             raise Error(-3)
         length = flat[0].numel() if flat else 0
@@ -402,6 +415,20 @@ class ReedSolomon16(_Codec):
         out = np.zeros(n * k, np.uint16)
         _check(lib.trs_rs16_generator(self._h, out.ctypes.data_as(_lib.c_u16p), out.size))
         return out.reshape(n, k)
+
+
+def decode_matrix(field: int, k: int, present: Sequence[bool]) -> np.ndarray:
+    """The fused decode rows a reconstruct with these presence flags applies
+    (missing data rows, then missing parity rows, over the first k present
+    shards); computed on the host."""
+    n = len(present)
+    rows = ctypes.c_size_t(0)
+    flags = _flags(present)
+    _check(lib.trs_decode_matrix(field, k, n, flags, None, 0, ctypes.byref(rows)))
+    out = np.zeros((rows.value, k), np.uint8 if field == 8 else np.uint16)
+    _check(lib.trs_decode_matrix(field, k, n, flags, out.ctypes.data_as(ctypes.c_void_p), out.size,
+                                 ctypes.byref(rows)))
+    return out
 
 
 def fft16_program_info(k: int, m: int) -> Optional[dict]:

@@ -74,6 +74,8 @@ _sig("trs_rs16_encode_sep_fft", c_int, c_vp, c_ptrs, c_sizep, c_size, c_ptrs, c_
      c_vp)
 _sig("trs_rs16_encode_stripes", c_int, c_vp, c_ptrs, c_size, c_size, c_int, c_vp)
 _sig("trs_fft16_program_info", c_int, c_size, c_size, ctypes.POINTER(c_size), c_size)
+_sig("trs_decode_matrix", c_int, c_int, c_size, c_size, c_u8p, c_vp, c_size,
+     ctypes.POINTER(c_size))
 c_u32p = ctypes.POINTER(ctypes.c_uint32)
 _sig("trs_fft16_lowered_program", c_int, c_size, c_size, c_u32p, c_size, c_u32p, c_size, c_u32p,
      c_size, c_u32p, c_size, ctypes.POINTER(c_size))

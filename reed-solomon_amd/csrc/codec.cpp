@@ -254,24 +254,43 @@ void split_indices(const Mask &m, size_t k, size_t n, Plan &p) {
   }
 }
 
+// Fused decode rows for the missing shards of plan p: row for missing index e
+// is V_e * inv(V_P), P the first k present indices.  With G = V * inv(V_top)
+// (src/reedsolomon.rs:154-159) the reference's rows are exactly these: the
+// missing-data rows of inv(G_P) = V_top * inv(V_P) (src/reedsolomon.rs:
+// 539-571) and the parity rows G_e * inv(G_P) = V_e * inv(V_P) of its second
+// pass (:866-889).  V_e * inv(V_P) is the Lagrange basis over the nodes P
+// evaluated at e, L_j(e) = prod_l (e - p_l) * w_j / (e - p_j) with barycentric
+// weights w_j = 1 / prod_{l != j} (p_j - p_l), so the matrix comes out in
+// O(k (k + e)) field operations instead of a k x k Gauss-Jordan.  The inverse is
+// unique, so the rows are identical to the elimination's.
 template <class F>
 bool fused_rows(const Matrix<F> &G, size_t k, const Plan &p, std::vector<typename F::T> &rows) {
-  Matrix<F> sub(k, k), inv;
-  for (size_t r = 0; r < k; ++r) std::copy(G.row(p.valid[r]), G.row(p.valid[r]) + k, sub.row(r));
-  if (!invert(sub, inv)) return false;
-  const size_t nd = p.missing_data.size(), np = p.missing_parity.size();
-  rows.assign((nd + np) * k, 0);
-  for (size_t d = 0; d < nd; ++d)
-    std::copy(inv.row(p.missing_data[d]), inv.row(p.missing_data[d]) + k, rows.data() + d * k);
-  for (size_t q = 0; q < np; ++q) {
-    const auto *g = G.row(p.missing_parity[q]);
-    auto *o = rows.data() + (nd + q) * k;
-    for (size_t i = 0; i < k; ++i) {
-      if (!g[i]) continue;
-      const auto *ir = inv.row(i);
-      for (size_t c = 0; c < k; ++c) o[c] ^= F::mul(g[i], ir[c]);
+  using T = typename F::T;
+  (void)G;
+  const std::vector<uint32_t> &nodes = p.valid;
+  if (nodes.size() != k) return false;
+  std::vector<uint32_t> targets(p.missing_data);
+  targets.insert(targets.end(), p.missing_parity.begin(), p.missing_parity.end());
+  std::vector<T> w(k);
+  parallel_rows(k, k, [&](size_t j0, size_t j1) {
+    for (size_t j = j0; j < j1; ++j) {
+      T d = 1;
+      for (size_t l = 0; l < k; ++l)
+        if (l != j) d = F::mul(d, T(nodes[j] ^ nodes[l]));
+      w[j] = F::inv(d);
     }
-  }
+  });
+  rows.assign(targets.size() * k, 0);
+  parallel_rows(targets.size(), 3 * k, [&](size_t t0, size_t t1) {
+    for (size_t t = t0; t < t1; ++t) {
+      const uint32_t e = targets[t];
+      T lam = 1;
+      for (size_t l = 0; l < k; ++l) lam = F::mul(lam, T(e ^ nodes[l]));
+      T *o = rows.data() + t * k;
+      for (size_t j = 0; j < k; ++j) o[j] = F::mul(F::mul(lam, w[j]), F::inv(T(e ^ nodes[j])));
+    }
+  });
   return true;
 }
 
@@ -1031,6 +1050,32 @@ int trs_fft16_lowered_program(size_t k, size_t m, uint32_t *steps, size_t steps_
   std::copy(L.consts.begin(), L.consts.end(), consts);
   std::copy(L.ops.begin(), L.ops.end(), ops);
   std::copy(L.parity.begin(), L.parity.end(), parity);
+  return TRS_OK;
+}
+
+int trs_decode_matrix(int field, size_t k, size_t n, const uint8_t *present, void *out,
+                      size_t out_len, size_t *rows_out) {
+  if (!present || !rows_out || (field != 8 && field != 16) || k == 0 || n <= k)
+    return TRS_ERR_INVALID_ARGUMENT;
+  if (n > (field == 8 ? 256u : 65536u)) return TRS_TOO_MANY_SHARDS;
+  Mask m = make_mask(present, n);
+  size_t count = 0;
+  for (size_t i = 0; i < n; ++i) count += present[i] != 0;
+  if (count < k) return TRS_TOO_FEW_SHARDS_PRESENT;
+  Plan p;
+  split_indices(m, k, n, p);
+  const size_t nrows = p.missing_data.size() + p.missing_parity.size();
+  *rows_out = nrows;
+  if (!out || out_len < nrows * k) return out ? TRS_ERR_INVALID_ARGUMENT : TRS_OK;
+  if (field == 8) {
+    std::vector<uint8_t> rows;
+    fused_rows(Matrix<F8>(), k, p, rows);
+    std::copy(rows.begin(), rows.end(), static_cast<uint8_t *>(out));
+  } else {
+    std::vector<uint16_t> rows;
+    fused_rows(Matrix<F16>(), k, p, rows);
+    std::copy(rows.begin(), rows.end(), static_cast<uint16_t *>(out));
+  }
   return TRS_OK;
 }
 

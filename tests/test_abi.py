@@ -5,6 +5,7 @@ src/reedsolomon16.rs:282-290).  No compute calls: there is no GPU here."""
 import ctypes
 import re
 
+import numpy as np
 import pytest
 
 
@@ -54,3 +55,47 @@ def test_slice_length_mismatch_is_rejected_before_device(trs):
     # returns TRS_ERR_INVALID_ARGUMENT without touching the device
     rc = trs._lib.lib.trs_gf8_mul_slice(ctypes.c_void_p(16), 10, ctypes.c_void_p(32), 11, 3, 0, None)
     assert rc == -3
+
+
+def _gauss_jordan_rows(field, k, n, present):
+    """The reference's plan rows by its own steps: invert the sub-generator of
+    the first k present rows (src/reedsolomon.rs:537-571), missing-data rows of
+    the inverse, then parity rows times the inverse (:866-889)."""
+    import oracle as O
+
+    G = O.build_matrix(field, k, n)
+    valid = [i for i in range(n) if present[i]][:k]
+    rc, inv = O.invert(field, np.ascontiguousarray(G[valid]))
+    assert rc == 0
+    data = [i for i in range(k) if not present[i]]
+    par = [i for i in range(k, n) if not present[i]]
+    rows = [inv[d] for d in data]
+    if par:
+        rows += list(O.multiply(field, np.ascontiguousarray(G[par]), inv))
+    return np.array(rows).reshape(len(rows), k)
+
+
+@pytest.mark.parametrize("field,k,m,seed", [
+    (8, 10, 4, 1), (8, 7, 13, 2), (8, 32, 32, 3), (8, 200, 55, 4), (8, 1, 3, 5),
+    (16, 4, 2, 6), (16, 17, 33, 7), (16, 200, 100, 8), (16, 300, 700, 9)])
+def test_decode_matrix_equals_gauss_jordan(trs, field, k, m, seed):
+    n = k + m
+    rng = np.random.default_rng(seed)
+    for _ in range(3):
+        e = int(rng.integers(1, m + 1))
+        erased = set(rng.choice(n, e, replace=False).tolist())
+        present = [i not in erased for i in range(n)]
+        got = trs.decode_matrix(field, k, present)
+        assert (got == _gauss_jordan_rows(field, k, n, present)).all(), (field, k, m, erased)
+
+
+def test_decode_matrix_c3_pattern(trs):
+    # (1024,1024) with 1024 erasures: the per-pattern plan of the tower
+    # workload, against the oracle's 1024 x 1024 Gauss-Jordan.
+    k = m = 1024
+    rng = np.random.default_rng(0xC3)
+    erased = set(rng.choice(k + m, 1024, replace=False).tolist())
+    present = [i not in erased for i in range(k + m)]
+    got = trs.decode_matrix(16, k, present)
+    assert got.shape == (1024, 1024)
+    assert (got == _gauss_jordan_rows(16, k, k + m, present)).all()

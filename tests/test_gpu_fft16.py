@@ -107,7 +107,8 @@ def test_c3_workload_fft_equals_matrix(trs):
     cols = np.r_[0:40, 2000:2048]  # symbol columns, both plane halves
     host = buf.cpu().numpy()
     for s in (0, stripes - 1):
-        sub = np.concatenate([host[s, :k][:, cols], host[s, :k][:, L // 2 + cols]], axis=1)
+        sub = np.ascontiguousarray(
+            np.concatenate([host[s, :k][:, cols], host[s, :k][:, L // 2 + cols]], axis=1))
         want = O.rs16_symbolwise(G[k:], list(sub), sub.shape[1])
         got = np.concatenate([host[s, k:][:, cols], host[s, k:][:, L // 2 + cols]], axis=1)
         assert (got == want).all(), s
