@@ -148,9 +148,10 @@ def tower_leg(trs, torch, O, stripes: int = 64):
     """RS16 (1024,1024), 4 KiB split-plane shards (2048 symbols), `stripes`
     stripes resident in HBM: encode through the tower-FFT route and through the
     matrix route (one multi-stripe call each), then per stripe erase 1024
-    uniformly chosen shards and reconstruct through the matrix route with a
-    decode plan per pattern (first call builds it, the timed pass reuses it)."""
-    lib = trs._lib.lib
+    uniformly chosen shards and reconstruct all stripes in one call with a
+    decode plan per pattern (first call builds them, timed passes reuse them),
+    through the additive-FFT erasure decoder (one launch for all stripes) and
+    through the matrix route (one apply per stripe)."""
     k = m = 1024
     n, L = k + m, 4096
     rs = trs.ReedSolomon16(k, m)
@@ -158,7 +159,6 @@ def tower_leg(trs, torch, O, stripes: int = 64):
     for s in range(stripes):
         trs.fill_random(buf[s, :k].reshape(-1), TOWER_SEED + s)
     enc_bytes = stripes * n * L
-    sptr = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
     def timed(route, reps):
         rs.encode_stripes(buf, route)
@@ -189,60 +189,59 @@ def tower_leg(trs, torch, O, stripes: int = 64):
     pristine = buf.clone()
 
     rng = np.random.default_rng(TOWER_SEED)
-    base = buf.data_ptr()
-    lens = (ctypes.c_size_t * n)(*([L] * n))
-    calls = []
+    present = np.ones((stripes, n), bool)
     for s in range(stripes):
-        erased = rng.choice(n, 1024, replace=False)
-        flags = np.ones(n, np.uint8)
-        flags[erased] = 0
-        ptrs = (ctypes.c_void_p * n)(*[base + (s * n + i) * L for i in range(n)])
-        calls.append((ptrs, (ctypes.c_uint8 * n)(*flags.tolist()), erased))
+        present[s, rng.choice(n, 1024, replace=False)] = False
+    erased_mask = torch.from_numpy(~present).to("cuda")
     rec_bytes = stripes * (k + 1024) * L
 
     def erase_all():
-        for s, (_, _, erased) in enumerate(calls):
-            buf[s, torch.from_numpy(erased).to("cuda")] = 0
+        buf[erased_mask] = 0
 
     def reconstruct_all():
-        for ptrs, flags, _ in calls:
-            assert lib.trs_rs16_reconstruct(rs._h, ptrs, lens, flags, n, 0, sptr) == 0
+        rs.reconstruct_stripes(buf, present)
 
-    erase_all()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    reconstruct_all()  # builds and caches one plan per pattern
-    torch.cuda.synchronize()
-    first_s = time.perf_counter() - t0
-    first_exact = bool(torch.equal(buf, pristine))
-    reps = 2
-    ev = []
-    for _ in range(reps):
+    rec = {}
+    for route in ("fft", "matrix"):
+        rs.set_decode_route(route)
         erase_all()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        reconstruct_all()
-        e1.record()
-        ev.append((e0, e1))
-    torch.cuda.synchronize()
-    rec_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-    cached_exact = bool(torch.equal(buf, pristine))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reconstruct_all()  # builds (first route) and caches one plan per pattern
+        torch.cuda.synchronize()
+        first_s = time.perf_counter() - t0
+        first_exact = bool(torch.equal(buf, pristine))
+        ev = []
+        for _ in range(3 if route == "fft" else 2):
+            erase_all()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            reconstruct_all()
+            e1.record()
+            ev.append((e0, e1))
+        torch.cuda.synchronize()
+        ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+        rec[route] = {"ms": round(ms, 3), "gbs": round(rec_bytes / ms / 1e6, 2),
+                      "first_call_s": round(first_s, 3), "first_exact": first_exact,
+                      "cached_exact": bool(torch.equal(buf, pristine))}
     del buf, pristine
     torch.cuda.empty_cache()
     return {
         "workload": f"RS16 (1024,1024), {stripes} stripes of 2048 x 4 KiB split-plane shards; "
                     "encode via tower FFT and via matrix; reconstruct of 1024 uniformly erased "
-                    "shards per stripe (one pattern per stripe) via the matrix route",
+                    "shards per stripe (one pattern per stripe) via the FFT decoder and via "
+                    "the matrix route",
         "encode_bytes": enc_bytes,
         "encode_fft": {"ms": round(fft_ms, 3), "gbs": round(enc_bytes / fft_ms / 1e6, 1)},
         "encode_matrix": {"ms": round(matrix_ms, 3), "gbs": round(enc_bytes / matrix_ms / 1e6, 2)},
         "fft_vs_matrix": round(matrix_ms / fft_ms, 1),
         "reconstruct_bytes": rec_bytes,
-        "reconstruct_cached_plans": {"ms": round(rec_ms, 3),
-                                     "gbs": round(rec_bytes / rec_ms / 1e6, 2)},
-        "reconstruct_first_call_s": round(first_s, 3),
+        "reconstruct_fft": {key: v for key, v in rec["fft"].items() if "exact" not in key},
+        "reconstruct_matrix": {key: v for key, v in rec["matrix"].items() if "exact" not in key},
+        "reconstruct_fft_vs_matrix": round(rec["matrix"]["ms"] / rec["fft"]["ms"], 1),
         "byte_exact": {"fft_equals_matrix": routes_equal, "oracle_sampled_columns": oracle_ok,
-                       "reconstruct_first": first_exact, "reconstruct_cached": cached_exact},
+                       **{f"reconstruct_{r}_{w}": rec[r][f"{w}_exact"]
+                          for r in ("fft", "matrix") for w in ("first", "cached")}},
     }
 
 

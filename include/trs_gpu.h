@@ -195,6 +195,32 @@ int trs_rs16_encode_stripes(const trs_rs16 *rs, uint8_t *const *shards, size_t s
  * row count; out may be NULL to query it. */
 int trs_decode_matrix(int field, size_t k, size_t n, const uint8_t *present, void *out,
                       size_t out_len, size_t *rows_out);
+/* Host-only dump of the additive-FFT erasure decoder the tower codec runs for
+ * presence flags `present` (n entries, k data shards); no reference
+ * counterpart, tests replay it on the CPU.  Layout as
+ * trs_fft16_lowered_program; counts[0..6): step words, constant words, op
+ * words, loaded registers (pointer slots 0..), stored registers, register
+ * count.  TRS_ERR_UNSUPPORTED past 2048 shards. */
+int trs_fft16_decode_program(size_t k, size_t n, const uint8_t *present, uint32_t *steps,
+                             size_t steps_cap, uint32_t *consts, size_t consts_cap, uint32_t *ops,
+                             size_t ops_cap, uint32_t *load, size_t load_cap, uint32_t *store,
+                             size_t store_cap, size_t *counts);
+/* Reconstruct route of the tower codec: 0 auto, 1 matrix (the reference's
+ * decode plans, src/reedsolomon16.rs:595-640), 2 additive-FFT erasure decoder
+ * (no reference counterpart; byte-identical, since the erased symbols are
+ * unique).  Auto takes the FFT when k * (shards rebuilt) >= N log2 N, N the
+ * power of two >= k + m, and k + m <= 2048.  Route 2 past 2048 shards:
+ * TRS_ERR_UNSUPPORTED.  Prepared decoders keep the route they were made with. */
+int trs_rs16_set_decode_route(trs_rs16 *rs, int route);
+/* Reconstruct of `stripes` stripes of the codec's shape in one call, each with
+ * its own presence flags: shards holds stripes * (k + m) pointers, present
+ * stripes * (k + m) flags, every shard `len` bytes.  Per stripe this is
+ * ReedSolomon16::reconstruct / reconstruct_data (src/reedsolomon16.rs:508-515)
+ * with the same checks; every stripe is validated and planned before any is
+ * rebuilt, and stripes on the FFT decode route run in one launch. */
+int trs_rs16_reconstruct_stripes(const trs_rs16 *rs, uint8_t *const *shards,
+                                 const uint8_t *present, size_t stripes, size_t len, int data_only,
+                                 void *stream);
 /* ReedSolomon16::verify, src/reedsolomon16.rs:453-505. */
 int trs_rs16_verify(const trs_rs16 *rs, const uint8_t *const *shards, const size_t *lens,
                     size_t n_shards, int *ok, void *stream);

@@ -382,6 +382,30 @@ class ReedSolomon16(_Codec):
                                            _stream()))
 
     ENCODE_ROUTES = ("auto", "matrix", "fft")
+    DECODE_ROUTES = ("auto", "matrix", "fft")
+
+    def set_decode_route(self, route: str) -> None:
+        """Reconstruct route: "matrix" (the reference's decode plans), "fft"
+        (additive-FFT erasure decoder) or "auto"."""
+        _check(lib.trs_rs16_set_decode_route(self._h, self.DECODE_ROUTES.index(route)))
+
+    def reconstruct_stripes(self, stripes, present, data_only: bool = False) -> None:
+        """Rebuilds the missing shards of many stripes in one call: ``stripes``
+        is a uint8 device tensor (stripes, k + m, length) with a contiguous last
+        dimension, ``present`` a (stripes, k + m) boolean array."""
+        t = stripes
+        n = self.total_shard_count()
+        if t.dim() != 3 or t.shape[1] != n or t.stride(2) != 1 or t.dtype.itemsize != 1:
+            raise Error(-3)
+        flags = np.ascontiguousarray(np.asarray(present, dtype=bool), dtype=np.uint8)
+        if flags.shape != (t.shape[0], n):
+            raise Error(-3)
+        idx = np.arange(t.shape[0], dtype=np.uint64)[:, None] * np.uint64(t.stride(0)) + \
+            np.arange(n, dtype=np.uint64)[None, :] * np.uint64(t.stride(1))
+        ptrs = (idx + np.uint64(t.data_ptr())).reshape(-1)
+        _check(lib.trs_rs16_reconstruct_stripes(
+            self._h, ptrs.ctypes.data_as(ctypes.POINTER(ctypes.c_void_p)),
+            flags.ctypes.data_as(_lib.c_u8p), t.shape[0], t.shape[2], int(data_only), _stream()))
 
     def encode_stripes(self, stripes, route: str = "auto") -> None:
         """Encodes many stripes (each k data then m parity tensors of one

@@ -73,12 +73,16 @@ _sig("trs_rs16_encode_sep_matrix", c_int, c_vp, c_ptrs, c_sizep, c_size, c_ptrs,
 _sig("trs_rs16_encode_sep_fft", c_int, c_vp, c_ptrs, c_sizep, c_size, c_ptrs, c_sizep, c_size,
      c_vp)
 _sig("trs_rs16_encode_stripes", c_int, c_vp, c_ptrs, c_size, c_size, c_int, c_vp)
+_sig("trs_rs16_set_decode_route", c_int, c_vp, c_int)
+_sig("trs_rs16_reconstruct_stripes", c_int, c_vp, c_ptrs, c_u8p, c_size, c_size, c_int, c_vp)
 _sig("trs_fft16_program_info", c_int, c_size, c_size, ctypes.POINTER(c_size), c_size)
 _sig("trs_decode_matrix", c_int, c_int, c_size, c_size, c_u8p, c_vp, c_size,
      ctypes.POINTER(c_size))
 c_u32p = ctypes.POINTER(ctypes.c_uint32)
 _sig("trs_fft16_lowered_program", c_int, c_size, c_size, c_u32p, c_size, c_u32p, c_size, c_u32p,
      c_size, c_u32p, c_size, ctypes.POINTER(c_size))
+_sig("trs_fft16_decode_program", c_int, c_size, c_size, c_u8p, c_u32p, c_size, c_u32p, c_size,
+     c_u32p, c_size, c_u32p, c_size, c_u32p, c_size, ctypes.POINTER(c_size))
 for fam in ("dec8", "dec16"):
     _sig(f"trs_{fam}_reconstruct", c_int, c_vp, c_ptrs, c_sizep, c_u8p, c_size, c_int, c_vp)
     _sig(f"trs_{fam}_free", None, c_vp)

@@ -208,7 +208,40 @@ struct Plan {
   DevCoef coef;                          // [missing data rows; missing parity rows] x k
   std::vector<uint8_t> rows8;            // host copy (GF(2^8) plans)
   DevBuf frag;                           // MFMA A fragments of the same rows (k <= 32)
+  // Tower-FFT erasure decoder of this pattern (RS16), built on first use:
+  // loads the valid shards, stores [missing data; missing parity].
+  std::once_flag fft_once;
+  hipError_t fft_status = hipSuccess;
+  Fft16DevicePtr fft;
 };
+
+size_t log2_ceil(size_t n) {
+  size_t l = 0;
+  while ((size_t(1) << l) < n) ++l;
+  return l;
+}
+
+// Decode route for a tower reconstruct: 1 = matrix, 2 = FFT, 0 = FFT when the
+// matrix apply (k * nout products per symbol) costs more than one transform
+// over the N = 2^ceil(log2 n) points (N log2 N) and the shape fits.
+bool use_fft_decode(int route, size_t n, size_t k, size_t nout) {
+  if (route == 1 || nout == 0 || n > kMaxDecodePoints) return false;
+  if (route == 2) return true;
+  const size_t lg = std::max<size_t>(1, log2_ceil(n));
+  return k * nout >= (size_t(1) << lg) * lg;
+}
+
+int plan_fft16(Plan &p, size_t n, const Fft16Device **out) {
+  std::call_once(p.fft_once, [&p, n] {
+    std::vector<uint32_t> outputs(p.missing_data);
+    outputs.insert(outputs.end(), p.missing_parity.begin(), p.missing_parity.end());
+    p.fft_status = fft16_device_upload(fft16_decode_lowered(n, p.valid, outputs), &p.fft);
+  });
+  if (p.fft_status == hipErrorInvalidValue) return TRS_ERR_UNSUPPORTED;
+  if (p.fft_status != hipSuccess) return hip_status(p.fft_status);
+  *out = p.fft.get();
+  return TRS_OK;
+}
 
 int upload_frags(DevBuf &dst, const uint8_t *rows, size_t nout, size_t k) {
   if (!mfma_available(uint32_t(k)) || nout == 0) return TRS_OK;
@@ -374,6 +407,7 @@ struct trs_rs16 {
   Matrix<F16> G;
   DevCoef parity;
   PlanCache plans;
+  std::atomic<int> decode_route{0};  // reconstruct: 0 auto, 1 matrix, 2 tower FFT
   // The staged tower-FFT program (src/reedsolomon16.rs:300-308), built for
   // 2 <= k <= 1024; `staged` when the routing rule picks it.  Past k + m =
   // 4096 a program has more than 2048 registers and never routes, so none is
@@ -388,6 +422,7 @@ struct trs_rs16 {
 
 struct trs_dec16 {
   size_t k = 0, n = 0;
+  int decode_route = 0;  // the codec's decode route when the decoder was prepared
   std::shared_ptr<Plan> plan;
 };
 
@@ -925,8 +960,29 @@ int run_fft16(const trs_rs16 *rs, const std::vector<const uint8_t *> &ptrs, size
   c.ptrs = static_cast<const uint8_t *const *>(table.d);
   c.plane = len / 2;
   c.stripes = uint32_t(stripes);
+  c.nstore = uint32_t(fft16_device_stores(*d));
   c.vec_ok = al ? 1u : 0u;
   return hip_status(launch_fft16(*d, c, stream));
+}
+
+// One launch of a device program over one stripe: ptrs = nload loaded shards
+// then nstore stored shards.
+int run_fft16_program(const Fft16Device &d, const std::vector<const uint8_t *> &ptrs,
+                      size_t nstore, size_t len, hipStream_t stream) {
+  if (len == 0 || nstore == 0) return TRS_OK;
+  if (ptrs.size() != fft16_device_loads(d) + nstore) return TRS_ERR_INVALID_ARGUMENT;
+  bool al = true;
+  for (auto q : ptrs) al = al && aligned16(q);
+  PtrTable table;
+  int st = table.upload(ptrs, stream);
+  if (st) return st;
+  Fft16Call c{};
+  c.ptrs = static_cast<const uint8_t *const *>(table.d);
+  c.plane = len / 2;
+  c.stripes = 1;
+  c.nstore = uint32_t(nstore);
+  c.vec_ok = al ? 1u : 0u;
+  return hip_status(launch_fft16(d, c, stream));
 }
 
 int encode16_ptrs(const trs_rs16 *rs, const std::vector<const uint8_t *> &ptrs, size_t len,
@@ -1040,7 +1096,7 @@ int trs_fft16_lowered_program(size_t k, size_t m, uint32_t *steps, size_t steps_
   counts[0] = L.steps.size() * 8;
   counts[1] = L.consts.size();
   counts[2] = L.ops.size();
-  counts[3] = L.parity.size();
+  counts[3] = L.store.size();
   counts[4] = L.regs;
   if (steps_cap < counts[0] || consts_cap < counts[1] || ops_cap < counts[2] ||
       parity_cap < counts[3])
@@ -1049,7 +1105,7 @@ int trs_fft16_lowered_program(size_t k, size_t m, uint32_t *steps, size_t steps_
   std::memcpy(steps, L.steps.data(), counts[0] * sizeof(uint32_t));
   std::copy(L.consts.begin(), L.consts.end(), consts);
   std::copy(L.ops.begin(), L.ops.end(), ops);
-  std::copy(L.parity.begin(), L.parity.end(), parity);
+  std::copy(L.store.begin(), L.store.end(), parity);
   return TRS_OK;
 }
 
@@ -1077,6 +1133,94 @@ int trs_decode_matrix(int field, size_t k, size_t n, const uint8_t *present, voi
     std::copy(rows.begin(), rows.end(), static_cast<uint16_t *>(out));
   }
   return TRS_OK;
+}
+
+int trs_fft16_decode_program(size_t k, size_t n, const uint8_t *present, uint32_t *steps,
+                             size_t steps_cap, uint32_t *consts, size_t consts_cap, uint32_t *ops,
+                             size_t ops_cap, uint32_t *load, size_t load_cap, uint32_t *store,
+                             size_t store_cap, size_t *counts) {
+  if (!counts || !present || k == 0 || n <= k) return TRS_ERR_INVALID_ARGUMENT;
+  if (n > kMaxDecodePoints) return TRS_ERR_UNSUPPORTED;
+  Plan p;
+  split_indices(make_mask(present, n), k, n, p);
+  if (p.valid.size() < k) return TRS_TOO_FEW_SHARDS_PRESENT;
+  std::vector<uint32_t> outputs(p.missing_data);
+  outputs.insert(outputs.end(), p.missing_parity.begin(), p.missing_parity.end());
+  const Fft16Lowered L = fft16_decode_lowered(n, p.valid, outputs);
+  const size_t c[6] = {L.steps.size() * 8, L.consts.size(), L.ops.size(), L.load.size(),
+                       L.store.size(), L.regs};
+  std::copy(c, c + 6, counts);
+  if (steps_cap < c[0] || consts_cap < c[1] || ops_cap < c[2] || load_cap < c[3] ||
+      store_cap < c[4])
+    return TRS_OK;  // sizes only
+  if (!steps || !consts || !ops || !load || !store) return TRS_ERR_INVALID_ARGUMENT;
+  std::memcpy(steps, L.steps.data(), c[0] * sizeof(uint32_t));
+  std::copy(L.consts.begin(), L.consts.end(), consts);
+  std::copy(L.ops.begin(), L.ops.end(), ops);
+  std::copy(L.load.begin(), L.load.end(), load);
+  std::copy(L.store.begin(), L.store.end(), store);
+  return TRS_OK;
+}
+
+int trs_rs16_reconstruct_stripes(const trs_rs16 *rs, uint8_t *const *shards,
+                                 const uint8_t *present, size_t stripes, size_t len, int data_only,
+                                 void *stream) {
+  if (!rs || (stripes && (!shards || !present))) return TRS_ERR_INVALID_ARGUMENT;
+  if (stripes == 0) return TRS_OK;
+  if (len == 0) return TRS_EMPTY_SHARD;
+  if (len & 1) return TRS_INCORRECT_SHARD_SIZE;
+  const size_t n = rs->n, k = rs->k;
+  const int route = rs->decode_route.load();
+  const std::vector<size_t> lens(n, len);
+  struct Work {
+    std::shared_ptr<Plan> plan;
+    std::vector<const uint8_t *> ptrs;
+    size_t nout = 0;
+    const Fft16Device *fft = nullptr;
+  };
+  // Validate and plan every stripe before launching anything.
+  std::vector<Work> work;
+  for (size_t i = 0; i < stripes; ++i) {
+    Mask mask;
+    size_t count = 0, l = 0;
+    int st = scan_shards(lens.data(), present + i * n, n, true, mask, count, l);
+    if (st) return st;
+    if (count == n) continue;
+    if (count < k) return TRS_TOO_FEW_SHARDS_PRESENT;
+    Work w;
+    if ((st = plan16_for(rs, mask, &w.plan))) return st;
+    if ((st = gather(*w.plan, shards + i * n, lens.data(), k, len, data_only != 0, w.ptrs, w.nout)))
+      return st;
+    if (w.nout == 0) continue;
+    if (use_fft_decode(route, n, k, w.nout) && (st = plan_fft16(*w.plan, n, &w.fft))) return st;
+    work.push_back(std::move(w));
+  }
+  hipStream_t s = as_stream(stream);
+  std::vector<const uint8_t *> all;
+  std::vector<size_t> offset;
+  for (Work &w : work) {
+    if (!w.fft) {
+      const int st = run_gf16(w.ptrs, k, w.nout, w.plan->coef, len, kWrite, nullptr, s);
+      if (st) return st;
+      continue;
+    }
+    offset.push_back(all.size());
+    all.insert(all.end(), w.ptrs.begin(), w.ptrs.end());
+  }
+  if (all.empty()) return TRS_OK;
+  PtrTable table;
+  int st = table.upload(all, s);
+  if (st) return st;
+  std::vector<Fft16JobSpec> jobs;
+  size_t j = 0;
+  for (Work &w : work) {
+    if (!w.fft) continue;
+    bool al = true;
+    for (auto q : w.ptrs) al = al && aligned16(q);
+    jobs.push_back({w.fft, static_cast<const uint8_t *const *>(table.d) + offset[j++], len / 2,
+                    uint32_t(w.nout), al ? 1u : 0u});
+  }
+  return hip_status(launch_fft16_jobs(jobs, s));
 }
 
 int trs_rs16_encode_stripes(const trs_rs16 *rs, uint8_t *const *shards, size_t stripes,
@@ -1121,7 +1265,7 @@ int trs_rs16_verify(const trs_rs16 *rs, const uint8_t *const *shards, const size
 namespace {
 int reconstruct16(const trs_rs16 *rs, size_t k, size_t total, std::shared_ptr<Plan> plan,
                   uint8_t *const *shards, const size_t *lens, const uint8_t *present, size_t n,
-                  int data_only, hipStream_t s) {
+                  int data_only, hipStream_t s, int route) {
   if (n && (!shards || !lens || !present)) return TRS_ERR_INVALID_ARGUMENT;
   if (n != total) return n < total ? TRS_TOO_FEW_SHARDS : TRS_TOO_MANY_SHARDS;
   Mask mask;
@@ -1140,6 +1284,11 @@ int reconstruct16(const trs_rs16 *rs, size_t k, size_t total, std::shared_ptr<Pl
   std::vector<const uint8_t *> ptrs;
   size_t nout = 0;
   if ((st = gather(*plan, shards, lens, k, len, data_only != 0, ptrs, nout))) return st;
+  if (use_fft_decode(route, total, k, nout)) {
+    const Fft16Device *d = nullptr;
+    if ((st = plan_fft16(*plan, total, &d))) return st;
+    return run_fft16_program(*d, ptrs, nout, len, s);
+  }
   return run_gf16(ptrs, k, nout, plan->coef, len, kWrite, nullptr, s);
 }
 }  // namespace
@@ -1148,7 +1297,14 @@ int trs_rs16_reconstruct(const trs_rs16 *rs, uint8_t *const *shards, const size_
                          const uint8_t *present, size_t n, int data_only, void *stream) {
   if (!rs) return TRS_ERR_INVALID_ARGUMENT;
   return reconstruct16(rs, rs->k, rs->n, nullptr, shards, lens, present, n, data_only,
-                       as_stream(stream));
+                       as_stream(stream), rs->decode_route.load());
+}
+
+int trs_rs16_set_decode_route(trs_rs16 *rs, int route) {
+  if (!rs || route < 0 || route > 2) return TRS_ERR_INVALID_ARGUMENT;
+  if (route == 2 && rs->n > kMaxDecodePoints) return TRS_ERR_UNSUPPORTED;
+  rs->decode_route.store(route);
+  return TRS_OK;
 }
 
 int trs_rs16_reconstruct_rows(const trs_rs16 *rs, uint8_t *rows, size_t rows_len, size_t row_len,
@@ -1163,7 +1319,7 @@ int trs_rs16_reconstruct_rows(const trs_rs16 *rs, uint8_t *rows, size_t rows_len
   std::vector<size_t> lens(rs->n, row_len);
   for (size_t i = 0; i < rs->n; ++i) shards[i] = rows + i * row_len;
   return reconstruct16(rs, rs->k, rs->n, nullptr, shards.data(), lens.data(), present, rs->n, 0,
-                       as_stream(stream));
+                       as_stream(stream), rs->decode_route.load());
 }
 
 int trs_rs16_prepare_decode(const trs_rs16 *rs, const uint8_t *present, size_t n,
@@ -1181,6 +1337,7 @@ int trs_rs16_prepare_decode(const trs_rs16 *rs, const uint8_t *present, size_t n
   auto d = std::make_unique<trs_dec16>();
   d->k = rs->k;
   d->n = rs->n;
+  d->decode_route = rs->decode_route.load();
   d->plan = plan;
   *out = d.release();
   return TRS_OK;
@@ -1190,7 +1347,7 @@ int trs_dec16_reconstruct(const trs_dec16 *dec, uint8_t *const *shards, const si
                           const uint8_t *present, size_t n, int data_only, void *stream) {
   if (!dec) return TRS_ERR_INVALID_ARGUMENT;
   return reconstruct16(nullptr, dec->k, dec->n, dec->plan, shards, lens, present, n, data_only,
-                       as_stream(stream));
+                       as_stream(stream), dec->decode_route);
 }
 
 void trs_dec16_free(trs_dec16 *dec) { delete dec; }
@@ -1585,6 +1742,7 @@ struct SrvReq {
   std::vector<const uint8_t *> ptrs16;
   uint64_t len16 = 0;
   uint32_t k16 = 0, nout16 = 0;
+  const Fft16Device *fft16 = nullptr;  // tower-FFT program (encode: the codec's; decode: the plan's)
   uint64_t bytes = 0;                  // algorithmic bytes (k + nout) * len
 };
 
@@ -1685,9 +1843,12 @@ void server_dispatch(trs_server *S) {
       st = trs_batch8_prepare(&slot->batch);
       if (!st) st = trs_batch8_run(&slot->batch, slot->stream);
     }
-    for (auto &q : slot->reqs)
-      if (q.gf16 && !st) st = run_gf16(q.ptrs16, q.k16, q.nout16, *q.coef16, q.len16, kWrite, nullptr,
-                                      slot->stream);
+    for (auto &q : slot->reqs) {
+      if (!q.gf16 || st) continue;
+      st = q.fft16 ? run_fft16_program(*q.fft16, q.ptrs16, q.nout16, q.len16, slot->stream)
+                   : run_gf16(q.ptrs16, q.k16, q.nout16, *q.coef16, q.len16, kWrite, nullptr,
+                              slot->stream);
+    }
     if (!st) st = hip_status(hipEventRecord(slot->done, slot->stream));
     lk.lock();
     slot->launch_status = st;
@@ -1839,6 +2000,7 @@ int trs_server_submit_encode16(trs_server *S, const trs_rs16 *rs, uint8_t *const
   q.k16 = uint32_t(rs->k);
   q.nout16 = uint32_t(rs->m);
   q.bytes = uint64_t(rs->n) * len;
+  if (rs->staged && (st = fft16_device_for(rs, &q.fft16))) return st;
   return server_enqueue(S, std::move(q), ticket);
 }
 
@@ -1867,6 +2029,9 @@ int trs_server_submit_reconstruct16(trs_server *S, const trs_rs16 *rs, uint8_t *
       q.k16 = uint32_t(rs->k);
       q.nout16 = uint32_t(nout);
       q.bytes = uint64_t(rs->k + nout) * len;
+      if (use_fft_decode(rs->decode_route.load(), rs->n, rs->k, nout) &&
+          (st = plan_fft16(*plan, rs->n, &q.fft16)))
+        return st;
     }
   }
   return server_enqueue(S, std::move(q), ticket);

@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -32,14 +33,18 @@
 namespace trs {
 
 struct Fft16Device {
-  size_t k = 0, m = 0, regs = 0;
+  size_t nload = 0, nstore = 0, regs = 0;
   size_t nsteps = 0, max_count = 0;
-  void *mem = nullptr;  // one allocation: steps | consts | ops | parity regs
+  void *mem = nullptr;  // one allocation: steps | consts | ops | load regs | store regs
   const Fft16Step *steps = nullptr;
   const uint32_t *consts = nullptr;
   const uint2 *ops = nullptr;
-  const uint32_t *parity = nullptr;
+  const uint32_t *load = nullptr;
+  const uint32_t *store = nullptr;
 };
+
+size_t fft16_device_loads(const Fft16Device &d) { return d.nload; }
+size_t fft16_device_stores(const Fft16Device &d) { return d.nstore; }
 
 void fft16_device_free(Fft16Device *d) {
   if (!d) return;
@@ -176,68 +181,223 @@ std::pair<uint32_t, uint32_t> encode_op(const Fft16Op &op) {
 
 }  // namespace
 
+namespace {
+
+using OpWords = std::pair<uint32_t, uint32_t>;
+
+void glue_step(Fft16Lowered &L, const std::vector<OpWords> &ops) {
+  if (ops.empty()) return;
+  L.steps.push_back({1, uint32_t(ops.size()), uint32_t(L.ops.size() / 2), 0, 0, 0, 0, 0});
+  for (const auto &o : ops) {
+    L.ops.push_back(o.first);
+    L.ops.push_back(o.second);
+  }
+  L.max_count = std::max(L.max_count, ops.size());
+}
+
+// Schedules a run of glue ops (in program order) into steps of mutually
+// independent ops: no op of a step reads or writes a register that another op
+// of the step writes.
+void schedule_glue(Fft16Lowered &L, const Fft16Op *ops, size_t n) {
+  std::vector<int> lw(L.regs, -1), lr(L.regs, -1);
+  std::vector<std::vector<OpWords>> levels;
+  for (size_t j = 0; j < n; ++j) {
+    const Fft16Op &op = ops[j];
+    const bool reads_dst = op.kind == kOpXor || op.kind == kOpMulXor;
+    const int lvl = std::max({lw[op.src] + 1, lw[op.dst] + 1, lr[op.dst] + 1});
+    if (size_t(lvl) >= levels.size()) levels.resize(lvl + 1);
+    levels[lvl].push_back(encode_op(op));
+    lw[op.dst] = std::max(lw[op.dst], lvl);
+    lr[op.src] = std::max(lr[op.src], lvl);
+    if (reads_dst) lr[op.dst] = std::max(lr[op.dst], lvl);
+  }
+  for (const auto &lv : levels) glue_step(L, lv);
+}
+
+// One step per transform level with at least one live block.  consts[b] /
+// skip[b] run over the transform's blocks in the order of src/fft16.rs:
+// 593-607 (levels low to high for the inverse, high to low forward).
+void emit_transform(Fft16Lowered &L, uint32_t start, uint32_t size, bool inverse,
+                    const uint16_t *consts, const uint8_t *skip) {
+  int levels = 0;
+  while ((size_t(1) << levels) < size) ++levels;
+  uint32_t coff = 0;
+  for (int step = 0; step < levels; ++step) {
+    const int level = inverse ? step : levels - 1 - step;
+    const uint32_t nblocks = size >> (level + 1);
+    bool any = false;
+    for (uint32_t b = 0; b < nblocks; ++b) any |= !skip[coff + b];
+    if (any) {
+      const uint32_t c0 = uint32_t(L.consts.size());
+      for (uint32_t b = 0; b < nblocks; ++b)
+        L.consts.push_back(kara_word(consts[coff + b]) | (skip[coff + b] ? kConstSkip : 0u));
+      L.steps.push_back({0, size / 2, start, uint32_t(level), c0, inverse ? 1u : 0u, 0, 0});
+      L.max_count = std::max<size_t>(L.max_count, size / 2);
+    }
+    coff += nblocks;
+  }
+}
+
+}  // namespace
+
 Fft16Lowered fft16_lower(const Fft16Program &program) {
   Fft16Program p = program;
   coalesce_and_compact(p);
   Fft16Lowered L;
   L.regs = p.register_count;
-  using OpWords = std::pair<uint32_t, uint32_t>;
-  auto glue_step = [&](const std::vector<OpWords> &ops) {
-    if (ops.empty()) return;
-    L.steps.push_back({1, uint32_t(ops.size()), uint32_t(L.ops.size() / 2), 0, 0, 0, 0, 0});
-    for (const auto &o : ops) {
-      L.ops.push_back(o.first);
-      L.ops.push_back(o.second);
-    }
-    L.max_count = std::max(L.max_count, ops.size());
-  };
   {
     std::vector<OpWords> z;
     for (uint16_t r : p.zero_regs) z.push_back(encode_op({kOpZero, r, r, 0}));
-    glue_step(z);
+    glue_step(L, z);
   }
-  std::vector<int> lw(L.regs), lr(L.regs);
   for (const auto &st : p.stages) {
-    if (st.transform) {
-      int levels = 0;
-      while ((size_t(1) << levels) < st.size) ++levels;
-      uint32_t coff = st.consts_start;
-      for (int step = 0; step < levels; ++step) {
-        const int level = st.inverse ? step : levels - 1 - step;
-        const uint32_t nblocks = st.size >> (level + 1);
-        bool any = false;
-        for (uint32_t b = 0; b < nblocks; ++b) any |= !p.consts_skip[coff + b];
-        if (any) {
-          const uint32_t c0 = uint32_t(L.consts.size());
-          for (uint32_t b = 0; b < nblocks; ++b)
-            L.consts.push_back(kara_word(p.consts[coff + b]) |
-                               (p.consts_skip[coff + b] ? kConstSkip : 0u));
-          L.steps.push_back({0, st.size / 2, st.start, uint32_t(level), c0,
-                             st.inverse ? 1u : 0u, 0, 0});
-          L.max_count = std::max<size_t>(L.max_count, st.size / 2);
-        }
-        coff += nblocks;
-      }
-      continue;
-    }
-    // Schedule the run into levels of mutually independent ops (no op in a
-    // level reads or writes a register another op of the level writes).
-    std::fill(lw.begin(), lw.end(), -1);
-    std::fill(lr.begin(), lr.end(), -1);
-    std::vector<std::vector<OpWords>> levels;
-    for (size_t j = st.glue_start; j < size_t(st.glue_start) + st.glue_len; ++j) {
-      const Fft16Op &op = p.glue[j];
-      const bool reads_dst = op.kind == kOpXor || op.kind == kOpMulXor;
-      const int lvl = std::max({lw[op.src] + 1, lw[op.dst] + 1, lr[op.dst] + 1});
-      if (size_t(lvl) >= levels.size()) levels.resize(lvl + 1);
-      levels[lvl].push_back(encode_op(op));
-      lw[op.dst] = std::max(lw[op.dst], lvl);
-      lr[op.src] = std::max(lr[op.src], lvl);
-      if (reads_dst) lr[op.dst] = std::max(lr[op.dst], lvl);
-    }
-    for (const auto &lv : levels) glue_step(lv);
+    if (st.transform)
+      emit_transform(L, st.start, st.size, st.inverse, p.consts.data() + st.consts_start,
+                     p.consts_skip.data() + st.consts_start);
+    else
+      schedule_glue(L, p.glue.data() + st.glue_start, st.glue_len);
   }
-  for (uint16_t r : p.parity_regs) L.parity.push_back(r);
+  for (size_t r = 0; r < p.k; ++r) L.load.push_back(uint32_t(r));
+  for (uint16_t r : p.parity_regs) L.store.push_back(r);
+  return L;
+}
+
+namespace {
+
+// Formal-derivative constants of the normalised basis: D_j is the linear
+// coefficient of W_j / W_j(2^j), i.e. prod_{0<u<2^j} u / prod_{2^j<=u<2^(j+1)} u
+// (the linear coefficient of the subspace polynomial W_j is the product of the
+// nonzero points of its subspace).
+const std::vector<uint16_t> &derivative_consts() {
+  static std::vector<uint16_t> d;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    const Gf16 &f = gf16();
+    for (int j = 0; j < 11; ++j) {
+      uint16_t num = 1, den = 1;
+      for (uint32_t u = 1; u < (1u << j); ++u) num = f.mul(num, uint16_t(u));
+      for (uint32_t u = 1u << j; u < (2u << j); ++u) den = f.mul(den, uint16_t(u));
+      d.push_back(f.mul(num, f.inv(den)));
+    }
+  });
+  return d;
+}
+
+}  // namespace
+
+Fft16Lowered fft16_decode_lowered(size_t n, const std::vector<uint32_t> &valid,
+                                  const std::vector<uint32_t> &outputs) {
+  size_t N = 2;
+  int levels = 1;
+  while (N < n) {
+    N <<= 1;
+    ++levels;
+  }
+  const Gf16 &f = gf16();
+  Fft16Lowered L;
+  L.regs = N;
+  std::vector<uint8_t> known(N, 0);
+  for (uint32_t r : valid) known[r] = 1;
+  std::vector<uint32_t> erased;
+  for (uint32_t r = 0; r < N; ++r)
+    if (!known[r]) erased.push_back(r);
+  // 1. unknown points start at zero (they hold Q = P * Lambda = 0 there)
+  {
+    std::vector<OpWords> z;
+    for (uint32_t r : erased) z.push_back(encode_op({kOpZero, uint16_t(r), uint16_t(r), 0}));
+    glue_step(L, z);
+  }
+  // log Lambda(x) = sum over e of log(x ^ e), mod 2^16 - 1
+  auto log_prod = [&](uint32_t x, bool skip_self) {
+    uint64_t acc = 0;
+    for (uint32_t e : erased) {
+      if (skip_self && e == x) continue;
+      acc += f.log[x ^ e];
+    }
+    return uint32_t(acc % 65535u);
+  };
+  // 2. known values times Lambda(x)
+  {
+    std::vector<uint32_t> lg(valid.size());
+    parallel_rows(valid.size(), erased.size(), [&](size_t a, size_t b) {
+      for (size_t i = a; i < b; ++i) lg[i] = log_prod(valid[i], false);
+    });
+    std::vector<OpWords> mul;
+    for (size_t i = 0; i < valid.size(); ++i) {
+      const uint16_t c = f.exp[lg[i]];
+      if (c != 1) mul.push_back(encode_op({kOpMulCopy, uint16_t(valid[i]), uint16_t(valid[i]), c}));
+    }
+    glue_step(L, mul);
+  }
+  // 3. values on [0, N) to coefficients
+  std::vector<uint16_t> consts;
+  std::vector<uint8_t> skip;
+  auto transform_consts = [&](bool inverse, const std::vector<uint8_t> *live) {
+    consts.clear();
+    skip.clear();
+    for (int step = 0; step < levels; ++step) {
+      const int level = inverse ? step : levels - 1 - step;
+      const size_t stride = size_t(1) << level;
+      for (size_t block = 0; block < N; block += 2 * stride) {
+        consts.push_back(fft16_what(level, block));
+        bool any = live == nullptr;
+        for (size_t j = block; !any && j < block + 2 * stride; ++j) any = (*live)[j] != 0;
+        skip.push_back(!any);
+      }
+    }
+  };
+  transform_consts(true, nullptr);
+  emit_transform(L, 0, uint32_t(N), true, consts.data(), skip.data());
+  // 4. coefficient t += sum over zero bits j of t of D_j * coefficient (t | 2^j)
+  // (the added Q itself vanishes on E).  With s_t = prod_{j in t} D_j and
+  // u_t = s_t q_t this is s_t Q'_t = sum_j u_(t | 2^j): scale by s_t, XOR the
+  // derivative in place in an order that reads every source before it is
+  // overwritten (blocks [i - w, i) ^= [i, i + w) for i = 1 .. N-1, w the lowest
+  // set bit of i), scale back by 1 / s_t.
+  {
+    const auto &D = derivative_consts();
+    std::vector<uint16_t> sc(N, 1);
+    for (size_t t = 1; t < N; ++t) {
+      int j = 0;
+      while (!((t >> j) & 1)) ++j;
+      sc[t] = f.mul(sc[t & (t - 1)], D[j]);
+    }
+    std::vector<OpWords> pre, post;
+    for (size_t t = 0; t < N; ++t) {
+      if (sc[t] == 1) continue;
+      pre.push_back(encode_op({kOpMulCopy, uint16_t(t), uint16_t(t), sc[t]}));
+      post.push_back(encode_op({kOpMulCopy, uint16_t(t), uint16_t(t), f.inv(sc[t])}));
+    }
+    glue_step(L, pre);
+    std::vector<Fft16Op> ops;
+    ops.reserve(size_t(levels) * N / 2);
+    for (size_t i = 1; i < N; ++i) {
+      const size_t w = i & (~i + 1);
+      for (size_t r = 0; r < w; ++r) ops.push_back({kOpXor, uint16_t(i - w + r), uint16_t(i + r), 0});
+    }
+    schedule_glue(L, ops.data(), ops.size());
+    glue_step(L, post);
+  }
+  // 5. coefficients to values, pruned to the blocks that reach an output
+  std::vector<uint8_t> live(N, 0);
+  for (uint32_t e : outputs) live[e] = 1;
+  transform_consts(false, &live);
+  emit_transform(L, 0, uint32_t(N), false, consts.data(), skip.data());
+  // 6. P(e) = Q'(e) / Lambda'(e)
+  {
+    std::vector<uint32_t> lg(outputs.size());
+    parallel_rows(outputs.size(), erased.size(), [&](size_t a, size_t b) {
+      for (size_t i = a; i < b; ++i) lg[i] = log_prod(outputs[i], true);
+    });
+    std::vector<OpWords> mul;
+    for (size_t i = 0; i < outputs.size(); ++i) {
+      const uint16_t c = f.exp[(65535u - lg[i]) % 65535u];
+      if (c != 1) mul.push_back(encode_op({kOpMulCopy, uint16_t(outputs[i]), uint16_t(outputs[i]), c}));
+    }
+    glue_step(L, mul);
+  }
+  L.load = valid;
+  L.store = outputs;
   return L;
 }
 
@@ -319,36 +479,60 @@ struct Fft16Args {
   const Fft16Step *steps;
   const uint32_t *consts;
   const uint2 *ops;
-  const uint32_t *parity;
+  const uint32_t *load;
+  const uint32_t *store;
   uint64_t plane;
-  uint32_t k, m, nsteps, gshift;
+  uint32_t nload, nstore, nsteps, gshift;
   uint32_t groups;   // column groups per shard
   uint32_t cblocks;  // workgroups per stripe
-  uint32_t total;    // workgroups
+  uint32_t total;    // workgroups (single-program launch); first workgroup (job table)
   uint32_t vec_ok;
 };
+
+// XCD-aware order: consecutive workgroups of one XCD take consecutive
+// column blocks, so the 32-byte segments of one 128-byte line meet in one L2.
+__device__ __forceinline__ uint32_t xcd_order(uint32_t wg, uint32_t total) {
+  return (total & 7u) == 0 ? (wg & 7u) * (total >> 3) + (wg >> 3) : wg;
+}
+
+template <int T>
+__device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
+                                            const uint8_t *const *ptr, uint4 *rf);
 
 template <int T>
 __global__ __launch_bounds__(T) void k_fft16(Fft16Args a) {
   extern __shared__ uint4 rf[];
-  const uint32_t G = 1u << a.gshift;
-  // XCD-aware order: consecutive workgroups of one XCD take consecutive
-  // column blocks, so the 32-byte segments of one 128-byte line meet in one L2.
-  uint32_t wg = blockIdx.x;
-  if ((a.total & 7u) == 0) wg = (wg & 7u) * (a.total >> 3) + (wg >> 3);
+  const uint32_t wg = xcd_order(blockIdx.x, a.total);
   const uint32_t stripe = wg / a.cblocks;
   const uint32_t g0 = (wg - stripe * a.cblocks) << a.gshift;
-  const uint8_t *const *ptr = a.ptrs + uint64_t(stripe) * (a.k + a.m);
+  run_program<T>(a, g0, a.ptrs + uint64_t(stripe) * (a.nload + a.nstore), rf);
+}
 
-  for (uint32_t q = threadIdx.x; q < (a.k << a.gshift); q += T) {
-    const uint32_t r = q >> a.gshift, g = q & (G - 1);
+// Many programs (one per job, e.g. one decode pattern per stripe) in one
+// launch: workgroup w runs job wg_job[w] on its column block w - first.
+template <int T>
+__global__ __launch_bounds__(T) void k_fft16_jobs(const Fft16Args *jobs, const uint32_t *wg_job,
+                                                  uint32_t total) {
+  extern __shared__ uint4 rf[];
+  const uint32_t wg = xcd_order(blockIdx.x, total);
+  const Fft16Args a = jobs[wg_job[wg]];
+  run_program<T>(a, (wg - a.total) << a.gshift, a.ptrs, rf);
+}
+
+template <int T>
+__device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
+                                            const uint8_t *const *ptr, uint4 *rf) {
+  const uint32_t G = 1u << a.gshift;
+
+  for (uint32_t q = threadIdx.x; q < (a.nload << a.gshift); q += T) {
+    const uint32_t slot = q >> a.gshift, g = q & (G - 1);
     uint32_t v[16];
     const uint32_t cg = g0 + g;
     if (cg < a.groups) {
       const uint64_t col = uint64_t(cg) * kGroupCols;
       const uint64_t rem = a.plane - col;
       const bool full = a.vec_ok && rem >= kGroupCols;
-      const uint8_t *s = ptr[r];
+      const uint8_t *s = ptr[slot];
       uint32_t lo[8], hi[8];
       load_planes(s + col, full, rem, lo);
       load_planes(s + a.plane + col, full, rem, hi);
@@ -361,7 +545,7 @@ __global__ __launch_bounds__(T) void k_fft16(Fft16Args a) {
 #pragma unroll
       for (int j = 0; j < 16; ++j) v[j] = 0;
     }
-    rf_store(rf, q, v);
+    rf_store(rf, (a.load[slot] << a.gshift) + g, v);
   }
   __syncthreads();
 
@@ -430,16 +614,16 @@ __global__ __launch_bounds__(T) void k_fft16(Fft16Args a) {
     __syncthreads();
   }
 
-  for (uint32_t q = threadIdx.x; q < (a.m << a.gshift); q += T) {
+  for (uint32_t q = threadIdx.x; q < (a.nstore << a.gshift); q += T) {
     const uint32_t j = q >> a.gshift, g = q & (G - 1);
     const uint32_t cg = g0 + g;
     if (cg >= a.groups) continue;
     uint32_t v[16];
-    rf_load(rf, (a.parity[j] << a.gshift) + g, v);
+    rf_load(rf, (a.store[j] << a.gshift) + g, v);
     const uint64_t col = uint64_t(cg) * kGroupCols;
     const uint64_t rem = a.plane - col;
     const bool full = a.vec_ok && rem >= kGroupCols;
-    uint8_t *d = const_cast<uint8_t *>(ptr[a.k + j]);
+    uint8_t *d = const_cast<uint8_t *>(ptr[a.nload + j]);
     uint32_t lo[8], hi[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
@@ -453,39 +637,44 @@ __global__ __launch_bounds__(T) void k_fft16(Fft16Args a) {
 
 }  // namespace
 
-hipError_t fft16_device_new(const Fft16Program &p, Fft16DevicePtr *out) {
-  const Fft16Lowered L = fft16_lower(p);
+hipError_t fft16_device_upload(const Fft16Lowered &L, Fft16DevicePtr *out) {
   if (L.regs * kRegBytes > kLdsMax) return hipErrorInvalidValue;
   auto d = Fft16DevicePtr(new Fft16Device);
-  d->k = p.k;
-  d->m = p.m;
+  d->nload = L.load.size();
+  d->nstore = L.store.size();
   d->regs = L.regs;
   d->nsteps = L.steps.size();
   d->max_count = L.max_count;
-  const size_t s_bytes = L.steps.size() * sizeof(Fft16Step);
-  const size_t c_bytes = L.consts.size() * sizeof(uint32_t);
-  const size_t o_bytes = L.ops.size() * sizeof(uint32_t);
-  const size_t p_bytes = L.parity.size() * sizeof(uint32_t);
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
-  const size_t total = al(s_bytes) + al(c_bytes) + al(o_bytes) + al(p_bytes) + 256;
+  const size_t parts[5] = {L.steps.size() * sizeof(Fft16Step), L.consts.size() * 4,
+                           L.ops.size() * 4, L.load.size() * 4, L.store.size() * 4};
+  const void *src[5] = {L.steps.data(), L.consts.data(), L.ops.data(), L.load.data(),
+                        L.store.data()};
+  size_t total = 256;
+  for (size_t b : parts) total += al(b);
   hipError_t e = hipMalloc(&d->mem, total);
   if (e != hipSuccess) return e;
   std::vector<uint8_t> host(total, 0);
+  uint8_t *dev[5];
   size_t off = 0;
-  auto put = [&](const void *src, size_t n) {
-    std::memcpy(host.data() + off, src, n);
-    const size_t at = off;
-    off += al(n);
-    return static_cast<uint8_t *>(d->mem) + at;
-  };
-  d->steps = reinterpret_cast<const Fft16Step *>(put(L.steps.data(), s_bytes));
-  d->consts = reinterpret_cast<const uint32_t *>(put(L.consts.data(), c_bytes));
-  d->ops = reinterpret_cast<const uint2 *>(put(L.ops.data(), o_bytes));
-  d->parity = reinterpret_cast<const uint32_t *>(put(L.parity.data(), p_bytes));
+  for (int i = 0; i < 5; ++i) {
+    if (parts[i]) std::memcpy(host.data() + off, src[i], parts[i]);
+    dev[i] = static_cast<uint8_t *>(d->mem) + off;
+    off += al(parts[i]);
+  }
+  d->steps = reinterpret_cast<const Fft16Step *>(dev[0]);
+  d->consts = reinterpret_cast<const uint32_t *>(dev[1]);
+  d->ops = reinterpret_cast<const uint2 *>(dev[2]);
+  d->load = reinterpret_cast<const uint32_t *>(dev[3]);
+  d->store = reinterpret_cast<const uint32_t *>(dev[4]);
   e = hipMemcpy(d->mem, host.data(), total, hipMemcpyHostToDevice);
   if (e != hipSuccess) return e;
   *out = std::move(d);
   return hipSuccess;
+}
+
+hipError_t fft16_device_new(const Fft16Program &p, Fft16DevicePtr *out) {
+  return fft16_device_upload(fft16_lower(p), out);
 }
 
 namespace {
@@ -498,6 +687,28 @@ uint32_t choose_gshift(const Fft16Device &d, uint64_t groups) {
 }
 }  // namespace
 
+namespace {
+// Raises the dynamic LDS limit of every interpreter kernel once per device and
+// host thread when a launch needs more than the default 64 KiB.
+hipError_t raise_lds_limit(size_t lds) {
+  if (lds <= 65536) return hipSuccess;
+  static thread_local int raised = -1;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || raised == dev) return e;
+  const void *kernels[4] = {reinterpret_cast<const void *>(&k_fft16<512>),
+                            reinterpret_cast<const void *>(&k_fft16<256>),
+                            reinterpret_cast<const void *>(&k_fft16_jobs<512>),
+                            reinterpret_cast<const void *>(&k_fft16_jobs<256>)};
+  for (const void *k : kernels)
+    if ((e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(kLdsMax))) !=
+        hipSuccess)
+      return e;
+  raised = dev;
+  return hipSuccess;
+}
+}  // namespace
+
 hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t stream) {
   if (call.plane == 0 || call.stripes == 0) return hipSuccess;
   const uint64_t groups = (call.plane + kGroupCols - 1) / kGroupCols;
@@ -506,14 +717,16 @@ hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t
   const uint64_t total = cblocks * call.stripes;
   if (groups > 0xffffffffu || total > 0x7fffffffu) return hipErrorInvalidValue;
   Fft16Args a{};
+  if (call.nstore > d.nstore) return hipErrorInvalidValue;
   a.ptrs = call.ptrs;
   a.steps = d.steps;
   a.consts = d.consts;
   a.ops = d.ops;
-  a.parity = d.parity;
+  a.load = d.load;
+  a.store = d.store;
   a.plane = call.plane;
-  a.k = uint32_t(d.k);
-  a.m = uint32_t(d.m);
+  a.nload = uint32_t(d.nload);
+  a.nstore = call.nstore;
   a.nsteps = uint32_t(d.nsteps);
   a.gshift = gs;
   a.groups = uint32_t(groups);
@@ -522,27 +735,72 @@ hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t
   a.vec_ok = (call.vec_ok && call.plane % 16 == 0) ? 1u : 0u;
   const size_t lds = (d.regs * kRegBytes) << gs;
   const bool wide = (d.max_count << gs) >= 512;
-  if (lds > 65536) {
-    // Raise the dynamic LDS limit once per kernel instance and device.
-    static thread_local int raised = -1;
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return e;
-    if (raised != dev) {
-      e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fft16<512>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(kLdsMax));
-      if (e == hipSuccess)
-        e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_fft16<256>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, int(kLdsMax));
-      if (e != hipSuccess) return e;
-      raised = dev;
-    }
-  }
+  hipError_t e = raise_lds_limit(lds);
+  if (e != hipSuccess) return e;
   if (wide)
     hipLaunchKernelGGL(k_fft16<512>, dim3(uint32_t(total)), dim3(512), lds, stream, a);
   else
     hipLaunchKernelGGL(k_fft16<256>, dim3(uint32_t(total)), dim3(256), lds, stream, a);
   return hipGetLastError();
+}
+
+hipError_t launch_fft16_jobs(const std::vector<Fft16JobSpec> &jobs, hipStream_t stream) {
+  std::vector<Fft16Args> table;
+  std::vector<uint32_t> wg_job;
+  size_t lds = 0;
+  bool wide = false;
+  for (const Fft16JobSpec &j : jobs) {
+    const Fft16Device &d = *j.prog;
+    if (j.plane == 0 || j.nstore == 0) continue;
+    if (j.nstore > d.nstore) return hipErrorInvalidValue;
+    const uint64_t groups = (j.plane + kGroupCols - 1) / kGroupCols;
+    const uint32_t gs = choose_gshift(d, groups);
+    const uint64_t cblocks = (groups + (1u << gs) - 1) >> gs;
+    if (groups > 0xffffffffu || wg_job.size() + cblocks > 0x7fffffffu) return hipErrorInvalidValue;
+    Fft16Args a{};
+    a.ptrs = j.ptrs;
+    a.steps = d.steps;
+    a.consts = d.consts;
+    a.ops = d.ops;
+    a.load = d.load;
+    a.store = d.store;
+    a.plane = j.plane;
+    a.nload = uint32_t(d.nload);
+    a.nstore = j.nstore;
+    a.nsteps = uint32_t(d.nsteps);
+    a.gshift = gs;
+    a.groups = uint32_t(groups);
+    a.cblocks = uint32_t(cblocks);
+    a.total = uint32_t(wg_job.size());
+    a.vec_ok = (j.vec_ok && j.plane % 16 == 0) ? 1u : 0u;
+    wg_job.insert(wg_job.end(), cblocks, uint32_t(table.size()));
+    table.push_back(a);
+    lds = std::max(lds, (d.regs * kRegBytes) << gs);
+    wide = wide || (d.max_count << gs) >= 512;
+  }
+  if (table.empty()) return hipSuccess;
+  hipError_t e = raise_lds_limit(lds);
+  if (e != hipSuccess) return e;
+  const size_t tb = table.size() * sizeof(Fft16Args), mb = wg_job.size() * sizeof(uint32_t);
+  const size_t off = (tb + 255) & ~size_t(255);
+  void *mem = nullptr;
+  if ((e = hipMallocAsync(&mem, off + mb, stream)) != hipSuccess) return e;
+  std::vector<uint8_t> host(off + mb);
+  std::memcpy(host.data(), table.data(), tb);
+  std::memcpy(host.data() + off, wg_job.data(), mb);
+  e = hipMemcpyAsync(mem, host.data(), off + mb, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) {
+    const auto *jt = static_cast<const Fft16Args *>(mem);
+    const auto *wj = reinterpret_cast<const uint32_t *>(static_cast<uint8_t *>(mem) + off);
+    const uint32_t total = uint32_t(wg_job.size());
+    if (wide)
+      hipLaunchKernelGGL(k_fft16_jobs<512>, dim3(total), dim3(512), lds, stream, jt, wj, total);
+    else
+      hipLaunchKernelGGL(k_fft16_jobs<256>, dim3(total), dim3(256), lds, stream, jt, wj, total);
+    e = hipGetLastError();
+  }
+  const hipError_t f = hipFreeAsync(mem, stream);
+  return e != hipSuccess ? e : f;
 }
 
 }  // namespace trs

@@ -75,9 +75,10 @@ struct Fft16DeviceDeleter {
 using Fft16DevicePtr = std::unique_ptr<Fft16Device, Fft16DeviceDeleter>;
 
 struct Fft16Call {
-  const uint8_t *const *ptrs;  // device [stripes][k + m]: data then parity shards
+  const uint8_t *const *ptrs;  // device [stripes][nload + nstore]: loaded then stored shards
   uint64_t plane;              // symbols per shard (bytes per plane)
   uint32_t stripes;
+  uint32_t nstore;             // stored registers (a prefix of the program's store list)
   uint32_t vec_ok;             // every shard pointer 16-byte aligned
 };
 
@@ -85,17 +86,51 @@ struct Fft16Call {
 // range copies coalesced) and uploads it.  hipErrorInvalidValue: the register
 // file does not fit one workgroup's LDS.
 hipError_t fft16_device_new(const Fft16Program &p, Fft16DevicePtr *out);
+
 // p lowered for the device (host only): glue scheduled into independent
 // sub-steps, whole-range copies coalesced, registers renumbered.  ops holds
 // two words per op: dst | src << 16, kind | Karatsuba bytes << 8.
+// load[i]: register loaded from pointer slot i; store[j]: register written to
+// slot load.size() + j.
 struct Fft16Lowered {
   std::vector<Fft16Step> steps;
   std::vector<uint32_t> consts;
   std::vector<uint32_t> ops;
-  std::vector<uint32_t> parity;
+  std::vector<uint32_t> load, store;
   size_t regs = 0, max_count = 0;
 };
 Fft16Lowered fft16_lower(const Fft16Program &p);
+
+// WHAT[level][x] (src/fft16.rs:42-67), level < 11.
+uint16_t fft16_what(int level, size_t x);
+
+// Erasure decoding by the additive FFT (formal-derivative method; no reference
+// counterpart, the reference decodes through matrix plans).  The codeword of
+// length n is the evaluation of the data polynomial P (degree < k) at points
+// 0..n-1; with N = next power of two >= n, E = [0, N) minus `valid` (k known
+// points) and Lambda(x) = prod_{e in E} (x - e):
+//   Q = P * Lambda is known at all N points (zero on E), so an inverse
+//   transform gives its coefficients; at e in E, Q'(e) = P(e) Lambda'(e), so a
+//   formal derivative, a forward transform and a scale by 1 / Lambda'(e)
+//   recover P(e) for every requested output e.
+// Registers: one per point (N <= kMaxDecodePoints); load = valid, store =
+// outputs (in the given order).
+constexpr size_t kMaxDecodePoints = 2048;
+Fft16Lowered fft16_decode_lowered(size_t n, const std::vector<uint32_t> &valid,
+                                  const std::vector<uint32_t> &outputs);
+hipError_t fft16_device_upload(const Fft16Lowered &L, Fft16DevicePtr *out);
+size_t fft16_device_loads(const Fft16Device &d);
+size_t fft16_device_stores(const Fft16Device &d);
 hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t stream);
+
+// One stripe of one program; launch_fft16_jobs runs many in one launch.
+struct Fft16JobSpec {
+  const Fft16Device *prog;
+  const uint8_t *const *ptrs;  // device table: nload loaded then nstore stored shards
+  uint64_t plane;
+  uint32_t nstore;
+  uint32_t vec_ok;
+};
+hipError_t launch_fft16_jobs(const std::vector<Fft16JobSpec> &jobs, hipStream_t stream);
 
 }  // namespace trs

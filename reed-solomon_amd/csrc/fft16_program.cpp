@@ -40,6 +40,12 @@ const std::vector<uint16_t> &what_table() {
 
 inline uint16_t what(int level, size_t x) { return what_table()[(size_t(level) << 16) | x]; }
 
+}  // namespace
+
+uint16_t fft16_what(int level, size_t x) { return what(level, x); }
+
+namespace {
+
 inline int log2_exact(size_t n) {
   int l = 0;
   while ((size_t(1) << l) < n) ++l;

@@ -123,13 +123,13 @@ def _kara(cw, x):
     return ((t0 ^ F.gf8_mul(a1, xh ^ xl)) << 8) | (t0 ^ F.gf8_mul(a2, xh))
 
 
-def replay_lowered(trs, k, m, sym):
-    """Runs the device step list the way the kernel does (each step's items in
-    parallel), starting from a register file of stale values."""
-    steps, consts, ops, parity, regs = _lowered(trs, k, m)
-    n = sym.shape[1]
+def replay(steps, consts, ops, load, store, regs, inputs):
+    """Runs a device step list the way the kernel does (each step's items in
+    parallel) from a register file of stale values: inputs[i] (symbol
+    columns) goes to register load[i]; returns the registers of `store`."""
+    n = inputs.shape[1]
     R = np.full((regs, n), 0x5A5A, np.int64)
-    R[:k] = sym
+    R[load] = inputs
     for kind, count, a, b, c, d, _, _ in steps:
         o = np.arange(count)
         if kind == 0:
@@ -151,16 +151,22 @@ def replay_lowered(trs, k, m, sym):
         else:
             w0, w1 = ops[2 * (a + o)], ops[2 * (a + o) + 1]
             dst, src, knd, cw = w0 & 0xFFFF, w0 >> 16, w1 & 0xFF, (w1 >> 8)[:, None]
-            # independent: distinct destinations, no destination read as a source
+            # independent: distinct destinations, no destination read by another op
             assert len(set(dst)) == len(dst)
-            assert not (set(dst) & set(src[knd != K_ZERO]))
+            reads_other = (knd != K_ZERO) & (src != dst)
+            assert not (set(dst) & set(src[reads_other]))
             S = R[src]
             val = np.where((knd == K_MULXOR)[:, None] | (knd == K_MULCOPY)[:, None],
                            _kara(cw, S), S)
             val = np.where((knd == K_ZERO)[:, None], 0, val)
             acc = (knd == K_XOR) | (knd == K_MULXOR)
             R[dst] = np.where(acc[:, None], R[dst] ^ val, val)
-    return R[parity]
+    return R[store]
+
+
+def replay_lowered(trs, k, m, sym):
+    steps, consts, ops, parity, regs = _lowered(trs, k, m)
+    return replay(steps, consts, ops, np.arange(k), parity, regs, sym)
 
 
 @pytest.mark.parametrize("k,m", REFERENCE_SHAPES + EXTRA_SHAPES)
@@ -174,3 +180,59 @@ def test_device_lowering_c3_shape(trs):
     # build takes a few seconds in the C oracle).
     sym = _symbols(0xC3, 1024, 3)
     assert (replay_lowered(trs, 1024, 1024, sym) == _matrix_parity(1024, 1024, sym)).all()
+
+
+# ---- additive-FFT erasure decoder ------------------------------------------
+
+def _decode_program(trs, k, n, present):
+    lib = trs._lib.lib
+    flags = (ctypes.c_uint8 * n)(*[1 if p else 0 for p in present])
+    counts = (ctypes.c_size_t * 6)()
+    null = ctypes.POINTER(ctypes.c_uint32)()
+    rc = lib.trs_fft16_decode_program(k, n, flags, null, 0, null, 0, null, 0, null, 0, null, 0,
+                                      counts)
+    assert rc == 0
+    bufs = [np.zeros(max(1, int(c)), np.uint32) for c in counts[:5]]
+    ptr = [b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) for b in bufs]
+    rc = lib.trs_fft16_decode_program(k, n, flags, ptr[0], bufs[0].size, ptr[1], bufs[1].size,
+                                      ptr[2], bufs[2].size, ptr[3], bufs[3].size, ptr[4],
+                                      bufs[4].size, counts)
+    assert rc == 0
+    out = [b[:c].astype(np.int64) for b, c in zip(bufs, counts[:5])]
+    return out[0].reshape(-1, 8), out[1], out[2], out[3], out[4], int(counts[5])
+
+
+def _codeword(k, m, seed, cols):
+    sym = _symbols(seed, k, cols)
+    return np.concatenate([sym, _matrix_parity(k, m, sym)])
+
+
+@pytest.mark.parametrize("k,m", [(2, 2), (4, 2), (3, 13), (7, 13), (10, 4), (10, 6), (17, 33),
+                                 (32, 32), (100, 50), (200, 56), (300, 212), (1000, 24)])
+def test_fft_decoder_replays_to_codeword(trs, k, m):
+    """The decoder program rebuilds every erased symbol (data and parity) of
+    oracle-encoded codewords, for random erasure patterns up to m shards."""
+    n = k + m
+    cw = _codeword(k, m, k * 31 + m, 5)
+    rng = np.random.default_rng(k * 7 + m)
+    for e in sorted({1, m, int(rng.integers(1, m + 1))}):
+        erased = set(rng.choice(n, e, replace=False).tolist())
+        present = [i not in erased for i in range(n)]
+        steps, consts, ops, load, store, regs = _decode_program(trs, k, n, present)
+        assert list(load) == [i for i in range(n) if present[i]][:k]
+        assert list(store) == sorted(i for i in erased if i < k) + sorted(i for i in erased if i >= k)
+        got = replay(steps, consts, ops, load, store, regs, cw[load])
+        assert (got == cw[store]).all(), (k, m, sorted(erased))
+
+
+def test_fft_decoder_c3_pattern(trs):
+    # (1024,1024) with 1024 uniformly chosen erasures, a few symbol columns
+    k = m = 1024
+    cw = _codeword(k, m, 0xC3D, 2)
+    rng = np.random.default_rng(0xC3)
+    erased = set(rng.choice(k + m, 1024, replace=False).tolist())
+    present = [i not in erased for i in range(k + m)]
+    steps, consts, ops, load, store, regs = _decode_program(trs, k, k + m, present)
+    assert regs == 2048
+    got = replay(steps, consts, ops, load, store, regs, cw[load])
+    assert (got == cw[store]).all()
