@@ -142,6 +142,21 @@ def served_leg(trs, torch, requests: int, rates, seed: int):
 
 
 TOWER_SEED = 0x7A9E_5EED_0000_0003
+TRAFFIC_FILE = "profiles/hbm_traffic.json"
+
+
+def traffic_for(kernel: str, algorithmic_bytes: int):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a copy kernel of known
+    size); None when that file does not cover the workload."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
+            rec = json.load(f)["kernels"][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    if rec.get("algorithmic_bytes") != algorithmic_bytes:
+        return None
+    return rec["hbm_bytes_per_launch"]
 
 
 def tower_leg(trs, torch, O, stripes: int = 64):
@@ -342,7 +357,10 @@ def main() -> int:
     torch.cuda.synchronize()
     enc_exact = bool(torch.equal(buf, pristine))
 
-    # ---- batched steps
+    # ---- batched steps: each work list replays as one captured hipGraph
+    batch_enc.set_graph(True)
+    batch_rec.set_graph(True)
+
     def step_batched():
         batch_enc.run()
         batch_rec.run()
@@ -366,8 +384,8 @@ def main() -> int:
     enc_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in ev]))
     rec_ms = float(np.mean([b.elapsed_time(c) for _, b, c in ev]))
     value = (enc_bytes + rec_bytes) / (ms_step * 1e-3) / 1e9
-    kernel_ms = (enc_ms + rec_ms) / 2  # average k_batch8 launch duration
-    achieved = (enc_bytes + rec_bytes) / 2 / (kernel_ms * 1e-3) / 1e9
+    # dominant kernel: the batched reconstruct launch (k_batch8, table kernel)
+    achieved = rec_bytes / (rec_ms * 1e-3) / 1e9
     exact_after = bool(torch.equal(buf, pristine))
 
     # ---- plain per-call GPU path: one library call per stripe
@@ -473,9 +491,14 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic_for("k_batch8", rec_bytes),
             "kernel": "k_batch8",
-            "kernel_ms": {"encode": round(enc_ms, 3), "reconstruct": round(rec_ms, 3)},
+            "algorithmic_bytes": rec_bytes,
+            "kernel_ms": round(rec_ms, 3),
+            "traffic_source": TRAFFIC_FILE,
+            "other": {"k_batch_bs": {"role": "batched encode", "kernel_ms": round(enc_ms, 3),
+                                     "achieved": round(enc_bytes / (enc_ms * 1e-3) / 1e9, 1),
+                                     "traffic": traffic_for("k_batch_bs", enc_bytes)}},
         },
         "cpu_baseline": cpu,
         "batched": {

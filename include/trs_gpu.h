@@ -264,6 +264,10 @@ int trs_batch8_add_reconstruct(trs_batch8 *b, const trs_rs8 *rs, uint8_t *const 
 int trs_batch8_prepare(trs_batch8 *b);
 /* Run every queued request in one launch on `stream`. */
 int trs_batch8_run(trs_batch8 *b, void *stream);
+/* Graph mode: the first run after prepare captures the batch's launches into
+ * a hipGraph (kernel nodes only, on a private stream) and every run replays
+ * it on `stream`; prepare and clear drop the graph.  0 turns it off. */
+int trs_batch8_set_graph(trs_batch8 *b, int enable);
 /* Algorithmic bytes of the queued work: k reads + outputs written, per stripe. */
 uint64_t trs_batch8_bytes(const trs_batch8 *b);
 size_t trs_batch8_size(const trs_batch8 *b);
@@ -304,9 +308,11 @@ int trs_server_take(trs_server *s, uint64_t ticket, double *latency_us, uint64_t
 /* Times one apply candidate on synthetic device data: `nout` outputs from `k`
  * inputs of `len` bytes over GF(2^8) (field 8) or the tower (field 16,
  * split-plane, even len), random nonzero coefficients, through `route`
- * (0 = table kernel, 1 = generated bit-sliced program -- GF(2^8) shapes with
- * one only, 2 = int8 MFMA, k <= 32) in column strips of `strip` bytes (0 =
- * whole shard; one launch per strip).  *ms_out = mean milliseconds of `reps`
+ * (0 = matrix table kernel; 1 = bit-sliced program: the generated GF(2^8)
+ * program of the shape where it has one, or for the tower the staged LCH FFT
+ * encode program of shape (k, nout) on the LDS interpreter, 2 <= k <= 1024,
+ * k + nout <= 4096; 2 = int8 MFMA, GF(2^8), k <= 32) in column strips of
+ * `strip` bytes (0 = whole shard; one launch per strip).  *ms_out = mean milliseconds of `reps`
  * timed applies after one warm-up.  TRS_ERR_UNSUPPORTED for a route the shape
  * has no kernel for.  The reference tunes by hand (STRIP_BYTES,
  * src/reedsolomon16.rs:796-804; MULT_MARGIN, src/reedsolomon.rs:226-232). */

@@ -510,6 +510,10 @@ class Batch:
     def prepare(self) -> None:
         _check(lib.trs_batch8_prepare(self._h))
 
+    def set_graph(self, enable: bool = True) -> None:
+        """Replay the prepared launches as one captured hipGraph."""
+        _check(lib.trs_batch8_set_graph(self._h, int(enable)))
+
     def run(self, stream=None) -> None:
         s = _stream() if stream is None else ctypes.c_void_p(stream)
         _check(lib.trs_batch8_run(self._h, s))

@@ -94,6 +94,7 @@ _sig("trs_batch8_add_encode", c_int, c_vp, c_vp, c_ptrs, c_sizep, c_size)
 _sig("trs_batch8_add_reconstruct", c_int, c_vp, c_vp, c_ptrs, c_sizep, c_u8p, c_size, c_int)
 _sig("trs_batch8_prepare", c_int, c_vp)
 _sig("trs_batch8_run", c_int, c_vp, c_vp)
+_sig("trs_batch8_set_graph", c_int, c_vp, c_int)
 _sig("trs_batch8_bytes", ctypes.c_uint64, c_vp)
 _sig("trs_batch8_size", c_size, c_vp)
 

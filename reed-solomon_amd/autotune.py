@@ -1,8 +1,10 @@
 """Autotuner and learned cost model for route / strip selection.
 
 Candidates for one apply shape (field, k inputs, m outputs, shard length L)
-are a device route -- the fused table kernel, the generated bit-sliced program
-(GF(2^8) shapes that have one) or int8 MFMA (GF(2^8), k <= 32) -- times a
+are a device route -- "table" (the matrix route: fused table kernel),
+"bitsliced" (the FFT route: the generated bit-sliced LCH program of a GF(2^8)
+shape that has one, or for the tower the staged LCH FFT program on the LDS
+interpreter) or "mfma" (bit-sliced int8 MFMA, GF(2^8), k <= 32) -- times a
 column strip size (one launch per strip, 0 = whole shard), the GPU analogue of
 the reference's hand-tuned STRIP_BYTES (src/reedsolomon16.rs:796-804) and of
 its FFT-vs-matrix routing rule (src/fft16.rs:85-106, src/reedsolomon.rs:204-234).
@@ -15,8 +17,9 @@ its FFT-vs-matrix routing rule (src/fft16.rs:85-106, src/reedsolomon.rs:204-234)
   log time), trained on the exhaustive timings.
 * ``Tuner.choose(shape)`` ranks all candidates with the model and measures
   only the top ``top_n``; the exhaustive optimum judges its regret.
-* ``heuristic(shape)`` is the method without the model: the bit-sliced
-  program when the shape has one, else the table kernel, whole shard.
+* ``heuristic(shape)`` is the method without the model: the reference's
+  routing rules, whole shard -- GF(2^8): the generated program when the shape
+  has one; GF(2^16): the FFT when 9 * multiplies < 2 k m (src/fft16.rs:105).
 """
 from __future__ import annotations
 
@@ -69,9 +72,13 @@ def sample_shapes(n: int, seed: int) -> List[Shape]:
     return out
 
 
+def _tower_fft(s: Shape) -> bool:
+    return 2 <= s.k <= 1024 and s.k + s.m <= 4096
+
+
 def candidates(s: Shape) -> List[Candidate]:
     routes = ["table"]
-    if s.field == 8 and (s.k, s.m) in GENERATED:
+    if (s.field == 8 and (s.k, s.m) in GENERATED) or (s.field == 16 and _tower_fft(s)):
         routes.append("bitsliced")
     if s.field == 8 and s.k <= 32:
         routes.append("mfma")
@@ -127,6 +134,12 @@ def exhaustive(s: Shape, **kw) -> Dict[Candidate, float]:
 def heuristic(s: Shape) -> Candidate:
     if s.field == 8 and (s.k, s.m) in GENERATED:
         return ("bitsliced", 0)
+    if s.field == 16 and _tower_fft(s):
+        from . import fft16_program_info
+
+        info = fft16_program_info(s.k, s.m)
+        if info is not None and info["profitable"]:
+            return ("bitsliced", 0)
     return ("table", 0)
 
 

@@ -1379,6 +1379,21 @@ struct trs_batch8 {
   DevBuf ptrs;
   List table, bs, mfma;
   bool prepared = false;
+  // hipGraph of the prepared launches (captured on first run when enabled).
+  bool use_graph = false;
+  hipStream_t capture = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~trs_batch8() {
+    drop_graph();
+    if (capture) (void)hipStreamDestroy(capture);
+  }
 };
 extern "C" {
 
@@ -1394,6 +1409,7 @@ void trs_batch8_clear(trs_batch8 *b) {
   if (!b) return;
   b->reqs.clear();
   b->prepared = false;
+  b->drop_graph();
   b->table.njobs = b->bs.njobs = b->mfma.njobs = 0;
   b->table.total_tiles = b->bs.total_tiles = b->mfma.total_tiles = 0;
 }
@@ -1461,6 +1477,7 @@ int trs_batch8_add_reconstruct(trs_batch8 *b, const trs_rs8 *rs, uint8_t *const 
 
 int trs_batch8_prepare(trs_batch8 *b) {
   if (!b) return TRS_ERR_INVALID_ARGUMENT;
+  b->drop_graph();
   size_t nptr = 0;
   for (auto &r : b->reqs) nptr += r.ptrs.size();
   TRS_TRY(b->ptrs.ensure(nptr * sizeof(void *) + 8));
@@ -1544,13 +1561,8 @@ int trs_batch8_prepare(trs_batch8 *b) {
   return TRS_OK;
 }
 
-int trs_batch8_run(trs_batch8 *b, void *stream) {
-  if (!b) return TRS_ERR_INVALID_ARGUMENT;
-  if (!b->prepared) {
-    int st = trs_batch8_prepare(b);
-    if (st) return st;
-  }
-  hipStream_t s = as_stream(stream);
+namespace {
+int batch8_launch(trs_batch8 *b, hipStream_t s) {
   if (b->bs.total_tiles) {
     const int st = hip_status(launch_batch_bs(static_cast<const BatchJob *>(b->bs.jobs.p),
                                               static_cast<const uint32_t *>(b->bs.tiles.p),
@@ -1569,6 +1581,41 @@ int trs_batch8_run(trs_batch8 *b, void *stream) {
                                     b->table.total_tiles, b->table.grid, s));
   return TRS_OK;
 }
+}  // namespace
+
+int trs_batch8_set_graph(trs_batch8 *b, int enable) {
+  if (!b) return TRS_ERR_INVALID_ARGUMENT;
+  b->use_graph = enable != 0;
+  if (!b->use_graph) b->drop_graph();
+  return TRS_OK;
+}
+
+int trs_batch8_run(trs_batch8 *b, void *stream) {
+  if (!b) return TRS_ERR_INVALID_ARGUMENT;
+  if (!b->prepared) {
+    int st = trs_batch8_prepare(b);
+    if (st) return st;
+  }
+  hipStream_t s = as_stream(stream);
+  if (!b->use_graph) return batch8_launch(b, s);
+  if (!b->exec) {
+    // The launches read only buffers uploaded by prepare, so the captured
+    // graph replays them unchanged; it holds kernel nodes only.
+    if (!b->capture) TRS_TRY(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
+    TRS_TRY(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
+    const int st = batch8_launch(b, b->capture);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->capture, &g);
+    if (st) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    TRS_TRY(e);
+    b->graph = g;
+    TRS_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  return hip_status(hipGraphLaunch(b->exec, s));
+}
 
 }  // extern "C"
 
@@ -1583,6 +1630,7 @@ struct TuneSpace {
   DevBuf data;
   DevCoef coef;
   DevBuf frag;
+  std::vector<std::pair<uint64_t, Fft16DevicePtr>> fft16;  // (k << 32 | nout) -> program
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -1616,11 +1664,28 @@ extern "C" int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len
     if (!bs_route) return TRS_ERR_UNSUPPORTED;
   } else if (field == 8 && route == 2) {
     if (!mfma_available(k)) return TRS_ERR_UNSUPPORTED;
+  } else if (field == 16 && route == 1) {
+    if (k < 2 || k > kMaxFftDataShards || k + nout > 4096) return TRS_ERR_UNSUPPORTED;
   } else if (route != 0) {
     return TRS_ERR_UNSUPPORTED;
   }
   TuneSpace &T = tune_space();
   std::lock_guard<std::mutex> lk(T.mu);
+  const Fft16Device *prog16 = nullptr;
+  if (field == 16 && route == 1) {
+    const uint64_t key = (uint64_t(k) << 32) | nout;
+    for (auto &e : T.fft16)
+      if (e.first == key) prog16 = e.second.get();
+    if (!prog16) {
+      Fft16DevicePtr d;
+      const hipError_t e = fft16_device_new(build_staged_program16(k, nout), &d);
+      if (e == hipErrorInvalidValue) return TRS_ERR_UNSUPPORTED;
+      TRS_TRY(e);
+      if (T.fft16.size() >= 64) T.fft16.clear();
+      prog16 = d.get();
+      T.fft16.emplace_back(key, std::move(d));
+    }
+  }
   if (!T.stream) {
     TRS_TRY(hipStreamCreateWithFlags(&T.stream, hipStreamNonBlocking));
     TRS_TRY(hipEventCreate(&T.ev0));
@@ -1650,8 +1715,12 @@ extern "C" int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len
   const uint64_t sw = (strip == 0 || strip >= len) ? len : strip;
   PtrTable table16;
   if (field == 16) {
+    // route 0: one table; route 1: one table per column strip (pointers at
+    // the strip's column offset in the low plane)
     std::vector<const uint8_t *> p16;
-    for (uint32_t i = 0; i < k + nout; ++i) p16.push_back(base + uint64_t(i) * len);
+    const uint64_t step = route == 1 ? sw / 2 : len / 2;
+    for (uint64_t off = 0; off < len / 2; off += step)
+      for (uint32_t i = 0; i < k + nout; ++i) p16.push_back(base + uint64_t(i) * len + off);
     if ((st = table16.upload(p16, T.stream))) return st;
   }
   auto run_once = [&]() -> int {
@@ -1659,7 +1728,16 @@ extern "C" int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len
       const uint64_t span =
           std::min<uint64_t>(field == 16 ? sw / 2 : sw, (field == 16 ? len / 2 : len) - off);
       hipError_t e = hipSuccess;
-      if (field == 16) {
+      if (field == 16 && route == 1) {
+        Fft16Call c{};
+        c.ptrs = static_cast<const uint8_t *const *>(table16.d) + (off / (sw / 2)) * (k + nout);
+        c.plane = len / 2;
+        c.cols = span;
+        c.stripes = 1;
+        c.nstore = nout;
+        c.vec_ok = ((len / 2) % 16 == 0 && off % 16 == 0) ? 1u : 0u;
+        e = launch_fft16(*prog16, c, T.stream);
+      } else if (field == 16) {
         Gf16Call c{};
         c.ptrs = static_cast<const uint8_t *const *>(table16.d);
         c.coef = T.coef.ptr();

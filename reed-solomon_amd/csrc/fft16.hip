@@ -434,6 +434,58 @@ __device__ __forceinline__ void gf8_mul_acc(uint32_t a, const uint32_t *v, uint3
   }
 }
 
+// acc ^= a * v for a wave-uniform byte a (scalar register): zero bits are
+// skipped with scalar branches and the doubling chain stops at the top bit.
+__device__ __forceinline__ void gf8_mul_acc_uniform(uint32_t a, const uint32_t *v,
+                                                    uint32_t *acc) {
+  uint32_t p[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) p[j] = v[j];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    if ((a >> b) == 0) break;
+    if ((a >> b) & 1u) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[j] ^= p[j];
+    }
+    if (b < 7) {
+      const uint32_t t = p[7];
+      p[7] = p[6];
+      p[6] = p[5];
+      p[5] = p[4];
+      p[4] = p[3] ^ t;
+      p[3] = p[2] ^ t;
+      p[2] = p[1] ^ t;
+      p[1] = p[0];
+      p[0] = t;
+    }
+  }
+}
+
+// tower_mul_acc for a wave-uniform constant word.
+__device__ __forceinline__ void tower_mul_acc_uniform(uint32_t cw, const uint32_t (&x)[16],
+                                                      uint32_t (&out)[16]) {
+  const uint32_t a0 = cw & 0xffu, a1 = (cw >> 8) & 0xffu, a2 = (cw >> 16) & 0xffu;
+  if (a2 == 0) {
+    // subfield constant: c * x = (c_lo x_lo, c_lo x_hi)
+    gf8_mul_acc_uniform(a0, x, out);
+    gf8_mul_acc_uniform(a0, x + 8, out + 8);
+    return;
+  }
+  uint32_t t0[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  gf8_mul_acc_uniform(a0, x, t0);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    out[j] ^= t0[j];
+    out[8 + j] ^= t0[j];
+  }
+  uint32_t s[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s[j] = x[j] ^ x[8 + j];
+  gf8_mul_acc_uniform(a1, s, out + 8);
+  gf8_mul_acc_uniform(a2, x + 8, out);
+}
+
 // out ^= c * x over the tower, c given as its Karatsuba word.
 __device__ __forceinline__ void tower_mul_acc(uint32_t cw, const uint32_t (&x)[16],
                                               uint32_t (&out)[16]) {
@@ -481,7 +533,8 @@ struct Fft16Args {
   const uint2 *ops;
   const uint32_t *load;
   const uint32_t *store;
-  uint64_t plane;
+  uint64_t plane;  // offset of the high plane from a shard pointer
+  uint64_t cols;   // symbol columns processed (from each pointer)
   uint32_t nload, nstore, nsteps, gshift;
   uint32_t groups;   // column groups per shard
   uint32_t cblocks;  // workgroups per stripe
@@ -530,7 +583,7 @@ __device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
     const uint32_t cg = g0 + g;
     if (cg < a.groups) {
       const uint64_t col = uint64_t(cg) * kGroupCols;
-      const uint64_t rem = a.plane - col;
+      const uint64_t rem = a.cols - col;
       const bool full = a.vec_ok && rem >= kGroupCols;
       const uint8_t *s = ptr[slot];
       uint32_t lo[8], hi[8];
@@ -573,7 +626,13 @@ __device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
 #pragma unroll
           for (int j = 0; j < 16; ++j) L[j] ^= H[j];
         } else if (!(cw & kConstZero)) {
-          tower_mul_acc(cw, H, L);
+          // One block spans the wave at strides >= 64 (one column group per
+          // workgroup): then the constant sits in a scalar register.
+          const uint32_t cw0 = __builtin_amdgcn_readfirstlane(cw);
+          if (__builtin_amdgcn_ballot_w64(cw != cw0) == 0)
+            tower_mul_acc_uniform(cw0, H, L);
+          else
+            tower_mul_acc(cw, H, L);
         }
         if (!st.d) {
 #pragma unroll
@@ -621,7 +680,7 @@ __device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
     uint32_t v[16];
     rf_load(rf, (a.store[j] << a.gshift) + g, v);
     const uint64_t col = uint64_t(cg) * kGroupCols;
-    const uint64_t rem = a.plane - col;
+    const uint64_t rem = a.cols - col;
     const bool full = a.vec_ok && rem >= kGroupCols;
     uint8_t *d = const_cast<uint8_t *>(ptr[a.nload + j]);
     uint32_t lo[8], hi[8];
@@ -711,7 +770,8 @@ hipError_t raise_lds_limit(size_t lds) {
 
 hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t stream) {
   if (call.plane == 0 || call.stripes == 0) return hipSuccess;
-  const uint64_t groups = (call.plane + kGroupCols - 1) / kGroupCols;
+  const uint64_t cols = call.cols ? call.cols : call.plane;
+  const uint64_t groups = (cols + kGroupCols - 1) / kGroupCols;
   const uint32_t gs = choose_gshift(d, groups);
   const uint64_t cblocks = (groups + (1u << gs) - 1) >> gs;
   const uint64_t total = cblocks * call.stripes;
@@ -725,6 +785,7 @@ hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t
   a.load = d.load;
   a.store = d.store;
   a.plane = call.plane;
+  a.cols = cols;
   a.nload = uint32_t(d.nload);
   a.nstore = call.nstore;
   a.nsteps = uint32_t(d.nsteps);
@@ -732,7 +793,7 @@ hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t
   a.groups = uint32_t(groups);
   a.cblocks = uint32_t(cblocks);
   a.total = uint32_t(total);
-  a.vec_ok = (call.vec_ok && call.plane % 16 == 0) ? 1u : 0u;
+  a.vec_ok = (call.vec_ok && call.plane % 16 == 0) ? 1u : 0u;  // pointers carry any column offset
   const size_t lds = (d.regs * kRegBytes) << gs;
   const bool wide = (d.max_count << gs) >= 512;
   hipError_t e = raise_lds_limit(lds);
@@ -765,6 +826,7 @@ hipError_t launch_fft16_jobs(const std::vector<Fft16JobSpec> &jobs, hipStream_t 
     a.load = d.load;
     a.store = d.store;
     a.plane = j.plane;
+    a.cols = j.plane;
     a.nload = uint32_t(d.nload);
     a.nstore = j.nstore;
     a.nsteps = uint32_t(d.nsteps);

@@ -80,6 +80,8 @@ struct Fft16Call {
   uint32_t stripes;
   uint32_t nstore;             // stored registers (a prefix of the program's store list)
   uint32_t vec_ok;             // every shard pointer 16-byte aligned
+  uint64_t cols;               // columns from each pointer (0: plane); pointers may sit at a
+                               // column offset within their low plane
 };
 
 // Lowers p for the device (glue scheduled into independent sub-steps, whole

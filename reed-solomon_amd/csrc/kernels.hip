@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
+#include "uniform.hpp"
 
 namespace trs {
 
@@ -62,6 +63,13 @@ __device__ __forceinline__ void store_lane(uint8_t *p, bool full, uint64_t rem,
   }
 }
 
+// acc ^ (p & m) in one v_bitop3 (truth table over (S0, S1, S2) = (acc, p, m)
+// indexed S0 S1 S2 from the high bit: 0x78); the compiler does not fuse it
+// when m is a scalar register.
+__device__ __forceinline__ uint32_t xor_and(uint32_t acc, uint32_t p, uint32_t m) {
+  return __builtin_amdgcn_bitop3_b32(acc, p, m, 0x78);
+}
+
 // acc[j] ^= c_j * x for the NT coefficients at cw (transposed, padded rows:
 // byte j of the NT/4 words at cw is output j's coefficient).
 template <int NT>
@@ -76,16 +84,16 @@ __device__ __forceinline__ void mac_rows(uint32_t (&acc)[NT][4], const uint32_t 
     for (int w = 0; w < 4; ++w) p[b][w] = xtime4(p[b - 1][w]);
 #pragma unroll
   for (int q = 0; q < NT / 4; ++q) {
-    const uint32_t word = cw[q];
+    const uint32_t word = uni(cw[q]);  // same address in every lane
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const uint32_t c = (word >> (8 * s)) & 0xffu;
       if (c) {
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-          const uint32_t msk = 0u - ((c >> b) & 1u);
+          const uint32_t msk = 0u - ((c >> b) & 1u);  // scalar
 #pragma unroll
-          for (int w = 0; w < 4; ++w) acc[q * 4 + s][w] ^= p[b][w] & msk;
+          for (int w = 0; w < 4; ++w) acc[q * 4 + s][w] = xor_and(acc[q * 4 + s][w], p[b][w], msk);
         }
       }
     }
@@ -142,7 +150,7 @@ template <int NT, uint32_t MODE>
 __global__ __launch_bounds__(kBlockThreads) void k_gf8_apply(const Gf8Call a) {
   const uint64_t off = (uint64_t(blockIdx.x) * kBlockThreads + threadIdx.x) * kLaneBytes;
   if (off >= a.len) return;
-  gf8_tile<NT, MODE>([&](uint32_t idx) { return a.ptr[idx]; }, a.coef, a.len, a.k, a.nout,
+  gf8_tile<NT, MODE>([&](uint32_t idx) { return uni(a.ptr[idx]); }, a.coef, a.len, a.k, a.nout,
                      a.ostride, a.vec_ok != 0, blockIdx.y * NT, off, a.flag);
 }
 
@@ -154,16 +162,18 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch8(const BatchJob *jobs,
                                                          const uint32_t *tile_job,
                                                          uint64_t total) {
   for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
-    const BatchJob &J = jobs[tile_job[t]];
-    const uint64_t local = t - J.tile_begin;
-    const uint64_t chunk = local / J.out_tiles;
-    const uint32_t ot = uint32_t(local - chunk * J.out_tiles);
+    const BatchJob &J = jobs[uni(tile_job[t])];
+    const uint64_t len = uni(J.len);
+    const uint32_t out_tiles = uni(J.out_tiles);
+    const uint64_t local = t - uni(J.tile_begin);
+    const uint64_t chunk = local / out_tiles;
+    const uint32_t ot = uint32_t(local - chunk * out_tiles);
     const uint64_t off = chunk * kChunkBytes + uint64_t(threadIdx.x) * kLaneBytes;
-    if (off < J.len) {
-      const uint8_t *const *ptrs = J.ptrs;
-      gf8_tile<kOutStride, kWrite>([&](uint32_t idx) { return ptrs[idx]; }, J.coef, J.len, J.k,
-                                   J.nout, J.ostride, J.vec_ok != 0, ot * kOutStride, off,
-                                   nullptr);
+    if (off < len) {
+      const uint8_t *const *ptrs = uni(J.ptrs);
+      gf8_tile<kOutStride, kWrite>([&](uint32_t idx) { return uni(ptrs[idx]); }, uni(J.coef),
+                                   len, uni(J.k), uni(J.nout), uni(J.ostride),
+                                   uni(J.vec_ok) != 0, ot * kOutStride, off, nullptr);
     }
   }
 }
@@ -190,7 +200,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_gf16_apply(const Gf16Call a) 
     for (int w = 0; w < 4; ++w) A[j][w] = B[j][w] = C[j][w] = 0;
 
   for (uint32_t i = 0; i < a.k; ++i) {
-    const uint8_t *s = a.ptrs[i];
+    const uint8_t *s = uni(a.ptrs[i]);
     uint32_t lo[4], hi[4], sum[4];
     load_lane(s + off, full, rem, lo);
     load_lane(s + a.plane + off, full, rem, hi);

@@ -106,12 +106,12 @@ def emit_body(k: int, ops, nreg: int) -> str:
     out = []
     out.append(f"  uint32_t R[{nreg}][8];")
     for s in range(k):
-        out.append(f"  load_planes(in[{s}] + off, full, rem, R[{s}]);")
+        out.append(f"  load_planes(uni(in[{s}]) + off, full, rem, R[{s}]);")
     for op in ops:
         kind = op[0]
         if kind == "st":
             _, j, r = op
-            out.append(f"  store_planes(out[{j}] + off, full, rem, R[{r}]);")
+            out.append(f"  store_planes(uni(out[{j}]) + off, full, rem, R[{r}]);")
         elif kind == "zero":
             out.append(f"  for (int r = 0; r < 8; ++r) R[{op[1]}][r] = 0;")
         elif kind in ("xo", "cp"):
@@ -150,6 +150,7 @@ def generate() -> str:
         "// GENERATED by reed-solomon_amd/gf8_codegen.py -- do not edit.",
         "// Bit-sliced GF(2^8) encode programs with compile-time coefficients.",
         '#include "bitslice.hpp"',
+        '#include "uniform.hpp"',
         "",
         "namespace trs {",
         "namespace {",
@@ -184,14 +185,16 @@ def generate() -> str:
     parts.append("__global__ __launch_bounds__(kBlockThreads) void k_batch_bs(const BatchJob *jobs,")
     parts.append("    const uint32_t *tile_job, uint64_t total) {")
     parts.append("  for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {")
-    parts.append("    const BatchJob &J = jobs[tile_job[t]];")
-    parts.append("    const uint64_t off = (t - J.tile_begin) * 8192 + uint64_t(threadIdx.x) * 32;")
-    parts.append("    if (off >= J.len) continue;")
-    parts.append("    const uint64_t rem = J.len - off;")
-    parts.append("    const bool full = J.vec_ok && rem >= 32;")
-    parts.append("    const uint8_t *const *in = J.ptrs;")
-    parts.append("    uint8_t *const *out = const_cast<uint8_t *const *>(J.ptrs + J.k);")
-    parts.append("    switch (J.prog) {")
+    parts.append("    // job fields are the same in every lane: keep them in scalar registers")
+    parts.append("    const BatchJob &J = jobs[uni(tile_job[t])];")
+    parts.append("    const uint64_t len = uni(J.len);")
+    parts.append("    const uint64_t off = (t - uni(J.tile_begin)) * 8192 + uint64_t(threadIdx.x) * 32;")
+    parts.append("    if (off >= len) continue;")
+    parts.append("    const uint64_t rem = len - off;")
+    parts.append("    const bool full = uni(J.vec_ok) && rem >= 32;")
+    parts.append("    const uint8_t *const *in = uni(J.ptrs);")
+    parts.append("    uint8_t *const *out = const_cast<uint8_t *const *>(in + uni(J.k));")
+    parts.append("    switch (uni(J.prog)) {")
     for pid, (route, k, m, name) in enumerate(table, start=1):
         parts.append(f"      case {pid}: {name}(in, out, off, full, rem); break;")
     parts.append("      default: break;")

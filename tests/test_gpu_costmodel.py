@@ -56,7 +56,11 @@ def test_tune_apply_routes(trs):
     assert routes == {"table", "bitsliced", "mfma"}
     assert all(t > 0 for t in got.values())
     s16 = A.Shape(16, 40, 12, 8192)
-    assert {c[0] for c in A.exhaustive(s16)} == {"table"}
+    assert {c[0] for c in A.exhaustive(s16)} == {"table", "bitsliced"}
+    # the tower FFT candidate over strips equals nothing in output terms, but
+    # must time every strip size; shapes past k + m = 4096 have none
+    assert A.measure(A.Shape(16, 1000, 24, 65536), ("bitsliced", 8192)) > 0
+    assert A.measure(A.Shape(16, 8, 4100, 4096), ("bitsliced", 0)) is None
     assert A.measure(A.Shape(8, 40, 5, 4096), ("mfma", 0)) is None
 
 
