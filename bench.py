@@ -260,6 +260,56 @@ def tower_leg(trs, torch, O, stripes: int = 64):
     }
 
 
+def counter_run(trs, torch, stripes: int) -> int:
+    """The launches whose HBM traffic `roofline.traffic` reports -- the
+    batched encode (k_batch_bs) and reconstruct (k_batch8) work lists of the
+    default workload and the tower FFT encode (k_fft16) of the tower workload,
+    three launches each -- then a calibration kernel of known traffic
+    (bitwise_not over 1 GiB: 1 GiB read + 1 GiB written)."""
+    lib = trs._lib.lib
+    sptr = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    meta = build_workload(stripes, np.random.default_rng(SEED))
+    codecs = {s: trs.ReedSolomon(*s) for s in SHAPES}
+    sizes = [(k + m) * L for (k, m, L, _) in meta]
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    buf = torch.empty(int(offsets[-1]), dtype=torch.uint8, device="cuda")
+    base = buf.data_ptr()
+    be, br = trs.Batch(), trs.Batch()
+    for s, (k, m, L, erased) in enumerate(meta):
+        o, n = int(offsets[s]), k + m
+        trs.fill_random(buf[o:o + k * L], SEED + s)
+        ptrs = (ctypes.c_void_p * n)(*[base + o + i * L for i in range(n)])
+        lens = (ctypes.c_size_t * n)(*([L] * n))
+        present = (ctypes.c_uint8 * n)(*[0 if i in erased else 1 for i in range(n)])
+        rs = codecs[(k, m)]
+        assert lib.trs_rs8_encode(rs._h, ptrs, lens, n, sptr) == 0
+        assert lib.trs_batch8_add_encode(be._h, rs._h, ptrs, lens, n) == 0
+        assert lib.trs_batch8_add_reconstruct(br._h, rs._h, ptrs, lens, present, n, 0) == 0
+    be.prepare()
+    br.prepare()
+    for _ in range(3):
+        be.run()
+        br.run()
+    torch.cuda.synchronize()
+    print(json.dumps({"k_batch_bs": be.bytes(), "k_batch8": br.bytes()}), flush=True)
+    del buf
+    torch.cuda.empty_cache()
+    r16 = trs.ReedSolomon16(1024, 1024)
+    t = torch.zeros((64, 2048, 4096), dtype=torch.uint8, device="cuda")
+    trs.fill_random(t.reshape(-1), TOWER_SEED)
+    for _ in range(3):
+        r16.encode_stripes(t, "fft")
+    torch.cuda.synchronize()
+    del t
+    torch.cuda.empty_cache()
+    a = torch.full((1 << 30,), 7, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.bitwise_not(a, out=b)
+    torch.cuda.synchronize()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -271,6 +321,9 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=800)
     ap.add_argument("--no-tower", action="store_true")
+    ap.add_argument("--counters", action="store_true",
+                    help="run only the measured launches (3 each) and a 1 GiB calibration "
+                         "kernel, for rocprofv3 --pmc passes")
     ap.add_argument("--tower-stripes", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -288,6 +341,9 @@ def main() -> int:
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
     sptr = ctypes.c_void_p(stream.cuda_stream)
+
+    if args.counters:
+        return counter_run(trs, torch, args.stripes)
 
     rng = np.random.default_rng(SEED)
     meta = build_workload(args.stripes, rng)

@@ -20,6 +20,12 @@
 #include "kernels.hpp"
 #include "uniform.hpp"
 
+// 1: apply each coefficient bit as a full-word mask (branch-free);
+// 0: skip zero bits with scalar branches.
+#ifndef TRS_MASKED_MAC
+#define TRS_MASKED_MAC 0
+#endif
+
 namespace trs {
 
 namespace {
@@ -87,15 +93,25 @@ __device__ __forceinline__ void mac_rows(uint32_t (&acc)[NT][4], const uint32_t 
     const uint32_t word = uni(cw[q]);  // same address in every lane
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      const uint32_t c = (word >> (8 * s)) & 0xffu;
+      const uint32_t c = (word >> (8 * s)) & 0xffu;  // scalar
+#if TRS_MASKED_MAC
       if (c) {
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-          const uint32_t msk = 0u - ((c >> b) & 1u);  // scalar
+          const uint32_t msk = 0u - ((c >> b) & 1u);
 #pragma unroll
           for (int w = 0; w < 4; ++w) acc[q * 4 + s][w] = xor_and(acc[q * 4 + s][w], p[b][w], msk);
         }
       }
+#else
+      // zero bits cost a scalar branch, not VALU work
+#pragma unroll
+      for (int b = 0; b < 8; ++b)
+        if ((c >> b) & 1u) {
+#pragma unroll
+          for (int w = 0; w < 4; ++w) acc[q * 4 + s][w] ^= p[b][w];
+        }
+#endif
     }
   }
 }
