@@ -31,6 +31,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0xC057)
     ap.add_argument("--cap-mib", type=int, default=32)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--detail", default=None,
+                    help="JSON lines, one per shape: every timed candidate, the best, the "
+                         "heuristic and (held-out shapes) the model's choice")
     args = ap.parse_args()
 
     import __graft_entry__
@@ -53,6 +56,19 @@ def main() -> int:
     train_s = time.time() - t1
     tuner = A.Tuner(model, top_n=args.top_n)
     reg_model, reg_top1, reg_heur, measured_counts = [], [], [], []
+    detail = open(args.detail, "w") if args.detail else None
+    for idx, (s, meas) in enumerate(zip(shapes, timings)):
+        if detail is None:
+            break
+        best = min(meas, key=meas.get)
+        rec = {"field": s.field, "k": s.k, "m": s.m, "length": s.length, "held_out": idx >= ntrain,
+               "best": list(best), "best_ms": meas[best], "heuristic": list(A.heuristic(s)),
+               "times_ms": {f"{r}/{st}": t for (r, st), t in meas.items()}}
+        if idx >= ntrain:
+            rec["model"] = list(tuner.choose(s, meas))
+        detail.write(json.dumps(rec) + "\n")
+    if detail is not None:
+        detail.close()
     for s, meas in zip(shapes[ntrain:], timings[ntrain:]):
         reg_model.append(A.regret(meas, tuner.choose(s, meas)))
         reg_top1.append(A.regret(meas, A.Tuner(model, top_n=1).choose(s, meas)))

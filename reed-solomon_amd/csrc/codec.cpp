@@ -208,8 +208,11 @@ struct Plan {
   DevCoef coef;                          // [missing data rows; missing parity rows] x k
   std::vector<uint8_t> rows8;            // host copy (GF(2^8) plans)
   DevBuf frag;                           // MFMA A fragments of the same rows (k <= 32)
-  // Tower-FFT erasure decoder of this pattern (RS16), built on first use:
-  // loads the valid shards, stores [missing data; missing parity].
+  // Tower plans build their matrix rows (coef) on first use of the matrix
+  // route, and the tower-FFT erasure decoder on first use of the FFT route
+  // (it loads the valid shards and stores [missing data; missing parity]).
+  std::once_flag coef_once;
+  int coef_status = 0;
   std::once_flag fft_once;
   hipError_t fft_status = hipSuccess;
   Fft16DevicePtr fft;
@@ -454,13 +457,23 @@ int plan16_for(const trs_rs16 *rs, const Mask &m, std::shared_ptr<Plan> *out) {
   auto p = std::make_shared<Plan>();
   p->presence = m;
   split_indices(m, rs->k, rs->n, *p);
-  std::vector<uint16_t> rows;
-  if (!fused_rows(rs->G, rs->k, *p, rows)) return TRS_SINGULAR_MATRIX;
-  const int st = upload_coef16(p->coef, rows.data(),
-                               p->missing_data.size() + p->missing_parity.size(), rs->k);
-  if (st) return st;
+  if (p->valid.size() < rs->k) return TRS_TOO_FEW_SHARDS_PRESENT;
   *out = rs->plans.insert(m, p);
   return TRS_OK;
+}
+
+// The matrix rows of a tower plan, built and uploaded once.
+int plan16_matrix(Plan &p, size_t k) {
+  std::call_once(p.coef_once, [&p, k] {
+    std::vector<uint16_t> rows;
+    if (!fused_rows(Matrix<F16>(), k, p, rows)) {
+      p.coef_status = TRS_SINGULAR_MATRIX;
+      return;
+    }
+    p.coef_status = upload_coef16(p.coef, rows.data(),
+                                  p.missing_data.size() + p.missing_parity.size(), k);
+  });
+  return p.coef_status;
 }
 
 // scan_shards (src/reedsolomon.rs:801-821; RS16 adds the even-length rule,
@@ -1192,7 +1205,11 @@ int trs_rs16_reconstruct_stripes(const trs_rs16 *rs, uint8_t *const *shards,
     if ((st = gather(*w.plan, shards + i * n, lens.data(), k, len, data_only != 0, w.ptrs, w.nout)))
       return st;
     if (w.nout == 0) continue;
-    if (use_fft_decode(route, n, k, w.nout) && (st = plan_fft16(*w.plan, n, &w.fft))) return st;
+    if (use_fft_decode(route, n, k, w.nout)) {
+      if ((st = plan_fft16(*w.plan, n, &w.fft))) return st;
+    } else if ((st = plan16_matrix(*w.plan, k))) {
+      return st;
+    }
     work.push_back(std::move(w));
   }
   hipStream_t s = as_stream(stream);
@@ -1289,6 +1306,7 @@ int reconstruct16(const trs_rs16 *rs, size_t k, size_t total, std::shared_ptr<Pl
     if ((st = plan_fft16(*plan, total, &d))) return st;
     return run_fft16_program(*d, ptrs, nout, len, s);
   }
+  if ((st = plan16_matrix(*plan, k))) return st;
   return run_gf16(ptrs, k, nout, plan->coef, len, kWrite, nullptr, s);
 }
 }  // namespace
@@ -1377,7 +1395,9 @@ struct trs_batch8 {
   };
   std::vector<Req> reqs;
   DevBuf ptrs;
-  List table, bs, mfma;
+  List table, mfma;
+  static constexpr size_t kMaxPrograms = 32;
+  List bs[kMaxPrograms];  // one work list per generated program id (index 0 unused)
   bool prepared = false;
   // hipGraph of the prepared launches (captured on first run when enabled).
   bool use_graph = false;
@@ -1410,8 +1430,12 @@ void trs_batch8_clear(trs_batch8 *b) {
   b->reqs.clear();
   b->prepared = false;
   b->drop_graph();
-  b->table.njobs = b->bs.njobs = b->mfma.njobs = 0;
-  b->table.total_tiles = b->bs.total_tiles = b->mfma.total_tiles = 0;
+  b->table.njobs = b->mfma.njobs = 0;
+  b->table.total_tiles = b->mfma.total_tiles = 0;
+  for (auto &L : b->bs) {
+    L.njobs = 0;
+    L.total_tiles = 0;
+  }
 }
 
 size_t trs_batch8_size(const trs_batch8 *b) { return b ? b->reqs.size() : 0; }
@@ -1483,9 +1507,13 @@ int trs_batch8_prepare(trs_batch8 *b) {
   TRS_TRY(b->ptrs.ensure(nptr * sizeof(void *) + 8));
   std::vector<const uint8_t *> hp;
   hp.reserve(nptr);
-  std::vector<BatchJob> jt, jb, jm;
-  std::vector<uint64_t> tt, tb, tm;
-  uint64_t tile_t = 0, tile_b = 0, tile_m = 0;
+  const size_t nprog = std::min<size_t>(bs_program_count() + 1, trs_batch8::kMaxPrograms);
+  std::vector<BatchJob> jt, jm;
+  std::vector<uint64_t> tt, tm;
+  std::vector<std::vector<BatchJob>> jb(nprog);
+  std::vector<std::vector<uint64_t>> tb(nprog);
+  std::vector<uint64_t> tile_b(nprog, 0);
+  uint64_t tile_t = 0, tile_m = 0;
   const auto *dptr = static_cast<const uint8_t *const *>(b->ptrs.p);
   for (const auto &r : b->reqs) {
     BatchJob J{};
@@ -1509,11 +1537,11 @@ int trs_batch8_prepare(trs_batch8 *b) {
       tm.push_back(tile_m);
       tile_m += ((r.len + mfma_group_cols() - 1) / mfma_group_cols()) * ((r.nout + 7) / 8);
       jm.push_back(J);
-    } else if (r.prog) {
-      J.tile_begin = tile_b;
-      tb.push_back(tile_b);
-      tile_b += (r.len + kBsChunkBytes - 1) / kBsChunkBytes;
-      jb.push_back(J);
+    } else if (r.prog && r.prog < nprog) {
+      J.tile_begin = tile_b[r.prog];
+      tb[r.prog].push_back(tile_b[r.prog]);
+      tile_b[r.prog] += (r.len + kBsChunkBytes - 1) / kBsChunkBytes;
+      jb[r.prog].push_back(J);
     } else {
       J.tile_begin = tile_t;
       tt.push_back(tile_t);
@@ -1553,7 +1581,8 @@ int trs_batch8_prepare(trs_batch8 *b) {
   if (const char *e = std::getenv("TRS_BATCH_BS_GRID_PER_CU")) per_cu_bs = std::atoi(e);
   int st = upload(b->table, jt, tt, tile_t, per_cu_table);
   if (st) return st;
-  if ((st = upload(b->bs, jb, tb, tile_b, per_cu_bs))) return st;
+  for (size_t p = 1; p < nprog; ++p)
+    if ((st = upload(b->bs[p], jb[p], tb[p], tile_b[p], per_cu_bs))) return st;
   uint64_t per_cu_mfma = 2;
   if (const char *e = std::getenv("TRS_BATCH_MFMA_GRID_PER_CU")) per_cu_mfma = std::atoi(e);
   if ((st = upload(b->mfma, jm, tm, tile_m, per_cu_mfma))) return st;
@@ -1563,10 +1592,12 @@ int trs_batch8_prepare(trs_batch8 *b) {
 
 namespace {
 int batch8_launch(trs_batch8 *b, hipStream_t s) {
-  if (b->bs.total_tiles) {
-    const int st = hip_status(launch_batch_bs(static_cast<const BatchJob *>(b->bs.jobs.p),
-                                              static_cast<const uint32_t *>(b->bs.tiles.p),
-                                              b->bs.total_tiles, b->bs.grid, s));
+  for (size_t p = 1; p < trs_batch8::kMaxPrograms; ++p) {
+    const trs_batch8::List &L = b->bs[p];
+    if (!L.total_tiles) continue;
+    const int st = hip_status(launch_batch_bs(uint32_t(p), static_cast<const BatchJob *>(L.jobs.p),
+                                              static_cast<const uint32_t *>(L.tiles.p),
+                                              L.total_tiles, L.grid, s));
     if (st) return st;
   }
   if (b->mfma.total_tiles) {
@@ -2107,9 +2138,11 @@ int trs_server_submit_reconstruct16(trs_server *S, const trs_rs16 *rs, uint8_t *
       q.k16 = uint32_t(rs->k);
       q.nout16 = uint32_t(nout);
       q.bytes = uint64_t(rs->k + nout) * len;
-      if (use_fft_decode(rs->decode_route.load(), rs->n, rs->k, nout) &&
-          (st = plan_fft16(*plan, rs->n, &q.fft16)))
+      if (use_fft_decode(rs->decode_route.load(), rs->n, rs->k, nout)) {
+        if ((st = plan_fft16(*plan, rs->n, &q.fft16))) return st;
+      } else if ((st = plan16_matrix(*plan, rs->k))) {
         return st;
+      }
     }
   }
   return server_enqueue(S, std::move(q), ticket);

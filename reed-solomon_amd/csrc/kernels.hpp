@@ -100,9 +100,12 @@ enum EncodeRoute : int { kRouteTable = 0, kRouteBsMatrix = 1, kRouteBsFft = 2 };
 hipError_t launch_bs_encode(int route, const Gf8Call &call, hipStream_t stream);
 bool bs_encode_available(int route, uint32_t k, uint32_t m);
 uint32_t bs_program_id(int route, uint32_t k, uint32_t m);
-// Persistent work-list kernel over jobs with a generated program (8 KiB tiles).
+// Persistent work-list kernels over jobs with a generated program (8 KiB
+// tiles), one kernel per program id 1..bs_program_count(); every job of a
+// launch runs program `prog`.
 constexpr uint64_t kBsChunkBytes = 8192;
-hipError_t launch_batch_bs(const BatchJob *jobs, const uint32_t *tile_job, uint64_t total_tiles,
-                           uint32_t grid, hipStream_t stream);
+uint32_t bs_program_count();
+hipError_t launch_batch_bs(uint32_t prog, const BatchJob *jobs, const uint32_t *tile_job,
+                           uint64_t total_tiles, uint32_t grid, hipStream_t stream);
 
 }  // namespace trs

@@ -179,29 +179,25 @@ def generate() -> str:
             parts.append("}")
             parts.append("")
             table.append((route, k, m, name))
-    # Persistent work-list kernel over jobs that name a generated program:
-    # tiles are (job, 8 KiB column chunk), numbered job by job; tile_job[t]
-    # names the job of tile t.
-    parts.append("__global__ __launch_bounds__(kBlockThreads) void k_batch_bs(const BatchJob *jobs,")
-    parts.append("    const uint32_t *tile_job, uint64_t total) {")
-    parts.append("  for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {")
-    parts.append("    // job fields are the same in every lane: keep them in scalar registers")
-    parts.append("    const BatchJob &J = jobs[uni(tile_job[t])];")
-    parts.append("    const uint64_t len = uni(J.len);")
-    parts.append("    const uint64_t off = (t - uni(J.tile_begin)) * 8192 + uint64_t(threadIdx.x) * 32;")
-    parts.append("    if (off >= len) continue;")
-    parts.append("    const uint64_t rem = len - off;")
-    parts.append("    const bool full = uni(J.vec_ok) && rem >= 32;")
-    parts.append("    const uint8_t *const *in = uni(J.ptrs);")
-    parts.append("    uint8_t *const *out = const_cast<uint8_t *const *>(in + uni(J.k));")
-    parts.append("    switch (uni(J.prog)) {")
+    # Persistent work-list kernels, one per generated program so that each gets
+    # the register budget of its own program: tiles are (job, 8 KiB column
+    # chunk), numbered job by job; tile_job[t] names the job of tile t.
     for pid, (route, k, m, name) in enumerate(table, start=1):
-        parts.append(f"      case {pid}: {name}(in, out, off, full, rem); break;")
-    parts.append("      default: break;")
-    parts.append("    }")
-    parts.append("  }")
-    parts.append("}")
-    parts.append("")
+        parts.append(f"__global__ __launch_bounds__(kBlockThreads) void k_batch_{name}(const BatchJob *jobs,")
+        parts.append("    const uint32_t *tile_job, uint64_t total) {")
+        parts.append("  for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {")
+        parts.append("    // job fields are the same in every lane: keep them in scalar registers")
+        parts.append("    const BatchJob &J = jobs[uni(tile_job[t])];")
+        parts.append("    const uint64_t len = uni(J.len);")
+        parts.append("    const uint64_t off = (t - uni(J.tile_begin)) * 8192 + uint64_t(threadIdx.x) * 32;")
+        parts.append("    if (off >= len) continue;")
+        parts.append("    const uint64_t rem = len - off;")
+        parts.append("    const bool full = uni(J.vec_ok) && rem >= 32;")
+        parts.append("    const uint8_t *const *in = uni(J.ptrs);")
+        parts.append(f"    {name}(in, const_cast<uint8_t *const *>(in + {k}), off, full, rem);")
+        parts.append("  }")
+        parts.append("}")
+        parts.append("")
     parts.append("}  // namespace")
     parts.append("")
     parts.append("// Program id of a generated (route, k, m) program for BatchJob::prog, 0 if none.")
@@ -212,11 +208,17 @@ def generate() -> str:
     parts.append("  return 0;")
     parts.append("}")
     parts.append("")
-    parts.append("hipError_t launch_batch_bs(const BatchJob *jobs, const uint32_t *tile_job, "
+    parts.append(f"uint32_t bs_program_count() {{ return {len(table)}; }}")
+    parts.append("")
+    parts.append("hipError_t launch_batch_bs(uint32_t prog, const BatchJob *jobs, const uint32_t *tile_job, "
                  "uint64_t total_tiles, uint32_t grid, hipStream_t stream) {")
     parts.append("  if (total_tiles == 0) return hipSuccess;")
-    parts.append("  hipLaunchKernelGGL(k_batch_bs, dim3(grid), dim3(kBlockThreads), 0, stream, jobs,")
-    parts.append("                     tile_job, total_tiles);")
+    parts.append("  switch (prog) {")
+    for pid, (route, k, m, name) in enumerate(table, start=1):
+        parts.append(f"    case {pid}: hipLaunchKernelGGL(k_batch_{name}, dim3(grid), dim3(kBlockThreads), 0, "
+                     "stream, jobs, tile_job, total_tiles); break;")
+    parts.append("    default: return hipErrorNotSupported;")
+    parts.append("  }")
     parts.append("  return hipGetLastError();")
     parts.append("}")
     parts.append("")
