@@ -141,7 +141,91 @@ def served_leg(trs, torch, requests: int, rates, seed: int):
     return out
 
 
+C1_SEED = 0x7A9E_5EED_0000_0001
 TOWER_SEED = 0x7A9E_5EED_0000_0003
+
+
+def c1_leg(trs, torch, O, stripes: int = 256, L: int = 65536):
+    """RS8 (10,4), `stripes` stripes of 14 x L bytes resident in HBM, seeded
+    bytes: batched encode, then per stripe erase 4 of 14 shards chosen
+    uniformly (any data/parity mix) and batched reconstruct; byte-compared with
+    the pristine stripes and, on sampled stripes, the oracle's parity.  Also
+    times batched encode of the same shape at larger shards (the >= 256 KiB
+    regime) as a fraction of HBM peak."""
+    k, m = 10, 4
+    n = k + m
+    rs = trs.ReedSolomon(k, m)
+    buf = torch.empty((stripes, n, L), dtype=torch.uint8, device="cuda")
+    for s in range(stripes):
+        trs.fill_random(buf[s, :k].reshape(-1), C1_SEED + s)
+    rng = np.random.default_rng(C1_SEED)
+    erased = [sorted(rng.choice(n, 4, replace=False).tolist()) for _ in range(stripes)]
+    enc, rec = trs.Batch(), trs.Batch()
+    for s in range(stripes):
+        shards = [buf[s, i] for i in range(n)]
+        enc.add_encode(rs, shards)
+        rec.add_reconstruct(rs, shards, [i not in erased[s] for i in range(n)])
+    enc.prepare()
+    rec.prepare()
+    enc.set_graph(True)
+    rec.set_graph(True)
+    enc.run()
+    torch.cuda.synchronize()
+    pristine = buf.clone()
+    G = O.build_matrix(8, k, n)
+    oracle_ok = all(bool((buf[s, k:].cpu().numpy() ==
+                          O.rs8_encode(G, k, buf[s, :k].cpu().numpy())).all())
+                    for s in (0, stripes // 2, stripes - 1))
+    mask = torch.zeros((stripes, n), dtype=torch.bool, device="cuda")
+    for s in range(stripes):
+        mask[s, erased[s]] = True
+
+    def timed(batch, before=None, reps=10):
+        ev = []
+        for _ in range(reps):
+            if before is not None:
+                before()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            batch.run()
+            e1.record()
+            ev.append((e0, e1))
+        torch.cuda.synchronize()
+        return float(np.mean([a.elapsed_time(b) for a, b in ev]))
+
+    enc_ms = timed(enc)
+    rec_ms = timed(rec, lambda: buf.masked_fill_(mask[:, :, None], 0))
+    exact = bool(torch.equal(buf, pristine))
+    enc_bytes = stripes * n * L
+    rec_bytes = stripes * (k + 4) * L
+    del buf, pristine, enc, rec
+    torch.cuda.empty_cache()
+    large = []
+    for Lb, S in ((256 << 10, 256), (1 << 20, 128), (4 << 20, 32)):
+        t = torch.empty((S, n, Lb), dtype=torch.uint8, device="cuda")
+        trs.fill_random(t.reshape(-1), C1_SEED ^ Lb)
+        b = trs.Batch()
+        for s in range(S):
+            b.add_encode(rs, [t[s, i] for i in range(n)])
+        b.prepare()
+        b.set_graph(True)
+        b.run()
+        ms = timed(b)
+        gbs = S * n * Lb / ms / 1e6
+        large.append({"shard_bytes": Lb, "stripes": S, "ms": round(ms, 3), "gbs": round(gbs, 1),
+                      "hbm_frac": round(gbs / HBM_PEAK_GBS, 3)})
+        del t, b
+        torch.cuda.empty_cache()
+    return {
+        "workload": f"RS8 (10,4), {stripes} stripes of 14 x {L // 1024} KiB; batched encode, then "
+                    "4 uniformly erased shards per stripe and batched reconstruct",
+        "encode_bytes": enc_bytes, "encode_ms": round(enc_ms, 3),
+        "encode_gbs": round(enc_bytes / enc_ms / 1e6, 1),
+        "reconstruct_bytes": rec_bytes, "reconstruct_ms": round(rec_ms, 3),
+        "reconstruct_gbs": round(rec_bytes / rec_ms / 1e6, 1),
+        "large_shard_encode": large,
+        "byte_exact": {"reconstruct": exact, "oracle_sampled_parity": oracle_ok},
+    }
 TRAFFIC_FILE = "profiles/hbm_traffic.json"
 
 
@@ -262,10 +346,11 @@ def tower_leg(trs, torch, O, stripes: int = 64):
 
 def counter_run(trs, torch, stripes: int) -> int:
     """The launches whose HBM traffic `roofline.traffic` reports -- the
-    batched encode (k_batch_bs) and reconstruct (k_batch8) work lists of the
-    default workload and the tower FFT encode (k_fft16) of the tower workload,
-    three launches each -- then a calibration kernel of known traffic
-    (bitwise_not over 1 GiB: 1 GiB read + 1 GiB written)."""
+    batched encode (k_batch_bs_*) and reconstruct (k_batch8) work lists of the
+    default workload, the tower FFT encode (k_fft16) and FFT decode
+    (k_fft16_jobs) of the tower workload, three launches each -- then a
+    calibration kernel of known traffic (bitwise_not over 1 GiB: 1 GiB read +
+    1 GiB written)."""
     lib = trs._lib.lib
     sptr = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     meta = build_workload(stripes, np.random.default_rng(SEED))
@@ -299,6 +384,13 @@ def counter_run(trs, torch, stripes: int) -> int:
     trs.fill_random(t.reshape(-1), TOWER_SEED)
     for _ in range(3):
         r16.encode_stripes(t, "fft")
+    rng = np.random.default_rng(TOWER_SEED)
+    present = np.ones((64, 2048), bool)
+    for s in range(64):
+        present[s, rng.choice(2048, 1024, replace=False)] = False
+    r16.set_decode_route("fft")
+    for _ in range(3):
+        r16.reconstruct_stripes(t, present)
     torch.cuda.synchronize()
     del t
     torch.cuda.empty_cache()
@@ -321,6 +413,7 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=800)
     ap.add_argument("--no-tower", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--counters", action="store_true",
                     help="run only the measured launches (3 each) and a 1 GiB calibration "
                          "kernel, for rocprofv3 --pmc passes")
@@ -511,6 +604,9 @@ def main() -> int:
     tower = None
     if not args.no_tower:
         tower = tower_leg(trs, torch, O, args.tower_stripes)
+    c1 = None
+    if not args.no_c1:
+        c1 = c1_leg(trs, torch, O)
 
     served = None
     if not args.no_serve:
@@ -569,6 +665,7 @@ def main() -> int:
                                        "p99": round(float(np.percentile(lat, 99)), 1)},
         },
         "batched_vs_per_call": round(value / percall_gbs, 2),
+        "c1": c1,
         "tower": tower,
         "served": served,
         "byte_exact": {"batched_reconstruct": rec_exact, "batched_encode": enc_exact,
@@ -579,6 +676,8 @@ def main() -> int:
     ok = rec_exact and enc_exact and exact_after and exact_percall and host_ok
     if tower is not None:
         ok = ok and all(tower["byte_exact"].values())
+    if c1 is not None:
+        ok = ok and all(c1["byte_exact"].values())
     return 0 if ok else 1
 
 

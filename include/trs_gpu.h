@@ -138,7 +138,11 @@ void trs_rs16_free(trs_rs16 *rs);
 size_t trs_rs16_data_shard_count(const trs_rs16 *rs);
 size_t trs_rs16_parity_shard_count(const trs_rs16 *rs);
 size_t trs_rs16_total_shard_count(const trs_rs16 *rs);
-/* ReedSolomon16::encode_route, src/reedsolomon16.rs:338-346. */
+/* ReedSolomon16::encode_route, src/reedsolomon16.rs:338-346: "fft" or
+ * "matrix".  The reference routes per host (src/fft16.rs:70-106; both routes
+ * are byte-identical); here the route is "fft" whenever the shape has a staged
+ * program (2 <= k <= 1024, k + m <= 4096) whose register file fits a
+ * workgroup's LDS, the GPU's measured crossover. */
 const char *trs_rs16_encode_route(const trs_rs16 *rs);
 /* Host copy of the n x k tower generator (row-major, n*k uint16). */
 int trs_rs16_generator(const trs_rs16 *rs, uint16_t *out, size_t out_len);

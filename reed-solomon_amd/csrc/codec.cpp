@@ -412,9 +412,12 @@ struct trs_rs16 {
   PlanCache plans;
   std::atomic<int> decode_route{0};  // reconstruct: 0 auto, 1 matrix, 2 tower FFT
   // The staged tower-FFT program (src/reedsolomon16.rs:300-308), built for
-  // 2 <= k <= 1024; `staged` when the routing rule picks it.  Past k + m =
-  // 4096 a program has more than 2048 registers and never routes, so none is
-  // built.  The device form is uploaded on first use.
+  // 2 <= k <= 1024 and k + m <= 4096.  `staged`: the encode route takes it.
+  // The reference routes by a multiply-count rule measured on x86 GFNI
+  // (src/fft16.rs:85-106); on this GPU the interpreter beat the matrix kernel
+  // on 1771 of 1774 sampled tower shapes (the cost-model study), so the route
+  // is the FFT whenever the program's register file fits one workgroup's LDS.
+  // Both routes are byte-identical.  The device form is uploaded on first use.
   bool has_program = false;
   bool staged = false;
   Fft16Program program;
@@ -927,7 +930,7 @@ int trs_rs16_new(size_t k, size_t m, trs_rs16 **out) {
   if (k >= 2 && k <= kMaxFftDataShards && k + m <= 4096) {
     rs->program = build_staged_program16(k, m);
     rs->has_program = true;
-    rs->staged = fft16_profitable(rs->program, k, m);
+    rs->staged = fft16_fits_device(rs->program);
   }
   *out = rs.release();
   return TRS_OK;

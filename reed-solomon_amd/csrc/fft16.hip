@@ -696,6 +696,10 @@ __device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
 
 }  // namespace
 
+bool fft16_fits_device(const Fft16Program &p) {
+  return fft16_lower(p).regs * kRegBytes <= kLdsMax;
+}
+
 hipError_t fft16_device_upload(const Fft16Lowered &L, Fft16DevicePtr *out) {
   if (L.regs * kRegBytes > kLdsMax) return hipErrorInvalidValue;
   auto d = Fft16DevicePtr(new Fft16Device);

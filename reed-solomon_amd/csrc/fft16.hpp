@@ -102,6 +102,8 @@ struct Fft16Lowered {
   size_t regs = 0, max_count = 0;
 };
 Fft16Lowered fft16_lower(const Fft16Program &p);
+// Whether p's device register file fits one workgroup's LDS.
+bool fft16_fits_device(const Fft16Program &p);
 
 // WHAT[level][x] (src/fft16.rs:42-67), level < 11.
 uint16_t fft16_what(int level, size_t x);

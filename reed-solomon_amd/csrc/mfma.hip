@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
+#include "uniform.hpp"
 
 namespace trs {
 namespace {
@@ -209,7 +210,7 @@ template <int KS>
 __global__ __launch_bounds__(kBlockThreads) void k_gf8_mfma(const Gf8MfmaCall a) {
   const uint64_t wave_col = uint64_t(blockIdx.x) * kGroupCols + (threadIdx.x >> 6) * kWaveCols;
   if (wave_col >= a.len) return;
-  mfma_wave<KS>([&](uint32_t i) { return a.ptr[i]; }, a.frag, a.len, a.k, a.nout, blockIdx.y,
+  mfma_wave<KS>([&](uint32_t i) { return uni(a.ptr[i]); }, a.frag, a.len, a.k, a.nout, blockIdx.y,
                 wave_col, a.vec_ok != 0);
 }
 
@@ -220,25 +221,29 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch_mfma(const BatchJob *jo
                                                              const uint32_t *tile_job,
                                                              uint64_t total) {
   for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
-    const BatchJob &J = jobs[tile_job[t]];
-    const uint32_t passes = (J.nout + 7) / 8;
-    const uint64_t local = t - J.tile_begin;
+    // job fields are the same in every lane: keep them in scalar registers
+    const BatchJob &J = jobs[uni(tile_job[t])];
+    const uint32_t nout = uni(J.nout), k = uni(J.k);
+    const uint64_t len = uni(J.len);
+    const uint32_t passes = (nout + 7) / 8;
+    const uint64_t local = t - uni(J.tile_begin);
     const uint64_t group = local / passes;
     const uint32_t pass = uint32_t(local - group * passes);
-    const uint64_t wave_col = group * kGroupCols + (threadIdx.x >> 6) * kWaveCols;
-    if (wave_col >= J.len) continue;
-    const uint8_t *const *ptrs = J.ptrs;
-    auto at = [&](uint32_t i) { return ptrs[i]; };
-    const bool al = J.vec_ok != 0;
-    switch ((J.k + 3) / 4) {
-      case 1: mfma_wave<1>(at, J.coef, J.len, J.k, J.nout, pass, wave_col, al); break;
-      case 2: mfma_wave<2>(at, J.coef, J.len, J.k, J.nout, pass, wave_col, al); break;
-      case 3: mfma_wave<3>(at, J.coef, J.len, J.k, J.nout, pass, wave_col, al); break;
-      case 4: mfma_wave<4>(at, J.coef, J.len, J.k, J.nout, pass, wave_col, al); break;
-      case 5: mfma_wave<5>(at, J.coef, J.len, J.k, J.nout, pass, wave_col, al); break;
-      case 6: mfma_wave<6>(at, J.coef, J.len, J.k, J.nout, pass, wave_col, al); break;
-      case 7: mfma_wave<7>(at, J.coef, J.len, J.k, J.nout, pass, wave_col, al); break;
-      case 8: mfma_wave<8>(at, J.coef, J.len, J.k, J.nout, pass, wave_col, al); break;
+    const uint64_t wave_col = group * kGroupCols + uni(uint32_t(threadIdx.x >> 6)) * kWaveCols;
+    if (wave_col >= len) continue;
+    const uint8_t *const *ptrs = uni(J.ptrs);
+    const void *frag = uni(J.coef);
+    auto at = [&](uint32_t i) { return uni(ptrs[i]); };
+    const bool al = uni(J.vec_ok) != 0;
+    switch ((k + 3) / 4) {
+      case 1: mfma_wave<1>(at, frag, len, k, nout, pass, wave_col, al); break;
+      case 2: mfma_wave<2>(at, frag, len, k, nout, pass, wave_col, al); break;
+      case 3: mfma_wave<3>(at, frag, len, k, nout, pass, wave_col, al); break;
+      case 4: mfma_wave<4>(at, frag, len, k, nout, pass, wave_col, al); break;
+      case 5: mfma_wave<5>(at, frag, len, k, nout, pass, wave_col, al); break;
+      case 6: mfma_wave<6>(at, frag, len, k, nout, pass, wave_col, al); break;
+      case 7: mfma_wave<7>(at, frag, len, k, nout, pass, wave_col, al); break;
+      case 8: mfma_wave<8>(at, frag, len, k, nout, pass, wave_col, al); break;
       default: break;
     }
   }

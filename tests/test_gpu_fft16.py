@@ -51,7 +51,9 @@ def test_fft_route_unaligned_shards(trs):
 
 
 def test_routes(trs):
-    assert trs.ReedSolomon16(11, 20).encode_route() == "matrix"
+    # GPU routing: the FFT whenever the program fits (the reference's x86
+    # multiply-count rule would keep (11,20) on the matrix route).
+    assert trs.ReedSolomon16(11, 20).encode_route() == "fft"
     assert trs.ReedSolomon16(17, 33).encode_route() == "fft"
     assert trs.ReedSolomon16(1024, 1024).encode_route() == "fft"
     assert trs.ReedSolomon16(1, 5).encode_route() == "matrix"
