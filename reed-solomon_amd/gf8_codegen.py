@@ -145,6 +145,12 @@ def xor_cost(ops) -> int:
     return n
 
 
+# Programs whose batch kernels are compiled for three waves per SIMD (168
+# VGPRs).  Empty: forcing it spilled registers to scratch and measured slower
+# ((10,4): 3.67 -> 3.17 TB/s) or no faster ((7,13)) than two waves.
+THREE_WAVES = set()
+
+
 def generate() -> str:
     parts = [
         "// GENERATED by reed-solomon_amd/gf8_codegen.py -- do not edit.",
@@ -183,7 +189,10 @@ def generate() -> str:
     # the register budget of its own program: tiles are (job, 8 KiB column
     # chunk), numbered job by job; tile_job[t] names the job of tile t.
     for pid, (route, k, m, name) in enumerate(table, start=1):
-        parts.append(f"__global__ __launch_bounds__(kBlockThreads) void k_batch_{name}(const BatchJob *jobs,")
+        waves = ("__attribute__((amdgpu_waves_per_eu(3))) "
+                 if (route, k, m) in THREE_WAVES else "")
+        parts.append(f"__global__ __launch_bounds__(kBlockThreads) {waves}void k_batch_{name}("
+                     "const BatchJob *jobs,")
         parts.append("    const uint32_t *tile_job, uint64_t total) {")
         parts.append("  for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {")
         parts.append("    // job fields are the same in every lane: keep them in scalar registers")
