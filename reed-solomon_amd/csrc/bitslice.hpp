@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
+#include "uniform.hpp"
 
 namespace trs {
 
@@ -52,13 +53,14 @@ __device__ __forceinline__ void planes_to_bytes(uint32_t (&w)[8]) {
   swap_bits(w[6], w[7], 0x55555555u, 1);
 }
 
-// 32 bytes at p into planes; `full` lanes use two 16-byte loads, others load
-// the `rem` valid bytes and zero the rest.
+// 32 bytes at p into planes; `full` lanes use two 16-byte loads (non-temporal
+// when NT), others load the `rem` valid bytes and zero the rest.
+template <bool NT = false>
 __device__ __forceinline__ void load_planes(const uint8_t *p, bool full, uint64_t rem,
                                             uint32_t (&w)[8]) {
   if (full) {
-    const uint4 a = *reinterpret_cast<const uint4 *>(p);
-    const uint4 b = *reinterpret_cast<const uint4 *>(p + 16);
+    const uint4 a = ld16<NT>(p);
+    const uint4 b = ld16<NT>(p + 16);
     w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
     w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
   } else {
@@ -71,6 +73,7 @@ __device__ __forceinline__ void load_planes(const uint8_t *p, bool full, uint64_
   bytes_to_planes(w);
 }
 
+template <bool NT = false>
 __device__ __forceinline__ void store_planes(uint8_t *p, bool full, uint64_t rem,
                                              const uint32_t (&planes)[8]) {
   uint32_t w[8];
@@ -78,8 +81,8 @@ __device__ __forceinline__ void store_planes(uint8_t *p, bool full, uint64_t rem
   for (int i = 0; i < 8; ++i) w[i] = planes[i];
   planes_to_bytes(w);
   if (full) {
-    *reinterpret_cast<uint4 *>(p) = make_uint4(w[0], w[1], w[2], w[3]);
-    *reinterpret_cast<uint4 *>(p + 16) = make_uint4(w[4], w[5], w[6], w[7]);
+    st16<NT>(p, make_uint4(w[0], w[1], w[2], w[3]));
+    st16<NT>(p + 16, make_uint4(w[4], w[5], w[6], w[7]));
   } else {
 #pragma unroll
     for (int b = 0; b < 32; ++b)

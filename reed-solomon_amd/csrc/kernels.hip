@@ -45,7 +45,7 @@ __device__ __forceinline__ uint32_t xtime4(uint32_t x) {
 __device__ __forceinline__ void load_lane(const uint8_t *p, bool full, uint64_t rem,
                                           uint32_t x[4]) {
   if (full) {
-    const uint4 v = *reinterpret_cast<const uint4 *>(p);
+    const uint4 v = ld16(p);
     x[0] = v.x;
     x[1] = v.y;
     x[2] = v.z;
@@ -61,7 +61,7 @@ __device__ __forceinline__ void load_lane(const uint8_t *p, bool full, uint64_t 
 __device__ __forceinline__ void store_lane(uint8_t *p, bool full, uint64_t rem,
                                            const uint32_t x[4]) {
   if (full) {
-    *reinterpret_cast<uint4 *>(p) = make_uint4(x[0], x[1], x[2], x[3]);
+    st16(p, make_uint4(x[0], x[1], x[2], x[3]));
   } else {
 #pragma unroll
     for (int b = 0; b < 16; ++b)

@@ -27,4 +27,30 @@ __device__ __forceinline__ T *uni(T *p) {
   return reinterpret_cast<T *>(uni(uint64_t(reinterpret_cast<uintptr_t>(p))));
 }
 
+// 16-byte global accesses for shard data.  NT: non-temporal (no reuse
+// expected in the caches) -- right where every byte is read or written once
+// per launch (the generated encode programs); the table kernel re-reads
+// inputs per output tile and the tower interpreter shares 128-byte lines
+// between neighbouring workgroups through L2, so they use plain accesses.
+typedef unsigned int trs_v4u __attribute__((ext_vector_type(4)));
+
+template <bool NT = false>
+__device__ __forceinline__ uint4 ld16(const uint8_t *p) {
+  trs_v4u v;
+  if (NT)
+    v = __builtin_nontemporal_load(reinterpret_cast<const trs_v4u *>(p));
+  else
+    v = *reinterpret_cast<const trs_v4u *>(p);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+template <bool NT = false>
+__device__ __forceinline__ void st16(uint8_t *p, uint4 x) {
+  const trs_v4u v = {x.x, x.y, x.z, x.w};
+  if (NT)
+    __builtin_nontemporal_store(v, reinterpret_cast<trs_v4u *>(p));
+  else
+    *reinterpret_cast<trs_v4u *>(p) = v;
+}
+
 }  // namespace trs

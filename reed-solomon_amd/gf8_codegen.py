@@ -106,12 +106,12 @@ def emit_body(k: int, ops, nreg: int) -> str:
     out = []
     out.append(f"  uint32_t R[{nreg}][8];")
     for s in range(k):
-        out.append(f"  load_planes(uni(in[{s}]) + off, full, rem, R[{s}]);")
+        out.append(f"  load_planes<true>(uni(in[{s}]) + off, full, rem, R[{s}]);")
     for op in ops:
         kind = op[0]
         if kind == "st":
             _, j, r = op
-            out.append(f"  store_planes(uni(out[{j}]) + off, full, rem, R[{r}]);")
+            out.append(f"  store_planes<true>(uni(out[{j}]) + off, full, rem, R[{r}]);")
         elif kind == "zero":
             out.append(f"  for (int r = 0; r < 8; ++r) R[{op[1]}][r] = 0;")
         elif kind in ("xo", "cp"):
