@@ -264,7 +264,12 @@ int trs_batch8_add_encode(trs_batch8 *b, const trs_rs8 *rs, uint8_t *const *shar
 int trs_batch8_add_reconstruct(trs_batch8 *b, const trs_rs8 *rs, uint8_t *const *shards,
                                const size_t *lens, const uint8_t *present, size_t n_shards,
                                int data_only);
-/* Upload the work list for the queued requests (once per batch shape). */
+/* Upload the work list for the queued requests (once per batch shape).
+ * Requests whose shape has a generated program run it (one kernel per
+ * program); the others run the bit-sliced runtime-coefficient kernel
+ * (8 KiB tiles), or the packed-byte table kernel when the environment sets
+ * TRS_BATCH8_TILE=table.  TRS_BATCH_GRID_PER_CU / TRS_BATCH_BS_GRID_PER_CU
+ * set the persistent grids' workgroups per CU (default 8 / 4). */
 int trs_batch8_prepare(trs_batch8 *b);
 /* Run every queued request in one launch on `stream`. */
 int trs_batch8_run(trs_batch8 *b, void *stream);

@@ -16,7 +16,8 @@ import os
 import torch  # noqa: F401
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtrs_gpu.so")
+# TRS_GPU_LIB names another build of the library (for comparing builds).
+LIB_PATH = os.environ.get("TRS_GPU_LIB") or os.path.join(HERE, "libtrs_gpu.so")
 HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "trs_gpu.h")
 
 if not os.path.exists(LIB_PATH):

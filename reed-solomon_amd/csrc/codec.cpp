@@ -1399,6 +1399,9 @@ struct trs_batch8 {
   std::vector<Req> reqs;
   DevBuf ptrs;
   List table, mfma;
+  // Runtime-coefficient jobs run on the bit-sliced tile (k_batch8_planes)
+  // unless TRS_BATCH8_TILE=table selects the packed-byte table tile.
+  bool planes = true;
   static constexpr size_t kMaxPrograms = 32;
   List bs[kMaxPrograms];  // one work list per generated program id (index 0 unused)
   bool prepared = false;
@@ -1517,6 +1520,9 @@ int trs_batch8_prepare(trs_batch8 *b) {
   std::vector<std::vector<uint64_t>> tb(nprog);
   std::vector<uint64_t> tile_b(nprog, 0);
   uint64_t tile_t = 0, tile_m = 0;
+  b->planes = true;
+  if (const char *e = std::getenv("TRS_BATCH8_TILE")) b->planes = std::strcmp(e, "table") != 0;
+  const uint64_t table_chunk = b->planes ? kPlanesChunkBytes : kChunkBytes;
   const auto *dptr = static_cast<const uint8_t *const *>(b->ptrs.p);
   for (const auto &r : b->reqs) {
     BatchJob J{};
@@ -1548,7 +1554,7 @@ int trs_batch8_prepare(trs_batch8 *b) {
     } else {
       J.tile_begin = tile_t;
       tt.push_back(tile_t);
-      tile_t += ((r.len + kChunkBytes - 1) / kChunkBytes) * J.out_tiles;
+      tile_t += ((r.len + table_chunk - 1) / table_chunk) * J.out_tiles;
       jt.push_back(J);
     }
   }
@@ -1610,9 +1616,9 @@ int batch8_launch(trs_batch8 *b, hipStream_t s) {
     if (st) return st;
   }
   if (b->table.total_tiles)
-    return hip_status(launch_batch8(static_cast<const BatchJob *>(b->table.jobs.p),
-                                    static_cast<const uint32_t *>(b->table.tiles.p),
-                                    b->table.total_tiles, b->table.grid, s));
+    return hip_status((b->planes ? launch_batch8_planes : launch_batch8)(
+        static_cast<const BatchJob *>(b->table.jobs.p),
+        static_cast<const uint32_t *>(b->table.tiles.p), b->table.total_tiles, b->table.grid, s));
   return TRS_OK;
 }
 }  // namespace

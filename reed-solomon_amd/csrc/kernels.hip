@@ -15,8 +15,13 @@
 // global load per input), so a 256-lane workgroup streams a 4 KiB column
 // chunk of all inputs once, coalesced, and keeps up to 16 output rows in
 // VGPRs.  blockIdx.y tiles larger output counts.
+//
+// The batched work list runs a bit-sliced variant of the same apply
+// (k_batch8_planes, below): 32 bytes per lane as 8 bit planes, where the
+// powers x*2^b cost three XORs each instead of a packed xtime4.
 #include <hip/hip_runtime.h>
 
+#include "bitslice.hpp"
 #include "kernels.hpp"
 #include "uniform.hpp"
 
@@ -54,7 +59,7 @@ __device__ __forceinline__ void load_lane(const uint8_t *p, bool full, uint64_t 
     x[0] = x[1] = x[2] = x[3] = 0;
 #pragma unroll
     for (int b = 0; b < 16; ++b)
-      if (uint64_t(b) < rem) x[b >> 2] |= uint32_t(p[b]) << (8 * (b & 3));
+      if (uint64_t(b) < rem) x[b >> 2] |= uint32_t(ld1(p + b)) << (8 * (b & 3));
   }
 }
 
@@ -65,7 +70,7 @@ __device__ __forceinline__ void store_lane(uint8_t *p, bool full, uint64_t rem,
   } else {
 #pragma unroll
     for (int b = 0; b < 16; ++b)
-      if (uint64_t(b) < rem) p[b] = uint8_t(x[b >> 2] >> (8 * (b & 3)));
+      if (uint64_t(b) < rem) st1(p + b, uint8_t(x[b >> 2] >> (8 * (b & 3))));
   }
 }
 
@@ -90,7 +95,7 @@ __device__ __forceinline__ void mac_rows(uint32_t (&acc)[NT][4], const uint32_t 
     for (int w = 0; w < 4; ++w) p[b][w] = xtime4(p[b - 1][w]);
 #pragma unroll
   for (int q = 0; q < NT / 4; ++q) {
-    const uint32_t word = uni(cw[q]);  // same address in every lane
+    const uint32_t word = ld_const(cw + q);  // same address in every lane: s_load
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const uint32_t c = (word >> (8 * s)) & 0xffu;  // scalar
@@ -119,12 +124,10 @@ __device__ __forceinline__ void mac_rows(uint32_t (&acc)[NT][4], const uint32_t 
 // One lane's share of a GF(2^8) apply tile: outputs jbase..jbase+nt over the
 // 16 bytes at `off`.  ptr_at(i) returns shard pointer i (inputs first).
 template <int NT, uint32_t MODE, typename PtrAt>
-__device__ __forceinline__ void gf8_tile(PtrAt ptr_at, const uint32_t *coef, uint64_t len,
-                                         uint32_t k, uint32_t nout, uint32_t ostride,
-                                         bool vec_ok, uint32_t jbase, uint64_t off,
-                                         uint32_t *flag) {
-  const uint64_t rem = len - off;
-  const bool full = vec_ok && rem >= 16;
+__device__ __forceinline__ void gf8_tile_body(PtrAt ptr_at, const uint32_t *coef, uint64_t rem,
+                                              bool full, uint32_t k, uint32_t nout,
+                                              uint32_t ostride, uint32_t jbase, uint64_t off,
+                                              uint32_t *flag) {
   const uint32_t nt = min(uint32_t(NT), nout - jbase);
   uint32_t acc[NT][4];
 #pragma unroll
@@ -162,12 +165,142 @@ __device__ __forceinline__ void gf8_tile(PtrAt ptr_at, const uint32_t *coef, uin
   if (MODE == kVerify && bad) *flag = 1u;
 }
 
+// The full / tail choice is made once per tile: the full path is then
+// straight-line code whose loads can all be in flight together, where a
+// per-load branch would put a wait after every load.
+template <int NT, uint32_t MODE, typename PtrAt>
+__device__ __forceinline__ void gf8_tile(PtrAt ptr_at, const uint32_t *coef, uint64_t len,
+                                         uint32_t k, uint32_t nout, uint32_t ostride,
+                                         bool vec_ok, uint32_t jbase, uint64_t tile,
+                                         uint64_t off, uint32_t *flag) {
+  // Every active lane has 16 bytes in bounds when the shard is aligned and its
+  // length a multiple of 16 or the whole tile is in bounds; the choice is the
+  // same for all lanes of the tile.
+  const uint64_t rem = len - off;
+  if (vec_ok && ((len & 15) == 0 || tile + kChunkBytes <= len))
+    gf8_tile_body<NT, MODE>(ptr_at, coef, rem, true, k, nout, ostride, jbase, off, flag);
+  else
+    gf8_tile_body<NT, MODE>(ptr_at, coef, rem, false, k, nout, ostride, jbase, off, flag);
+}
+
+// ---- Bit-sliced apply with runtime coefficients ----
+//
+// A lane holds 32 bytes of a shard as 8 bit planes (bitslice.hpp, split
+// layout).  In plane form multiplying by 2 (poly 0x11d: 2*v = v<<1 ^ (v7 ?
+// 0x1d : 0)) moves plane b to b+1 and XORs the old top plane into planes 0
+// (as a move), 2, 3 and 4: three XORs per 32 bytes, against ~40 operations
+// for the packed-byte xtime4.  c*x is the XOR of x*2^b over the set bits b of
+// c, so each input walks y = x, 2x, 4x, ... and XORs y into every output
+// whose coefficient has bit b set; the bit tests are scalar branches on the
+// wave-uniform coefficient words.
+
+__device__ __forceinline__ void planes_times2(uint32_t (&y)[8]) {
+  const uint32_t t = y[7];
+  y[7] = y[6];
+  y[6] = y[5];
+  y[5] = y[4];
+  y[4] = y[3] ^ t;
+  y[3] = y[2] ^ t;
+  y[2] = y[1] ^ t;
+  y[1] = y[0];
+  y[0] = t;
+}
+
+// acc[j] ^= c_j * y for the NT coefficients at cw (same layout as mac_rows);
+// y is consumed (left at y * 2^7).
+template <int NT>
+__device__ __forceinline__ void mac_planes(uint32_t (&acc)[NT][8], uint32_t (&y)[8],
+                                           const uint32_t *cw) {
+  uint32_t word[NT / 4];
+#pragma unroll
+  for (int q = 0; q < NT / 4; ++q) word[q] = ld_const(cw + q);  // wave-uniform: s_load
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+      if ((word[j >> 2] >> (8 * (j & 3) + b)) & 1u) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) acc[j][r] ^= y[r];
+      }
+    if (b < 7) planes_times2(y);
+  }
+}
+
+// One lane's share of a bit-sliced apply tile: outputs jbase..jbase+nt over
+// the lane's split 32 bytes whose first piece is at `off` (an active lane:
+// off < len).  The next input's bytes are loaded before the current input's
+// multiply-accumulate.
+template <int NT, uint32_t MODE, typename PtrAt>
+__device__ __forceinline__ void gf8_planes_body(PtrAt ptr_at, const uint32_t *coef, uint64_t len,
+                                                int mode, uint32_t k, uint32_t nout,
+                                                uint32_t ostride, uint32_t jbase, uint64_t off,
+                                                uint32_t *flag) {
+  const uint32_t nt = min(uint32_t(NT), nout - jbase);
+  uint32_t acc[NT][8];
+#pragma unroll
+  for (int j = 0; j < NT; ++j)
+#pragma unroll
+    for (int r = 0; r < 8; ++r) acc[j][r] = 0;
+
+  uint32_t nxt[8];
+  load_split_raw(ptr_at(0), off, len, mode, nxt);
+  for (uint32_t i = 0; i < k; ++i) {
+    uint32_t y[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) y[r] = nxt[r];
+    if (i + 1 < k) load_split_raw(ptr_at(i + 1), off, len, mode, nxt);
+    bytes_to_planes(y);
+    mac_planes<NT>(acc, y, coef + ((size_t)i * ostride + jbase) / 4);
+  }
+
+  uint32_t bad = 0;
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    if (uint32_t(j) < nt) {
+      uint8_t *o = const_cast<uint8_t *>(ptr_at(k + jbase + j));
+      if (MODE == kWrite) {
+        store_split(o, off, len, mode, acc[j]);
+      } else {
+        uint32_t y[8];
+        load_split(o, off, len, mode, y);
+        if (MODE == kVerify) {
+#pragma unroll
+          for (int r = 0; r < 8; ++r) bad |= y[r] ^ acc[j][r];
+        } else {
+#pragma unroll
+          for (int r = 0; r < 8; ++r) y[r] ^= acc[j][r];
+          store_split(o, off, len, mode, y);
+        }
+      }
+    }
+  }
+  if (MODE == kVerify && bad) *flag = 1u;
+}
+
+// One branch per tile on its access mode (split_mode), so the common paths
+// are straight-line copies of the body.
+template <int NT, uint32_t MODE, typename PtrAt>
+__device__ __forceinline__ void gf8_tile_planes(PtrAt ptr_at, const uint32_t *coef, uint64_t len,
+                                                uint32_t k, uint32_t nout, uint32_t ostride,
+                                                bool vec_ok, uint32_t jbase, uint64_t tile,
+                                                uint64_t off, uint32_t *flag) {
+  const int mode = split_mode(vec_ok, tile, len);
+  if (mode == kSplitFull)
+    gf8_planes_body<NT, MODE>(ptr_at, coef, len, kSplitFull, k, nout, ostride, jbase, off, flag);
+  else if (mode == kSplitMasked)
+    gf8_planes_body<NT, MODE>(ptr_at, coef, len, kSplitMasked, k, nout, ostride, jbase, off,
+                              flag);
+  else
+    gf8_planes_body<NT, MODE>(ptr_at, coef, len, mode, k, nout, ostride, jbase, off, flag);
+}
+
 template <int NT, uint32_t MODE>
 __global__ __launch_bounds__(kBlockThreads) void k_gf8_apply(const Gf8Call a) {
-  const uint64_t off = (uint64_t(blockIdx.x) * kBlockThreads + threadIdx.x) * kLaneBytes;
+  const uint64_t tile = uint64_t(blockIdx.x) * kChunkBytes;
+  const uint64_t off = tile + uint64_t(threadIdx.x) * kLaneBytes;
   if (off >= a.len) return;
   gf8_tile<NT, MODE>([&](uint32_t idx) { return uni(a.ptr[idx]); }, a.coef, a.len, a.k, a.nout,
-                     a.ostride, a.vec_ok != 0, blockIdx.y * NT, off, a.flag);
+                     a.ostride, a.vec_ok != 0, blockIdx.y * NT, tile, off, a.flag);
 }
 
 // Batched work list: tiles are (job, 4 KiB column chunk, 16-row output tile),
@@ -178,18 +311,42 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch8(const BatchJob *jobs,
                                                          const uint32_t *tile_job,
                                                          uint64_t total) {
   for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
-    const BatchJob &J = jobs[uni(tile_job[t])];
-    const uint64_t len = uni(J.len);
-    const uint32_t out_tiles = uni(J.out_tiles);
-    const uint64_t local = t - uni(J.tile_begin);
+    const auto &J = ld_const_ref(jobs + ld_const(tile_job + t));
+    const uint64_t len = J.len;
+    const uint32_t out_tiles = J.out_tiles;
+    const uint64_t local = t - J.tile_begin;
     const uint64_t chunk = local / out_tiles;
     const uint32_t ot = uint32_t(local - chunk * out_tiles);
-    const uint64_t off = chunk * kChunkBytes + uint64_t(threadIdx.x) * kLaneBytes;
+    const uint64_t tile = chunk * kChunkBytes;
+    const uint64_t off = tile + uint64_t(threadIdx.x) * kLaneBytes;
     if (off < len) {
-      const uint8_t *const *ptrs = uni(J.ptrs);
-      gf8_tile<kOutStride, kWrite>([&](uint32_t idx) { return uni(ptrs[idx]); }, uni(J.coef),
-                                   len, uni(J.k), uni(J.nout), uni(J.ostride),
-                                   uni(J.vec_ok) != 0, ot * kOutStride, off, nullptr);
+      const uint8_t *const *ptrs = J.ptrs;
+      gf8_tile<kOutStride, kWrite>([&](uint32_t idx) { return ld_const_ptr(ptrs + idx); },
+                                   J.coef, len, J.k, J.nout, J.ostride, J.vec_ok != 0,
+                                   ot * kOutStride, tile, off, nullptr);
+    }
+  }
+}
+
+// The same work list over the bit-sliced tile: tiles are (job, 8 KiB column
+// chunk, 16-row output tile).
+__global__ __launch_bounds__(kBlockThreads) void k_batch8_planes(const BatchJob *jobs,
+                                                                const uint32_t *tile_job,
+                                                                uint64_t total) {
+  for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
+    const auto &J = ld_const_ref(jobs + ld_const(tile_job + t));
+    const uint64_t len = J.len;
+    const uint32_t out_tiles = J.out_tiles;
+    const uint64_t local = t - J.tile_begin;
+    const uint64_t chunk = local / out_tiles;
+    const uint32_t ot = uint32_t(local - chunk * out_tiles);
+    const uint64_t tile = chunk * kPlanesChunkBytes;
+    const uint64_t off = tile + split_lane_offset(threadIdx.x);
+    if (off < len) {
+      const uint8_t *const *ptrs = J.ptrs;
+      gf8_tile_planes<kOutStride, kWrite>([&](uint32_t idx) { return ld_const_ptr(ptrs + idx); },
+                                          J.coef, len, J.k, J.nout, J.ostride, J.vec_ok != 0,
+                                          ot * kOutStride, tile, off, nullptr);
     }
   }
 }
@@ -200,12 +357,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch8(const BatchJob *jobs,
 // out_lo ^= t0^t2, out_hi ^= t0^t1 (src/gf/tower_x86.rs:80-93).  The three
 // product sums are accumulated separately and combined at the end.
 template <int NT, uint32_t MODE>
-__global__ __launch_bounds__(kBlockThreads) void k_gf16_apply(const Gf16Call a) {
-  const uint64_t rel = (uint64_t(blockIdx.x) * kBlockThreads + threadIdx.x) * kLaneBytes;
-  if (rel >= a.cols) return;
-  const uint64_t off = a.col0 + rel;
-  const uint64_t rem = a.cols - rel;
-  const bool full = a.vec_ok && rem >= 16;
+__device__ __forceinline__ void gf16_body(const Gf16Call &a, uint64_t off, uint64_t rem,
+                                          bool full) {
   const uint32_t jbase = blockIdx.y * NT;
   const uint32_t nt = min(uint32_t(NT), a.nout - jbase);
   const size_t csz = (size_t)a.k * a.ostride / 4;
@@ -216,7 +369,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_gf16_apply(const Gf16Call a) 
     for (int w = 0; w < 4; ++w) A[j][w] = B[j][w] = C[j][w] = 0;
 
   for (uint32_t i = 0; i < a.k; ++i) {
-    const uint8_t *s = uni(a.ptrs[i]);
+    const uint8_t *s = ld_const_ptr(a.ptrs + i);
     uint32_t lo[4], hi[4], sum[4];
     load_lane(s + off, full, rem, lo);
     load_lane(s + a.plane + off, full, rem, hi);
@@ -232,7 +385,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_gf16_apply(const Gf16Call a) 
 #pragma unroll
   for (int j = 0; j < NT; ++j) {
     if (uint32_t(j) < nt) {
-      uint8_t *o = const_cast<uint8_t *>(a.ptrs[a.k + jbase + j]);
+      uint8_t *o = const_cast<uint8_t *>(ld_const_ptr(a.ptrs + a.k + jbase + j));
       uint32_t olo[4], ohi[4];
 #pragma unroll
       for (int w = 0; w < 4; ++w) {
@@ -252,6 +405,19 @@ __global__ __launch_bounds__(kBlockThreads) void k_gf16_apply(const Gf16Call a) 
     }
   }
   if (MODE == kVerify && bad) *a.flag = 1u;
+}
+
+// The full / tail choice is the same for every lane of a workgroup (as in
+// gf8_tile).
+template <int NT, uint32_t MODE>
+__global__ __launch_bounds__(kBlockThreads) void k_gf16_apply(const Gf16Call a) {
+  const uint64_t tile = uint64_t(blockIdx.x) * kChunkBytes;
+  const uint64_t rel = tile + uint64_t(threadIdx.x) * kLaneBytes;
+  if (rel >= a.cols) return;
+  if (a.vec_ok && ((a.cols & 15) == 0 || tile + kChunkBytes <= a.cols))
+    gf16_body<NT, MODE>(a, a.col0 + rel, a.cols - rel, true);
+  else
+    gf16_body<NT, MODE>(a, a.col0 + rel, a.cols - rel, false);
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -327,6 +493,14 @@ hipError_t launch_batch8(const BatchJob *jobs, const uint32_t *tile_job, uint64_
                          uint32_t grid, hipStream_t stream) {
   if (total_tiles == 0) return hipSuccess;
   hipLaunchKernelGGL(k_batch8, dim3(grid), dim3(kBlockThreads), 0, stream, jobs, tile_job,
+                     total_tiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch8_planes(const BatchJob *jobs, const uint32_t *tile_job,
+                                uint64_t total_tiles, uint32_t grid, hipStream_t stream) {
+  if (total_tiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch8_planes, dim3(grid), dim3(kBlockThreads), 0, stream, jobs, tile_job,
                      total_tiles);
   return hipGetLastError();
 }

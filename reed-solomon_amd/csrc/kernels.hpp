@@ -83,6 +83,11 @@ hipError_t launch_gf16_apply(const Gf16Call &call, hipStream_t stream);
 // Batched persistent kernels: tile_job[t] is the job index of tile t.
 hipError_t launch_batch8(const BatchJob *jobs, const uint32_t *tile_job, uint64_t total_tiles,
                          uint32_t grid, hipStream_t stream);
+// The same work list on the bit-sliced runtime-coefficient tile (8 KiB
+// column chunks, 32 bytes per lane as bit planes).
+constexpr uint64_t kPlanesChunkBytes = 8192;
+hipError_t launch_batch8_planes(const BatchJob *jobs, const uint32_t *tile_job,
+                                uint64_t total_tiles, uint32_t grid, hipStream_t stream);
 hipError_t launch_fill(uint8_t *dst, uint64_t len, uint64_t seed, hipStream_t stream);
 
 // int8-MFMA apply (mfma.hip); available for k <= 32.

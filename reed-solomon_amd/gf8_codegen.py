@@ -1,6 +1,7 @@
 """Generates the bit-sliced GF(2^8) encode kernels (build step).
 
-Bit-sliced form: a lane loads 32 bytes of a shard (two 16-byte loads) and
+Bit-sliced form: a lane loads 32 bytes of a shard (two 16-byte pieces 1 KiB
+apart, so each wave-wide load covers 1 KiB contiguous; bitslice.hpp) and
 transposes them into 8 planes, plane b holding bit b of all 32 bytes.
 Multiplying by a constant c is then the 8x8 GF(2) matrix of "x -> c*x"
 (the same matrix GFNI's affine instruction takes, src/gf/gfni.rs:41-64):
@@ -102,16 +103,27 @@ def fft_program(k: int, m: int):
     return ops, p.register_count
 
 
+def _dispatch(name: str, ins: str, len_: str, vec: str, ind: str) -> str:
+    """One branch per tile on its access mode (bitslice.hpp, split_mode): the
+    full and masked paths get straight-line copies of the program."""
+    return "\n".join([
+        f"{ind}const int mode = split_mode({vec}, tile, {len_});",
+        f"{ind}if (mode == kSplitFull) {name}({ins}, out, off, {len_}, kSplitFull);",
+        f"{ind}else if (mode == kSplitMasked) {name}({ins}, out, off, {len_}, kSplitMasked);",
+        f"{ind}else {name}({ins}, out, off, {len_}, mode);",
+    ])
+
+
 def emit_body(k: int, ops, nreg: int) -> str:
     out = []
     out.append(f"  uint32_t R[{nreg}][8];")
     for s in range(k):
-        out.append(f"  load_planes<true>(uni(in[{s}]) + off, full, rem, R[{s}]);")
+        out.append(f"  load_split<true>(ld_const_ptr(in + {s}), off, len, mode, R[{s}]);")
     for op in ops:
         kind = op[0]
         if kind == "st":
             _, j, r = op
-            out.append(f"  store_planes<true>(uni(out[{j}]) + off, full, rem, R[{r}]);")
+            out.append(f"  store_split<true>(ld_const_ptr(out + {j}), off, len, mode, R[{r}]);")
         elif kind == "zero":
             out.append(f"  for (int r = 0; r < 8; ++r) R[{op[1]}][r] = 0;")
         elif kind in ("xo", "cp"):
@@ -145,10 +157,21 @@ def xor_cost(ops) -> int:
     return n
 
 
-# Programs whose batch kernels are compiled for three waves per SIMD (168
-# VGPRs).  Empty: forcing it spilled registers to scratch and measured slower
-# ((10,4): 3.67 -> 3.17 TB/s) or no faster ((7,13)) than two waves.
-THREE_WAVES = set()
+# Waves per SIMD the kernels are compiled for (the register budget is
+# 512 / waves VGPRs).  Two for every program but these, which need more than
+# 256 VGPRs without spilling: (32,32) fft keeps ~60 plane registers live, and
+# the compiler interleaves the output rows of the wide constant-matrix
+# programs.  With the full/tail branch taken once per tile the compiler
+# issues loads early, which raises register pressure: unbounded, (18,6) fft
+# fell to one wave.  Three waves spilled to scratch and measured slower
+# ((10,4): 3.67 -> 3.17 TB/s).
+ONE_WAVE = {("fft", 32, 32), ("mat", 7, 13), ("mat", 10, 10), ("mat", 18, 6)}
+
+
+def _waves_attr(route: str, k: int, m: int) -> str:
+    if (route, k, m) in ONE_WAVE:
+        return ""
+    return "__attribute__((amdgpu_waves_per_eu(2))) "
 
 
 def generate() -> str:
@@ -173,37 +196,40 @@ def generate() -> str:
             parts.append(f"// ({k},{m}) {route}: {nreg} registers, ~{xor_cost(ops)} plane XORs "
                          "per 32-byte lane chunk")
             parts.append(f"__device__ __forceinline__ void {name}(const uint8_t *const *in, "
-                         "uint8_t *const *out, uint64_t off, bool full, uint64_t rem) {")
+                         "uint8_t *const *out, uint64_t off, uint64_t len, int mode) {")
             parts.append(emit_body(k, ops, nreg))
             parts.append("}")
-            parts.append(f"__global__ __launch_bounds__(kBlockThreads) void k_{name}(const Gf8Call a) {{")
-            parts.append("  const uint64_t off = (uint64_t(blockIdx.x) * kBlockThreads + threadIdx.x) * 32;")
+            parts.append(f"__global__ __launch_bounds__(kBlockThreads) {_waves_attr(route, k, m)}"
+                         f"void k_{name}(const Gf8Call a) {{")
+            parts.append("  const uint64_t tile = uint64_t(blockIdx.x) * 8192;")
+            parts.append("  const uint64_t off = tile + split_lane_offset(threadIdx.x);")
             parts.append("  if (off >= a.len) return;")
-            parts.append("  const uint64_t rem = a.len - off;")
-            parts.append(f"  {name}(a.ptr, const_cast<uint8_t *const *>(a.ptr + {k}), off, "
-                         "a.vec_ok && rem >= 32, rem);")
+            parts.append(f"  uint8_t *const *out = const_cast<uint8_t *const *>(a.ptr + {k});")
+            parts.append(_dispatch(name, "a.ptr", "a.len", "a.vec_ok != 0", "  "))
             parts.append("}")
             parts.append("")
             table.append((route, k, m, name))
     # Persistent work-list kernels, one per generated program so that each gets
     # the register budget of its own program: tiles are (job, 8 KiB column
-    # chunk), numbered job by job; tile_job[t] names the job of tile t.
+    # chunk), numbered job by job; tile_job[t] names the job of tile t.  The
+    # full/tail choice is made once per tile, so the full path is straight-line
+    # code whose loads can all be in flight together (a per-load branch would
+    # put a wait after every load).
     for pid, (route, k, m, name) in enumerate(table, start=1):
-        waves = ("__attribute__((amdgpu_waves_per_eu(3))) "
-                 if (route, k, m) in THREE_WAVES else "")
+        waves = _waves_attr(route, k, m)
         parts.append(f"__global__ __launch_bounds__(kBlockThreads) {waves}void k_batch_{name}("
                      "const BatchJob *jobs,")
         parts.append("    const uint32_t *tile_job, uint64_t total) {")
         parts.append("  for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {")
-        parts.append("    // job fields are the same in every lane: keep them in scalar registers")
-        parts.append("    const BatchJob &J = jobs[uni(tile_job[t])];")
-        parts.append("    const uint64_t len = uni(J.len);")
-        parts.append("    const uint64_t off = (t - uni(J.tile_begin)) * 8192 + uint64_t(threadIdx.x) * 32;")
+        parts.append("    // job fields are the same in every lane: scalar loads")
+        parts.append("    const auto &J = ld_const_ref(jobs + ld_const(tile_job + t));")
+        parts.append("    const uint64_t len = J.len;")
+        parts.append("    const uint64_t tile = (t - J.tile_begin) * 8192;")
+        parts.append("    const uint64_t off = tile + split_lane_offset(threadIdx.x);")
         parts.append("    if (off >= len) continue;")
-        parts.append("    const uint64_t rem = len - off;")
-        parts.append("    const bool full = uni(J.vec_ok) && rem >= 32;")
-        parts.append("    const uint8_t *const *in = uni(J.ptrs);")
-        parts.append(f"    {name}(in, const_cast<uint8_t *const *>(in + {k}), off, full, rem);")
+        parts.append("    const uint8_t *const *in = J.ptrs;")
+        parts.append(f"    uint8_t *const *out = const_cast<uint8_t *const *>(in + {k});")
+        parts.append(_dispatch(name, "in", "len", "J.vec_ok != 0", "    "))
         parts.append("  }")
         parts.append("}")
         parts.append("")
@@ -235,8 +261,7 @@ def generate() -> str:
     parts.append("// without a generated program.")
     parts.append("hipError_t launch_bs_encode(int route, const Gf8Call &a, hipStream_t stream) {")
     parts.append("  if (a.len == 0) return hipSuccess;")
-    parts.append("  const uint64_t lanes = (a.len + 31) / 32;")
-    parts.append("  const dim3 grid(uint32_t((lanes + kBlockThreads - 1) / kBlockThreads));")
+    parts.append("  const dim3 grid(uint32_t((a.len + 8191) / 8192));  // one 8 KiB tile per workgroup")
     for route, k, m, name in table:
         rid = 1 if route == "mat" else 2
         parts.append(f"  if (route == {rid} && a.k == {k} && a.nout == {m}) {{")
