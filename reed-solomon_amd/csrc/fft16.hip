@@ -568,7 +568,7 @@ __global__ __launch_bounds__(T) void k_fft16_jobs(const Fft16Args *jobs, const u
                                                   uint32_t total) {
   extern __shared__ uint4 rf[];
   const uint32_t wg = xcd_order(blockIdx.x, total);
-  const Fft16Args a = jobs[wg_job[wg]];
+  const Fft16Args a = ld_const_val(jobs + ld_const(wg_job + wg));  // scalar loads
   run_program<T>(a, (wg - a.total) << a.gshift, a.ptrs, rf);
 }
 
@@ -603,7 +603,9 @@ __device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
   __syncthreads();
 
   for (uint32_t si = 0; si < a.nsteps; ++si) {
-    const Fft16Step st = a.steps[si];
+    // step descriptors are wave-uniform: scalar loads, not a vector-memory
+    // round trip at the head of every step
+    const Fft16Step st = ld_const_val(a.steps + si);
     const uint32_t n = st.count << a.gshift;
     if (st.kind == 0) {
       const uint32_t sl = st.b, smask = (1u << sl) - 1;

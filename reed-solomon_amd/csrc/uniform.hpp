@@ -86,6 +86,20 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T &ld_const_r
   return *(const __attribute__((address_space(4))) T *)(p);
 }
 
+// A copy of a read-only struct at a wave-uniform address, loaded word by word
+// through the scalar data cache (T: a multiple of 4 bytes, 4-byte aligned).
+template <class T>
+__device__ __forceinline__ T ld_const_val(const T *p) {
+  static_assert(sizeof(T) % 4 == 0, "word-sized struct");
+  T v;
+  uint32_t *dst = reinterpret_cast<uint32_t *>(&v);
+  const __attribute__((address_space(4))) uint32_t *src =
+      (const __attribute__((address_space(4))) uint32_t *)(p);
+#pragma unroll
+  for (unsigned i = 0; i < sizeof(T) / 4; ++i) dst[i] = src[i];
+  return v;
+}
+
 // An entry of a read-only pointer table at a wave-uniform address (s_load).
 template <class T>
 __device__ __forceinline__ T *ld_const_ptr(T *const *p) {
