@@ -114,11 +114,14 @@ int trs_rs8_reconstruct(const trs_rs8 *rs, uint8_t *const *shards, const size_t 
 /* ReedSolomon::reconstruct_rows, src/reedsolomon.rs:514-535. */
 int trs_rs8_reconstruct_rows(const trs_rs8 *rs, uint8_t *rows, size_t rows_len, size_t row_len,
                              const uint8_t *present, size_t n_present, void *stream);
-/* Reconstruct apply route of this codec (and of decoders and batch requests
- * created after the call): 0 = automatic, 1 = fused table kernel, 2 = int8
- * MFMA on the bit-expanded plan (k <= 32; TRS_ERR_UNSUPPORTED otherwise).
- * Every route is byte-identical; no reference counterpart (the reference has
- * one CPU kernel per ISA). */
+/* Matrix-apply route of this codec (reconstruct, and encode of shapes without
+ * a generated program; also decoders and batch requests created after the
+ * call): 0 = automatic, 1 = fused table kernel, 2 = int8 MFMA on the
+ * bit-expanded rows (k <= 32; TRS_ERR_UNSUPPORTED otherwise).  Automatic
+ * takes MFMA for single calls with k <= 32 and shards under 512 KiB (about
+ * twice as fast there, measured) and the table kernel otherwise; batch work
+ * lists use the bit-sliced planes kernel.  Every route is byte-identical; no
+ * reference counterpart (the reference has one CPU kernel per ISA). */
 int trs_rs8_set_apply_route(trs_rs8 *rs, int route);
 /* ReedSolomon::prepare_decode, src/reedsolomon.rs:484-507. */
 int trs_rs8_prepare_decode(const trs_rs8 *rs, const uint8_t *present, size_t n_present,

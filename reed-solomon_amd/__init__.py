@@ -328,8 +328,10 @@ class ReedSolomon(_Codec):
     APPLY_ROUTES = {"auto": 0, "table": 1, "mfma": 2}
 
     def set_apply_route(self, route: str) -> None:
-        """Device route for reconstruct applies: "auto", "table" or "mfma"
-        (int8 MFMA over the bit-expanded plan, k <= 32).  All byte-identical."""
+        """Device route for matrix applies (reconstruct; encode of shapes
+        without a generated program): "auto", "table" or "mfma" (int8 MFMA over
+        the bit-expanded rows, k <= 32).  "auto" takes MFMA for single calls
+        with k <= 32 and shards under 512 KiB.  All byte-identical."""
         _check(lib.trs_rs8_set_apply_route(self._h, self.APPLY_ROUTES[route]))
 
     def encode_fused(self, shards: Sequence) -> None:

@@ -393,9 +393,10 @@ using namespace trs;
 struct trs_rs8 {
   size_t k = 0, m = 0, n = 0;
   int route = kRouteTable;  // encode route: bit-sliced FFT > bit-sliced matrix > table kernel
-  std::atomic<int> apply_route{0};  // reconstruct apply: 0 auto, 1 table kernel, 2 int8 MFMA
+  std::atomic<int> apply_route{0};  // matrix applies: 0 auto, 1 table kernel, 2 int8 MFMA
   Matrix<F8> G;
   DevCoef parity;
+  DevBuf parity_frag;  // MFMA A fragments of the parity rows (k <= 32)
   PlanCache plans;
 };
 
@@ -543,6 +544,39 @@ int run_gf8(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout, con
   return hip_status(launch_gf8_apply(c, mode, stream));
 }
 
+// Route of a reconstruct apply.  Measured on MI355X (per-call applies of
+// 226 sampled GF(2^8) shapes with k <= 32): the int8-MFMA kernel is about 2x
+// faster than the table kernel below 256 KiB shards and still ahead below
+// 512 KiB -- its 2048-column workgroup tiles give a small launch more
+// workgroups -- and slower above.  Batched work lists keep the bit-sliced
+// planes tile, 3.4x faster than the batched MFMA kernel on the mixed C2
+// reconstruct.  Route 2 forces MFMA, route 1 the table kernel.
+constexpr size_t kMfmaAutoMaxLen = size_t(512) << 10;
+
+bool use_mfma(int route, size_t k, size_t len, bool batched) {
+  if (!mfma_available(uint32_t(k))) return false;
+  if (route == 2) return true;
+  return route == 0 && !batched && len < kMfmaAutoMaxLen;
+}
+
+int run_mfma(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout, const void *frag,
+             size_t len, hipStream_t stream) {
+  if (nout == 0 || len == 0) return TRS_OK;
+  if (k + nout > size_t(kInlinePtrs)) return TRS_ERR_INVALID_ARGUMENT;
+  Gf8MfmaCall c{};
+  bool al = true;
+  for (size_t i = 0; i < k + nout; ++i) {
+    c.ptr[i] = ptrs[i];
+    al = al && (reinterpret_cast<uintptr_t>(ptrs[i]) & 3) == 0;
+  }
+  c.frag = frag;
+  c.len = len;
+  c.k = uint32_t(k);
+  c.nout = uint32_t(nout);
+  c.vec_ok = al ? 1u : 0u;
+  return hip_status(launch_gf8_mfma(c, stream));
+}
+
 // Split-plane shards of `len` bytes (plane = len / 2).
 int run_gf16(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout,
              const DevCoef &coef, size_t len, uint32_t mode, uint32_t *flag,
@@ -651,8 +685,9 @@ int trs_rs8_new(size_t k, size_t m, trs_rs8 **out) {
   rs->m = m;
   rs->n = k + m;
   if (!build_generator<F8>(k, k + m, rs->G)) return TRS_SINGULAR_MATRIX;
-  const int st = upload_coef8(rs->parity, rs->G.row(k), m, k);
+  int st = upload_coef8(rs->parity, rs->G.row(k), m, k);
   if (st) return st;
+  if ((st = upload_frags(rs->parity_frag, rs->G.row(k), m, k))) return st;
   if (bs_encode_available(kRouteBsFft, uint32_t(k), uint32_t(m)))
     rs->route = kRouteBsFft;
   else if (bs_encode_available(kRouteBsMatrix, uint32_t(k), uint32_t(m)))
@@ -667,12 +702,14 @@ size_t trs_rs8_parity_shard_count(const trs_rs8 *rs) { return rs ? rs->m : 0; }
 size_t trs_rs8_total_shard_count(const trs_rs8 *rs) { return rs ? rs->n : 0; }
 
 const char *trs_rs8_encode_route(const trs_rs8 *rs, size_t shard_len) {
-  (void)shard_len;
   if (!rs) return "none";
   switch (rs->route) {
     case kRouteBsFft: return "gpu-bitsliced-fft";
     case kRouteBsMatrix: return "gpu-bitsliced-matrix";
-    default: return "gpu-fused-matrix";
+    default:
+      return use_mfma(rs->apply_route.load(), rs->k, shard_len, false) && rs->parity_frag.p
+                 ? "gpu-int8-mfma-matrix"
+                 : "gpu-fused-matrix";
   }
 }
 
@@ -698,8 +735,11 @@ int encode8_ptrs(const trs_rs8 *rs, std::vector<const uint8_t *> &ptrs, size_t l
                  hipStream_t s, int route) {
   for (auto q : ptrs)
     if (!q) return TRS_ERR_INVALID_ARGUMENT;
-  if (route == kRouteTable || len == 0)
+  if (route == kRouteTable || len == 0) {
+    if (use_mfma(rs->apply_route.load(), rs->k, len, false) && rs->parity_frag.p)
+      return run_mfma(ptrs, rs->k, rs->m, rs->parity_frag.p, len, s);
     return run_gf8(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, s);
+  }
   if (!bs_encode_available(route, uint32_t(rs->k), uint32_t(rs->m))) return TRS_ERR_UNSUPPORTED;
   Gf8Call c{};
   bool al = true;
@@ -798,33 +838,6 @@ int trs_rs8_verify(const trs_rs8 *rs, const uint8_t *const *shards, const size_t
 namespace {
 // reconstruct_internal / PreparedDecoder::reconstruct_internal
 // (src/reedsolomon.rs:615-641, :771-796).  `plan` non-null = prepared.
-// Route of a reconstruct apply.  Measured on MI355X the int8-MFMA kernel
-// does not yet beat the table kernel at any C2 shape (it is VALU-bound on
-// bit expansion and packing), so "auto" keeps the table kernel and MFMA runs
-// only when asked for.
-bool use_mfma(int route, size_t k, size_t nout) {
-  (void)nout;
-  return route == 2 && mfma_available(uint32_t(k));
-}
-
-int run_mfma(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout, const void *frag,
-             size_t len, hipStream_t stream) {
-  if (nout == 0 || len == 0) return TRS_OK;
-  if (k + nout > size_t(kInlinePtrs)) return TRS_ERR_INVALID_ARGUMENT;
-  Gf8MfmaCall c{};
-  bool al = true;
-  for (size_t i = 0; i < k + nout; ++i) {
-    c.ptr[i] = ptrs[i];
-    al = al && (reinterpret_cast<uintptr_t>(ptrs[i]) & 3) == 0;
-  }
-  c.frag = frag;
-  c.len = len;
-  c.k = uint32_t(k);
-  c.nout = uint32_t(nout);
-  c.vec_ok = al ? 1u : 0u;
-  return hip_status(launch_gf8_mfma(c, stream));
-}
-
 int reconstruct8(const trs_rs8 *rs, size_t k, size_t total, std::shared_ptr<Plan> plan,
                  uint8_t *const *shards, const size_t *lens, const uint8_t *present, size_t n,
                  int data_only, hipStream_t s, int apply_route) {
@@ -846,7 +859,7 @@ int reconstruct8(const trs_rs8 *rs, size_t k, size_t total, std::shared_ptr<Plan
   std::vector<const uint8_t *> ptrs;
   size_t nout = 0;
   if ((st = gather(*plan, shards, lens, k, len, data_only != 0, ptrs, nout))) return st;
-  if (use_mfma(apply_route, k, nout) && plan->frag.p)
+  if (use_mfma(apply_route, k, len, false) && plan->frag.p)
     return run_mfma(ptrs, k, nout, plan->frag.p, len, s);
   return run_gf8(ptrs, k, nout, plan->coef, len, kWrite, nullptr, s);
 }
@@ -1495,7 +1508,7 @@ int trs_batch8_add_reconstruct(trs_batch8 *b, const trs_rs8 *rs, uint8_t *const 
   r.len = len;
   r.k = uint32_t(rs->k);
   r.nout = uint32_t(nout);
-  if (use_mfma(rs->apply_route.load(), rs->k, nout) && plan->frag.p) {
+  if (use_mfma(rs->apply_route.load(), rs->k, len, true) && plan->frag.p) {
     bool al = true;
     for (auto q : r.ptrs) al = al && (reinterpret_cast<uintptr_t>(q) & 3) == 0;
     if (al) r.frag = plan->frag.p;
@@ -1565,8 +1578,14 @@ int trs_batch8_prepare(trs_batch8 *b) {
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   auto upload = [&](trs_batch8::List &L, std::vector<BatchJob> &jobs, std::vector<uint64_t> &tiles,
                     uint64_t total, uint64_t per_cu) -> int {
+    if (total == 0) {  // nothing for this kernel: no copies (each is a host round trip)
+      L.njobs = 0;
+      L.total_tiles = 0;
+      L.grid = 0;
+      return TRS_OK;
+    }
     // tile -> job map, one uint32 per tile
-    std::vector<uint32_t> tj(total ? total : 1, 0);
+    std::vector<uint32_t> tj(total, 0);
     for (size_t j = 0; j < jobs.size(); ++j) {
       const uint64_t end = j + 1 < jobs.size() ? tiles[j + 1] : total;
       for (uint64_t t = tiles[j]; t < end; ++t) tj[t] = uint32_t(j);

@@ -53,23 +53,23 @@ __device__ __forceinline__ uint32_t pack8(const v16i &a, int base) {
 // unaligned shard assemble the valid bytes.
 __device__ __forceinline__ uint32_t load4(const uint8_t *p, uint64_t col, uint64_t len,
                                           bool aligned) {
-  if (aligned && col + 4 <= len) return *reinterpret_cast<const uint32_t *>(p + col);
+  if (aligned && col + 4 <= len) return ld4(p + col);
   uint32_t v = 0;
 #pragma unroll
   for (int b = 0; b < 4; ++b)
-    if (col + b < len) v |= uint32_t(p[col + b]) << (8 * b);
+    if (col + b < len) v |= uint32_t(ld1(p + col + b)) << (8 * b);
   return v;
 }
 
 __device__ __forceinline__ void store4(uint8_t *p, uint64_t col, uint64_t len, bool aligned,
                                        uint32_t v) {
   if (aligned && col + 4 <= len) {
-    *reinterpret_cast<uint32_t *>(p + col) = v;
+    st4(p + col, v);
     return;
   }
 #pragma unroll
   for (int b = 0; b < 4; ++b)
-    if (col + b < len) p[col + b] = uint8_t(v >> (8 * b));
+    if (col + b < len) st1(p + col + b, uint8_t(v >> (8 * b)));
 }
 
 // Input dwords of one 128-column block: lane l reads columns 4*(l%32)..+3 of
@@ -97,8 +97,8 @@ __device__ __forceinline__ void mfma_load(PtrAt ptr_at, uint64_t len, uint32_t k
     const bool vb = (4 * s + (upper ? 3u : 1u)) < k;
     uint32_t x0, x1;
     if (FULL) {
-      x0 = *reinterpret_cast<const uint32_t *>(pa + col);
-      x1 = *reinterpret_cast<const uint32_t *>(pb + col);
+      x0 = ld4(pa + col);
+      x1 = ld4(pb + col);
     } else {
       x0 = col < len ? load4(pa, col, len, aligned) : 0u;
       x1 = col < len ? load4(pb, col, len, aligned) : 0u;
@@ -155,7 +155,7 @@ __device__ __forceinline__ void mfma_compute(PtrAt ptr_at, const v4i (&A0)[KS],
         const uint8_t *hi_ptr = o_hi < nout ? ptr_at(k + o_hi) : nullptr;
         uint8_t *dst = const_cast<uint8_t *>(upper ? hi_ptr : lo_ptr);
         if (full)
-          *reinterpret_cast<uint32_t *>(dst + col) = word;
+          st4(dst + col, word);
         else
           store4(dst, col, len, aligned, word);
       }
@@ -221,20 +221,20 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch_mfma(const BatchJob *jo
                                                              const uint32_t *tile_job,
                                                              uint64_t total) {
   for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
-    // job fields are the same in every lane: keep them in scalar registers
-    const BatchJob &J = jobs[uni(tile_job[t])];
-    const uint32_t nout = uni(J.nout), k = uni(J.k);
-    const uint64_t len = uni(J.len);
+    // job fields are the same in every lane: scalar loads
+    const auto &J = ld_const_ref(jobs + ld_const(tile_job + t));
+    const uint32_t nout = J.nout, k = J.k;
+    const uint64_t len = J.len;
     const uint32_t passes = (nout + 7) / 8;
-    const uint64_t local = t - uni(J.tile_begin);
+    const uint64_t local = t - J.tile_begin;
     const uint64_t group = local / passes;
     const uint32_t pass = uint32_t(local - group * passes);
     const uint64_t wave_col = group * kGroupCols + uni(uint32_t(threadIdx.x >> 6)) * kWaveCols;
     if (wave_col >= len) continue;
-    const uint8_t *const *ptrs = uni(J.ptrs);
-    const void *frag = uni(J.coef);
-    auto at = [&](uint32_t i) { return uni(ptrs[i]); };
-    const bool al = uni(J.vec_ok) != 0;
+    const uint8_t *const *ptrs = J.ptrs;
+    const void *frag = J.coef;
+    auto at = [&](uint32_t i) { return ld_const_ptr(ptrs + i); };
+    const bool al = J.vec_ok != 0;
     switch ((k + 3) / 4) {
       case 1: mfma_wave<1>(at, frag, len, k, nout, pass, wave_col, al); break;
       case 2: mfma_wave<2>(at, frag, len, k, nout, pass, wave_col, al); break;

@@ -63,6 +63,15 @@ __device__ __forceinline__ void st16(uint8_t *p, uint4 x) {
     *g = v;
 }
 
+// 4-byte shard words through global pointers.
+__device__ __forceinline__ uint32_t ld4(const uint8_t *p) {
+  return *(const __attribute__((address_space(1))) uint32_t *)(p);
+}
+
+__device__ __forceinline__ void st4(uint8_t *p, uint32_t v) {
+  *(__attribute__((address_space(1))) uint32_t *)(p) = v;
+}
+
 // One byte of a shard through a global pointer (tail and unaligned paths).
 __device__ __forceinline__ uint8_t ld1(const uint8_t *p) {
   return *(const __attribute__((address_space(1))) uint8_t *)(p);

@@ -339,6 +339,27 @@ def test_reconstruct_apply_routes(trs, k, m, route):
             assert (got[i] == want).all(), (k, m, length, i, route)
 
 
+@pytest.mark.parametrize("k,m", [(5, 3), (29, 9)])
+def test_auto_apply_route_by_length(trs, k, m):
+    """Automatic routing of single-call matrix applies (shapes without a
+    generated program, k <= 32): int8 MFMA below 512 KiB shards, the table
+    kernel above; encode and reconstruct match the oracle on both sides."""
+    G = O.build_matrix(8, k, k + m)
+    rs = trs.ReedSolomon(k, m)
+    assert rs.encode_route(65536) == "gpu-int8-mfma-matrix"
+    assert rs.encode_route(1 << 20) == "gpu-fused-matrix"
+    rng = np.random.default_rng(k)
+    for length in (65536 + 3, (512 << 10) - 16, 512 << 10, (1 << 20) + 5):
+        data = seeded(k, length, 7 + length)
+        original = np.concatenate([data, O.rs8_encode(G, k, data)])
+        cur = rows_to_dev(np.concatenate([data, np.zeros((m, length), np.uint8)]))
+        rs.encode(list(cur))
+        assert (to_host(cur) == original).all(), (k, m, length)
+        erased = set(rng.choice(k + m, m, replace=False).tolist())
+        rs.reconstruct(_erase_and_view(cur, erased))
+        assert (to_host(cur) == original).all(), (k, m, length, sorted(erased))
+
+
 def test_unaligned_rows_on_mfma_route(trs):
     # rows at odd offsets exercise the byte path of the MFMA kernel
     k, m, row_len = 10, 6, 1001
