@@ -74,12 +74,14 @@ __device__ __forceinline__ void store_lane(uint8_t *p, bool full, uint64_t rem,
   }
 }
 
+#if TRS_MASKED_MAC
 // acc ^ (p & m) in one v_bitop3 (truth table over (S0, S1, S2) = (acc, p, m)
 // indexed S0 S1 S2 from the high bit: 0x78); the compiler does not fuse it
 // when m is a scalar register.
 __device__ __forceinline__ uint32_t xor_and(uint32_t acc, uint32_t p, uint32_t m) {
   return __builtin_amdgcn_bitop3_b32(acc, p, m, 0x78);
 }
+#endif
 
 // acc[j] ^= c_j * x for the NT coefficients at cw (transposed, padded rows:
 // byte j of the NT/4 words at cw is output j's coefficient).
