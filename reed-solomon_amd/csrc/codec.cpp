@@ -1551,7 +1551,8 @@ int trs_batch8_prepare(trs_batch8 *b) {
     J.nout = r.nout;
     J.ostride = r.coef->ostride;
     J.vec_ok = al ? 1u : 0u;
-    J.out_tiles = (r.nout + kOutStride - 1) / kOutStride;
+    const uint32_t rows_per_tile = b->planes && !r.frag && !r.prog ? kPlanesOut : kOutStride;
+    J.out_tiles = (r.nout + rows_per_tile - 1) / rows_per_tile;
     J.prog = r.prog;
     if (r.frag) {
       J.coef = static_cast<const uint32_t *>(r.frag);

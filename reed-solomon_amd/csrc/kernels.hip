@@ -331,7 +331,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch8(const BatchJob *jobs,
 }
 
 // The same work list over the bit-sliced tile: tiles are (job, 8 KiB column
-// chunk, 16-row output tile).
+// chunk, kPlanesOut-row output tile).  Eight rows (64 accumulator VGPRs, five
+// waves per SIMD) measured faster than sixteen (three waves) although jobs
+// with more than eight outputs then read their inputs once per tile.
 __global__ __launch_bounds__(kBlockThreads) void k_batch8_planes(const BatchJob *jobs,
                                                                 const uint32_t *tile_job,
                                                                 uint64_t total) {
@@ -346,9 +348,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch8_planes(const BatchJob 
     const uint64_t off = tile + split_lane_offset(threadIdx.x);
     if (off < len) {
       const uint8_t *const *ptrs = J.ptrs;
-      gf8_tile_planes<kOutStride, kWrite>([&](uint32_t idx) { return ld_const_ptr(ptrs + idx); },
+      gf8_tile_planes<kPlanesOut, kWrite>([&](uint32_t idx) { return ld_const_ptr(ptrs + idx); },
                                           J.coef, len, J.k, J.nout, J.ostride, J.vec_ok != 0,
-                                          ot * kOutStride, tile, off, nullptr);
+                                          ot * kPlanesOut, tile, off, nullptr);
     }
   }
 }
