@@ -761,8 +761,10 @@ hipError_t raise_lds_limit(size_t lds) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess || raised == dev) return e;
-  const void *kernels[4] = {reinterpret_cast<const void *>(&k_fft16<512>),
+  const void *kernels[6] = {reinterpret_cast<const void *>(&k_fft16<1024>),
+                            reinterpret_cast<const void *>(&k_fft16<512>),
                             reinterpret_cast<const void *>(&k_fft16<256>),
+                            reinterpret_cast<const void *>(&k_fft16_jobs<1024>),
                             reinterpret_cast<const void *>(&k_fft16_jobs<512>),
                             reinterpret_cast<const void *>(&k_fft16_jobs<256>)};
   for (const void *k : kernels)
@@ -771,6 +773,14 @@ hipError_t raise_lds_limit(size_t lds) {
       return e;
   raised = dev;
   return hipSuccess;
+}
+// Workgroup size: 1024 threads when the register file takes more than half
+// the LDS (one workgroup per CU, e.g. the 128 KiB decoders) and the widest
+// step has that many items -- the CU then runs 16 waves instead of 8 -- else
+// 512 or 256 by the widest step.
+uint32_t choose_threads(size_t lds, uint64_t widest) {
+  if (lds > kLdsMax / 2 && widest >= 1024) return 1024;
+  return widest >= 512 ? 512 : 256;
 }
 }  // namespace
 
@@ -801,10 +811,12 @@ hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t
   a.total = uint32_t(total);
   a.vec_ok = (call.vec_ok && call.plane % 16 == 0) ? 1u : 0u;  // pointers carry any column offset
   const size_t lds = (d.regs * kRegBytes) << gs;
-  const bool wide = (d.max_count << gs) >= 512;
+  const uint32_t threads = choose_threads(lds, uint64_t(d.max_count) << gs);
   hipError_t e = raise_lds_limit(lds);
   if (e != hipSuccess) return e;
-  if (wide)
+  if (threads == 1024)
+    hipLaunchKernelGGL(k_fft16<1024>, dim3(uint32_t(total)), dim3(1024), lds, stream, a);
+  else if (threads == 512)
     hipLaunchKernelGGL(k_fft16<512>, dim3(uint32_t(total)), dim3(512), lds, stream, a);
   else
     hipLaunchKernelGGL(k_fft16<256>, dim3(uint32_t(total)), dim3(256), lds, stream, a);
@@ -815,7 +827,7 @@ hipError_t launch_fft16_jobs(const std::vector<Fft16JobSpec> &jobs, hipStream_t 
   std::vector<Fft16Args> table;
   std::vector<uint32_t> wg_job;
   size_t lds = 0;
-  bool wide = false;
+  uint64_t widest = 0;
   for (const Fft16JobSpec &j : jobs) {
     const Fft16Device &d = *j.prog;
     if (j.plane == 0 || j.nstore == 0) continue;
@@ -844,7 +856,7 @@ hipError_t launch_fft16_jobs(const std::vector<Fft16JobSpec> &jobs, hipStream_t 
     wg_job.insert(wg_job.end(), cblocks, uint32_t(table.size()));
     table.push_back(a);
     lds = std::max(lds, (d.regs * kRegBytes) << gs);
-    wide = wide || (d.max_count << gs) >= 512;
+    widest = std::max<uint64_t>(widest, uint64_t(d.max_count) << gs);
   }
   if (table.empty()) return hipSuccess;
   hipError_t e = raise_lds_limit(lds);
@@ -861,7 +873,10 @@ hipError_t launch_fft16_jobs(const std::vector<Fft16JobSpec> &jobs, hipStream_t 
     const auto *jt = static_cast<const Fft16Args *>(mem);
     const auto *wj = reinterpret_cast<const uint32_t *>(static_cast<uint8_t *>(mem) + off);
     const uint32_t total = uint32_t(wg_job.size());
-    if (wide)
+    const uint32_t threads = choose_threads(lds, widest);
+    if (threads == 1024)
+      hipLaunchKernelGGL(k_fft16_jobs<1024>, dim3(total), dim3(1024), lds, stream, jt, wj, total);
+    else if (threads == 512)
       hipLaunchKernelGGL(k_fft16_jobs<512>, dim3(total), dim3(512), lds, stream, jt, wj, total);
     else
       hipLaunchKernelGGL(k_fft16_jobs<256>, dim3(total), dim3(256), lds, stream, jt, wj, total);
