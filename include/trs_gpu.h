@@ -119,8 +119,9 @@ int trs_rs8_reconstruct_rows(const trs_rs8 *rs, uint8_t *rows, size_t rows_len, 
  * call): 0 = automatic, 1 = fused table kernel, 2 = int8 MFMA on the
  * bit-expanded rows (k <= 32; TRS_ERR_UNSUPPORTED otherwise).  Automatic
  * takes MFMA for single calls with k <= 32 and shards under 512 KiB (about
- * twice as fast there, measured) and the table kernel otherwise; batch work
- * lists use the bit-sliced planes kernel.  Every route is byte-identical; no
+ * twice as fast there, measured), otherwise the bit-sliced planes kernel from
+ * 128 KiB and the table kernel below; batch work lists use the planes
+ * kernel.  Every route is byte-identical; no
  * reference counterpart (the reference has one CPU kernel per ISA). */
 int trs_rs8_set_apply_route(trs_rs8 *rs, int route);
 /* ReedSolomon::prepare_decode, src/reedsolomon.rs:484-507. */

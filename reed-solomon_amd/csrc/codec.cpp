@@ -524,6 +524,21 @@ int gather(const Plan &p, uint8_t *const *shards, const size_t *lens, size_t k, 
   return TRS_OK;
 }
 
+// Single-call runtime-coefficient applies from this shard length up run on
+// the bit-sliced planes tile (8 KiB chunks, five waves per SIMD); shorter ones
+// on the 4 KiB table tile, whose smaller tiles give a short launch more
+// workgroups.  Measured per call at 128 KiB - 4 MiB on (10,4), (20,10),
+// (48,16), (100,30): planes 1.0-1.5x the table tile at 128-512 KiB, 1.3-1.4x at
+// 4 MiB.  TRS_APPLY_PLANES_MIN (bytes) overrides the crossover.
+size_t planes_min_len() {
+  static const size_t v = [] {
+    size_t d = size_t(128) << 10;
+    if (const char *e = std::getenv("TRS_APPLY_PLANES_MIN")) d = size_t(std::strtoull(e, nullptr, 10));
+    return d;
+  }();
+  return v;
+}
+
 int run_gf8(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout, const DevCoef &coef,
             size_t len, uint32_t mode, uint32_t *flag, hipStream_t stream) {
   if (nout == 0 || len == 0) return TRS_OK;
@@ -541,6 +556,7 @@ int run_gf8(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout, con
   c.ostride = coef.ostride;
   c.vec_ok = al ? 1u : 0u;
   c.flag = flag;
+  if (len >= planes_min_len()) return hip_status(launch_gf8_planes(c, mode, stream));
   return hip_status(launch_gf8_apply(c, mode, stream));
 }
 
@@ -707,9 +723,9 @@ const char *trs_rs8_encode_route(const trs_rs8 *rs, size_t shard_len) {
     case kRouteBsFft: return "gpu-bitsliced-fft";
     case kRouteBsMatrix: return "gpu-bitsliced-matrix";
     default:
-      return use_mfma(rs->apply_route.load(), rs->k, shard_len, false) && rs->parity_frag.p
-                 ? "gpu-int8-mfma-matrix"
-                 : "gpu-fused-matrix";
+      if (use_mfma(rs->apply_route.load(), rs->k, shard_len, false) && rs->parity_frag.p)
+        return "gpu-int8-mfma-matrix";
+      return shard_len >= planes_min_len() ? "gpu-bitsliced-planes-matrix" : "gpu-fused-matrix";
   }
 }
 

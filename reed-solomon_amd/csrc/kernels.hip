@@ -305,6 +305,18 @@ __global__ __launch_bounds__(kBlockThreads) void k_gf8_apply(const Gf8Call a) {
                      a.ostride, a.vec_ok != 0, blockIdx.y * NT, tile, off, a.flag);
 }
 
+// The single-call apply on the bit-sliced tile: 8 KiB column chunks in x,
+// NT-row output tiles in y.
+template <int NT, uint32_t MODE>
+__global__ __launch_bounds__(kBlockThreads) void k_gf8_planes(const Gf8Call a) {
+  const uint64_t tile = uint64_t(blockIdx.x) * kPlanesChunkBytes;
+  const uint64_t off = tile + split_lane_offset(threadIdx.x);
+  if (off >= a.len) return;
+  gf8_tile_planes<NT, MODE>([&](uint32_t idx) { return uni(a.ptr[idx]); }, a.coef, a.len, a.k,
+                            a.nout, a.ostride, a.vec_ok != 0, blockIdx.y * NT, tile, off,
+                            a.flag);
+}
+
 // Batched work list: tiles are (job, 4 KiB column chunk, 16-row output tile),
 // numbered job by job; tile_job[t] names the job of tile t.  A fixed grid
 // strides over the tiles, which interleaves heavy and light jobs across
@@ -490,6 +502,27 @@ hipError_t launch_gf16_apply(const Gf16Call &call, hipStream_t stream) {
     else TRS_GF16(8, kWrite);
   }
 #undef TRS_GF16
+  return hipGetLastError();
+}
+
+hipError_t launch_gf8_planes(const Gf8Call &call, uint32_t mode, hipStream_t stream) {
+  if (call.len == 0 || call.nout == 0) return hipSuccess;
+  const uint32_t gx = uint32_t((call.len + kPlanesChunkBytes - 1) / kPlanesChunkBytes);
+  const int nt = call.nout <= 4 ? 4 : kPlanesOut;
+  const dim3 grid(gx, (call.nout + nt - 1) / nt);
+  const dim3 block(kBlockThreads);
+#define TRS_GF8P(NT, MODE) hipLaunchKernelGGL((k_gf8_planes<NT, MODE>), grid, block, 0, stream, call)
+  if (mode == kWrite) {
+    if (nt == 4) TRS_GF8P(4, kWrite);
+    else TRS_GF8P(kPlanesOut, kWrite);
+  } else if (mode == kVerify) {
+    if (nt == 4) TRS_GF8P(4, kVerify);
+    else TRS_GF8P(kPlanesOut, kVerify);
+  } else {
+    if (nt == 4) TRS_GF8P(4, kXorInto);
+    else TRS_GF8P(kPlanesOut, kXorInto);
+  }
+#undef TRS_GF8P
   return hipGetLastError();
 }
 

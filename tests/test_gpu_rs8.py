@@ -191,7 +191,8 @@ def test_reconstruct_rows_roundtrip(trs):
 def test_verify_blocks(trs):
     # src/reedsolomon.rs:1188-1207
     k, m = 4, 3
-    for length in (100, 32 * 1024, 32 * 1024 + 17):
+    # the last two lengths take the bit-sliced planes kernel (from 128 KiB)
+    for length in (100, 32 * 1024, 32 * 1024 + 17, 256 * 1024, 256 * 1024 + 17):
         rs, shards, data = encoded(trs, k, m, length)
         assert rs.verify(shards)
         host = to_host(shards)
@@ -323,7 +324,7 @@ def test_reconstruct_apply_routes(trs, k, m, route):
     G = O.build_matrix(8, k, k + m)
     rs = trs.ReedSolomon(k, m)
     rs.set_apply_route(route)
-    for length in (1, 3, 4, 5, 127, 128, 129, 2048, 2049, 70003):
+    for length in (1, 3, 4, 5, 127, 128, 129, 2048, 2049, 70003, (128 << 10) + 5):
         data = seeded(k, length, 99 + length)
         original = np.concatenate([data, O.rs8_encode(G, k, data)])
         e = int(rng.integers(1, m + 1))
@@ -342,12 +343,16 @@ def test_reconstruct_apply_routes(trs, k, m, route):
 @pytest.mark.parametrize("k,m", [(5, 3), (29, 9)])
 def test_auto_apply_route_by_length(trs, k, m):
     """Automatic routing of single-call matrix applies (shapes without a
-    generated program, k <= 32): int8 MFMA below 512 KiB shards, the table
-    kernel above; encode and reconstruct match the oracle on both sides."""
+    generated program, k <= 32): int8 MFMA below 512 KiB shards, the
+    bit-sliced planes kernel above; encode and reconstruct match the oracle on
+    both sides."""
     G = O.build_matrix(8, k, k + m)
     rs = trs.ReedSolomon(k, m)
     assert rs.encode_route(65536) == "gpu-int8-mfma-matrix"
-    assert rs.encode_route(1 << 20) == "gpu-fused-matrix"
+    assert rs.encode_route(1 << 20) == "gpu-bitsliced-planes-matrix"
+    rs.set_apply_route("table")
+    assert rs.encode_route(65536) == "gpu-fused-matrix"
+    rs.set_apply_route("auto")
     rng = np.random.default_rng(k)
     for length in (65536 + 3, (512 << 10) - 16, 512 << 10, (1 << 20) + 5):
         data = seeded(k, length, 7 + length)
