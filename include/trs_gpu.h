@@ -321,7 +321,9 @@ int trs_server_take(trs_server *s, uint64_t ticket, double *latency_us, uint64_t
 /* Times one apply candidate on synthetic device data: `nout` outputs from `k`
  * inputs of `len` bytes over GF(2^8) (field 8) or the tower (field 16,
  * split-plane, even len), random nonzero coefficients, through `route`
- * (0 = matrix table kernel; 1 = bit-sliced program: the generated GF(2^8)
+ * (0 = matrix route with runtime coefficients, dispatched by strip length as
+ * the codec's single calls are: the table kernel below 128 KiB, the
+ * bit-sliced planes kernel above; 1 = bit-sliced program: the generated GF(2^8)
  * program of the shape where it has one, or for the tower the staged LCH FFT
  * encode program of shape (k, nout) on the LDS interpreter, 2 <= k <= 1024,
  * k + nout <= 4096; 2 = int8 MFMA, GF(2^8), k <= 32) in column strips of

@@ -1,7 +1,8 @@
 """Autotuner and learned cost model for route / strip selection.
 
 Candidates for one apply shape (field, k inputs, m outputs, shard length L)
-are a device route -- "table" (the matrix route: fused table kernel),
+are a device route -- "table" (the matrix route with runtime coefficients:
+the table kernel below 128 KiB strips, the bit-sliced planes kernel above),
 "bitsliced" (the FFT route: the generated bit-sliced LCH program of a GF(2^8)
 shape that has one, or for the tower the staged LCH FFT program on the LDS
 interpreter) or "mfma" (bit-sliced int8 MFMA, GF(2^8), k <= 32) -- times a

@@ -1844,7 +1844,10 @@ extern "C" int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len
         c.nout = nout;
         c.ostride = T.coef.ostride;
         c.vec_ok = (len % 16 == 0 && off % 16 == 0) ? 1u : 0u;
-        e = route == 1 ? launch_bs_encode(bs_route, c, T.stream) : launch_gf8_apply(c, kWrite, T.stream);
+        // the matrix candidate dispatches by length as the codec's calls do (run_gf8)
+        e = route == 1                     ? launch_bs_encode(bs_route, c, T.stream)
+            : span >= planes_min_len() ? launch_gf8_planes(c, kWrite, T.stream)
+                                       : launch_gf8_apply(c, kWrite, T.stream);
       }
       if (e != hipSuccess) return hip_status(e);
     }
