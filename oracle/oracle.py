@@ -91,6 +91,19 @@ def lib():
         L.orc_rs16_reconstruct.argtypes = [_u16p, ctypes.c_size_t, ctypes.c_size_t, pp, _u8p,
                                            ctypes.c_size_t, ctypes.c_int, ctypes.c_size_t]
         L.orc_fill.argtypes = [ctypes.c_uint64, _u8p, ctypes.c_size_t]
+        L.orc_simd_available.restype = ctypes.c_int
+        L.orc_simd_apply.argtypes = [_u8p, ctypes.c_size_t, ctypes.c_size_t, pp, pp,
+                                     ctypes.c_size_t]
+        _i32p = ctypes.POINTER(ctypes.c_int32)
+        L.orc_simd_fft_encode.argtypes = [_i32p, ctypes.c_size_t, ctypes.c_size_t,
+                                          ctypes.c_size_t, ctypes.c_size_t, _i32p, pp, pp,
+                                          ctypes.c_size_t]
+        L.orc_simd_fft_generated.restype = ctypes.c_int
+        L.orc_simd_fft_generated.argtypes = [ctypes.c_size_t, ctypes.c_size_t, pp, pp,
+                                             ctypes.c_size_t]
+        L.orc_simd_rs8_reconstruct.restype = ctypes.c_int
+        L.orc_simd_rs8_reconstruct.argtypes = [_u8p, ctypes.c_size_t, ctypes.c_size_t, pp, _u8p,
+                                               ctypes.c_size_t, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -279,3 +292,128 @@ def fill(seed: int, length: int) -> np.ndarray:
     out = np.zeros(length, np.uint8)
     lib().orc_fill(seed & 0xFFFFFFFFFFFFFFFF, _p8(out), length)
     return out
+
+
+# -- x86 GFNI / AVX-512 restatement (simd_ref.c): the CPU baseline -----------
+# Pinned byte for byte against the scalar functions above (tests/test_simd_ref.py).
+
+_FFT_KINDS = {"xo": 0, "cp": 1, "mx": 2, "mc": 3, "zero": 4}
+
+
+def simd_available() -> bool:
+    """True when this CPU has GFNI, AVX-512F/BW/VL (the simd_* functions need them)."""
+    return bool(lib().orc_simd_available())
+
+
+def simd_rs8_apply(rows: np.ndarray, inputs: Sequence[np.ndarray], length: int) -> np.ndarray:
+    """RowTables::apply on the fused GFNI kernel (src/gf/x86_fused.rs:13-130)."""
+    rows = np.ascontiguousarray(rows, dtype=np.uint8)
+    nout, k = rows.shape
+    out = np.zeros((nout, length), np.uint8)
+    lib().orc_simd_apply(_p8(rows), nout, k, _ptr_array(list(inputs)),
+                         _ptr_array([out[j] for j in range(nout)]), length)
+    return out
+
+
+def simd_fft_program(ops, parity_regs, register_count: int):
+    """Packs an LCH op list ((kind, dst, src[, c]) tuples, fftgen's format) for
+    simd_fft_encode."""
+    flat = []
+    for op in ops:
+        kind = _FFT_KINDS[op[0]]
+        flat += [kind, op[1], op[2] if len(op) > 2 else 0, op[3] if len(op) > 3 else 0]
+    return (np.array(flat, np.int32), np.array(list(parity_regs), np.int32), int(register_count))
+
+
+def simd_fft_encode(prog, k: int, data: np.ndarray) -> np.ndarray:
+    """Parity of `data` (k, L) through a packed LCH program (src/gf/fft_x86.rs:22-165)."""
+    ops, preg, nreg = prog
+    data = np.ascontiguousarray(data, dtype=np.uint8)
+    m, length = len(preg), data.shape[1]
+    out = np.zeros((m, length), np.uint8)
+    i32 = ctypes.POINTER(ctypes.c_int32)
+    lib().orc_simd_fft_encode(ops.ctypes.data_as(i32), len(ops) // 4, nreg, k, m,
+                              preg.ctypes.data_as(i32), _ptr_array([data[i] for i in range(k)]),
+                              _ptr_array([out[j] for j in range(m)]), length)
+    return out
+
+
+def simd_rs8_reconstruct(G: np.ndarray, k: int, m: int, shards: np.ndarray,
+                         present: Sequence[bool], data_only: bool = False) -> int:
+    """In-place reconstruct on the fused GFNI kernel (src/reedsolomon.rs:829-891)."""
+    G = np.ascontiguousarray(G, dtype=np.uint8)
+    pres = np.array([1 if p else 0 for p in present], np.uint8)
+    rows = [shards[i] for i in range(k + m)]
+    return lib().orc_simd_rs8_reconstruct(_p8(G), k, m, _ptr_array(rows), _p8(pres),
+                                          shards.shape[1], int(data_only))
+
+
+def simd_fft_generated(k: int, m: int, data: np.ndarray, out: np.ndarray = None):
+    """Parity through the straight-line encoder of a generated shape
+    (simd_fft_gen.c) into `out` (allocated when None); None when (k, m) has no
+    generated program."""
+    data = np.ascontiguousarray(data, dtype=np.uint8)
+    length = data.shape[1]
+    if out is None:
+        out = np.zeros((m, length), np.uint8)
+    rc = lib().orc_simd_fft_generated(k, m, _ptr_array([data[i] for i in range(k)]),
+                                      _ptr_array([out[j] for j in range(m)]), length)
+    return None if rc else out
+
+
+# Prepared calls for timing: pointer arrays are built once, so a timed call is
+# one C call (building ctypes arrays costs microseconds per row).
+
+def simd_encode_call(G: np.ndarray, k: int, m: int, data: np.ndarray, parity: np.ndarray):
+    """A zero-argument callable writing the parity of `data` into `parity`:
+    the straight-line program for a generated shape (the reference's
+    fft-generated route), the fused GFNI apply of G's parity rows otherwise
+    (src/reedsolomon.rs:204-234 routes the same way)."""
+    length = data.shape[1]
+    ins = _ptr_array([data[i] for i in range(k)])
+    outs = _ptr_array([parity[j] for j in range(m)])
+    L = lib()
+    # the callables hold the arrays their pointer tables point into
+    if L.orc_simd_fft_generated(k, m, None, None, 0) == 0:  # the shape has a program
+        return lambda keep=(data, parity): L.orc_simd_fft_generated(k, m, ins, outs, length)
+    rows = np.ascontiguousarray(G[k:], dtype=np.uint8)
+    rp = _p8(rows)
+    return lambda keep=(rows, data, parity): L.orc_simd_apply(rp, m, k, ins, outs, length)
+
+
+def rs8_decode_plan(G: np.ndarray, k: int, m: int, present: Sequence[bool]):
+    """DecodePlan (src/reedsolomon.rs:537-613): the first k present shards,
+    rows rebuilding each missing data shard from them, and the missing parity
+    indices (rebuilt from the data with G's rows)."""
+    valid = [i for i in range(k + m) if present[i]][:k]
+    sub = np.array([G[i] for i in valid], np.uint16)
+    rc, inv = invert(8, sub)
+    assert rc == 0
+    mdata = [i for i in range(k) if not present[i]]
+    mpar = [i for i in range(k, k + m) if not present[i]]
+    drows = np.array([inv[i] for i in mdata], np.uint8).reshape(len(mdata), k)
+    return valid, mdata, drows, mpar
+
+
+def simd_reconstruct_calls(G: np.ndarray, k: int, m: int, shards: np.ndarray, plan):
+    """Callables for the two fused applies of reconstruct_with_plan
+    (src/reedsolomon.rs:829-891): missing data from the plan's survivors, then
+    missing parity from the data."""
+    valid, mdata, drows, mpar = plan
+    length = shards.shape[1]
+    L = lib()
+    calls = []
+    if mdata:
+        src = _ptr_array([shards[i] for i in valid])
+        dst = _ptr_array([shards[i] for i in mdata])
+        dp = _p8(drows)
+        calls.append(lambda keep=(drows, shards): L.orc_simd_apply(dp, len(mdata), k, src, dst,
+                                                                  length))
+    if mpar:
+        prows = np.ascontiguousarray(G[mpar], dtype=np.uint8)
+        src2 = _ptr_array([shards[i] for i in range(k)])
+        dst2 = _ptr_array([shards[i] for i in mpar])
+        pp2 = _p8(prows)
+        calls.append(lambda keep=(prows, shards): L.orc_simd_apply(pp2, len(mpar), k, src2, dst2,
+                                                                  length))
+    return calls
