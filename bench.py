@@ -145,6 +145,51 @@ C1_SEED = 0x7A9E_5EED_0000_0001
 TOWER_SEED = 0x7A9E_5EED_0000_0003
 
 
+def host_leg(trs, torch, O, L: int = 32 << 20, reps: int = 3):
+    """Host-resident shards (SURVEY (f) 2): RS8 (10,4) with 32 MiB shards in
+    host memory, encode and a 4-erasure reconstruct through the pipelined host
+    path, with pinned (hipHostMalloc, through torch) and pageable (numpy)
+    buffers.  Rates count the stripe's moved bytes per second including both
+    PCIe directions; results are checked against the device path's parity."""
+    k, m = 10, 4
+    rs = trs.ReedSolomon(k, m)
+    dev = torch.empty((k + m, L), dtype=torch.uint8, device="cuda")
+    trs.fill_random(dev[:k].reshape(-1), SEED ^ 0x405)
+    rs.encode([dev[i] for i in range(k + m)])
+    want = dev.cpu().numpy()
+    del dev
+    torch.cuda.empty_cache()
+    out = {"workload": f"RS8 ({k},{m}), {L >> 20} MiB shards in host memory, 1 MiB pipeline chunks"}
+    for kind in ("pinned", "pageable"):
+        if kind == "pinned":
+            rows = [torch.from_numpy(want[i].copy()).pin_memory() for i in range(k + m)]
+            host = [r.numpy() for r in rows]
+        else:
+            rows = host = [want[i].copy() for i in range(k + m)]
+        for j in range(k, k + m):
+            host[j][:] = 0
+        rs.encode_host(rows[:k], rows[k:])
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            rs.encode_host(rows[:k], rows[k:])
+        enc_s = (time.perf_counter() - t0) / reps
+        exact = all((host[j] == want[j]).all() for j in range(k, k + m))
+        erased = [0, 3, 11, 13]
+        present = [i not in erased for i in range(k + m)]
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for i in erased:
+                host[i][:] = 0
+            rs.reconstruct_host(rows, present)
+        rec_s = (time.perf_counter() - t0) / reps
+        exact = exact and all((host[i] == want[i]).all() for i in range(k + m))
+        out[kind] = {"encode_gbs": round((k + m) * L / enc_s / 1e9, 2),
+                     "reconstruct_gbs": round((k + len(erased)) * L / rec_s / 1e9, 2),
+                     "byte_exact": bool(exact)}
+        del rows, host
+    return out
+
+
 def c1_leg(trs, torch, O, stripes: int = 256, L: int = 65536):
     """RS8 (10,4), `stripes` stripes of 14 x L bytes resident in HBM, seeded
     bytes: batched encode, then per stripe erase 4 of 14 shards chosen
@@ -527,6 +572,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=800)
     ap.add_argument("--no-tower", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip the host-resident (PCIe-inclusive) leg")
     ap.add_argument("--counters", action="store_true",
                     help="run only the measured launches (3 each) and a 1 GiB calibration "
                          "kernel, for rocprofv3 --pmc passes")
@@ -700,6 +747,9 @@ def main() -> int:
     c1 = None
     if not args.no_c1:
         c1 = c1_leg(trs, torch, O)
+    host = None
+    if not args.no_host:
+        host = host_leg(trs, torch, O)
 
     served = None
     if not args.no_serve:
@@ -759,6 +809,7 @@ def main() -> int:
         },
         "batched_vs_per_call": round(value / percall_gbs, 2),
         "c1": c1,
+        "host_path": host,
         "tower": tower,
         "served": served,
         "byte_exact": {"batched_reconstruct": rec_exact, "batched_encode": enc_exact,

@@ -317,6 +317,40 @@ int trs_server_drain(trs_server *s);
 int trs_server_take(trs_server *s, uint64_t ticket, double *latency_us, uint64_t *bytes,
                     int *status);
 
+/* ============ host-side data path and wire formats (no reference ============
+ * counterpart: the reference's shards are host slices) */
+/* ReedSolomon::encode_sep (src/reedsolomon.rs:284-309) for host-resident
+ * shards: data (k pointers) and parity (m pointers) in host memory, `len`
+ * bytes each.  The shards stream through device memory in chunks of `chunk`
+ * bytes per shard (0 = 1 MiB): the host-to-device copy of one chunk, the
+ * encode of the previous one and the device-to-host copy of the one before
+ * run on three streams at once.  Pinned caller memory (hipHostMalloc) is
+ * copied directly; pageable memory goes through library-owned pinned staging.
+ * Blocks until the parity is in host memory. */
+int trs_rs8_encode_host(const trs_rs8 *rs, const uint8_t *const *data, uint8_t *const *parity,
+                        size_t len, size_t chunk);
+/* ReedSolomon::reconstruct(_data) (src/reedsolomon.rs:470-477) for
+ * host-resident shards with presence flags, pipelined as above: only the first
+ * k present shards travel to the device and only the rebuilt shards come
+ * back.  Error order as trs_rs8_reconstruct. */
+int trs_rs8_reconstruct_host(const trs_rs8 *rs, uint8_t *const *shards, const uint8_t *present,
+                             size_t n, size_t len, int data_only, size_t chunk);
+/* ReedSolomon::reconstruct with Option<T> shards (src/reedsolomon.rs:88-101,
+ * :842-847): a NULL entry of `shards` is a missing shard that the codec
+ * allocates in device memory (the present shards' length) and rebuilds; on
+ * return the entry points to it and lens[i] holds its length.  With data_only
+ * missing parity entries stay NULL, as the reference leaves them None.  Free
+ * the buffers with trs_device_free.  On an error nothing stays allocated. */
+int trs_rs8_reconstruct_alloc(const trs_rs8 *rs, uint8_t **shards, size_t *lens, size_t n,
+                              int data_only, void *stream);
+int trs_device_free(void *p);
+/* Wire formats of a GF(2^16) shard (src/reedsolomon16.rs:4-7): n_symbols
+ * interleaved little-endian u16 symbols (value (high << 8) | low) <-> the
+ * split-plane layout [low n_symbols | high n_symbols] that the RS16 calls
+ * take.  Device buffers, stream-ordered; in and out must not overlap. */
+int trs_u16_to_split(const uint16_t *symbols, size_t n_symbols, uint8_t *shard, void *stream);
+int trs_split_to_u16(const uint8_t *shard, size_t n_symbols, uint16_t *symbols, void *stream);
+
 /* ================= autotuning (no reference counterpart) ================== */
 /* Times one apply candidate on synthetic device data: `nout` outputs from `k`
  * inputs of `len` bytes over GF(2^8) (field 8) or the tower (field 16,

@@ -338,6 +338,45 @@ class ReedSolomon(_Codec):
         """Alias of encode (src/reedsolomon.rs:354-359)."""
         self.encode(shards)
 
+    def encode_host(self, data: Sequence, parity: Sequence, chunk: int = 0) -> None:
+        """encode_sep of host-resident shards (numpy uint8 arrays or CPU uint8
+        tensors, pinned or not), pipelined through the device in `chunk`-byte
+        pieces (0 = 1 MiB); returns when the parity is in host memory."""
+        d, length = _host_ptrs(data)
+        p, plen = _host_ptrs(parity)
+        if plen != length:
+            raise Error(ERROR_NAMES.index("IncorrectShardSize"))
+        _check(lib.trs_rs8_encode_host(self._h, d, p, length, chunk))
+
+    def reconstruct_host(self, shards: Sequence, present: Sequence[bool], data_only: bool = False,
+                         chunk: int = 0) -> None:
+        """reconstruct(_data) of host-resident shards with presence flags,
+        pipelined through the device; missing entries are overwritten."""
+        ptrs, length = _host_ptrs(shards)
+        _check(lib.trs_rs8_reconstruct_host(self._h, ptrs, _flags(present), len(shards), length,
+                                            int(data_only), chunk))
+
+    def reconstruct_alloc(self, shards: Sequence, data_only: bool = False) -> list:
+        """reconstruct over Option<T> shards (src/reedsolomon.rs:88-101): None
+        entries are missing shards that the codec allocates on the device and
+        rebuilds; they come back as DeviceBuffer objects (missing parity stays
+        None with data_only)."""
+        n = len(shards)
+        arr = (ctypes.c_void_p * max(1, n))()
+        lens = (ctypes.c_size_t * max(1, n))()
+        for i, t in enumerate(shards):
+            if t is not None:
+                arr[i] = _dev_ptr(t)
+                lens[i] = t.numel()
+        _check(lib.trs_rs8_reconstruct_alloc(self._h, arr, lens, n, int(data_only), _stream()))
+        out = []
+        for i, t in enumerate(shards):
+            if t is None and arr[i]:
+                out.append(DeviceBuffer(arr[i], lens[i]))
+            else:
+                out.append(t)
+        return out
+
     def encode_rows_into(self, data, row_len: int, parity) -> None:
         _check(lib.trs_rs8_encode_rows_into(self._h, ctypes.c_void_p(_dev_ptr(data)), data.numel(),
                                             row_len, ctypes.c_void_p(_dev_ptr(parity)),
@@ -618,6 +657,69 @@ def mul_slice_xor(out, inp, c: int) -> None:
     if out.numel() != inp.numel():
         raise AssertionError("assertion `left == right` failed: slice lengths differ")
     _slice(out, inp, c, True)
+
+
+def _host_ptrs(arrays: Sequence) -> tuple:
+    """Host pointers and the common length of numpy uint8 arrays or CPU uint8
+    tensors."""
+    import torch
+
+    arr = (ctypes.c_void_p * max(1, len(arrays)))()
+    length = None
+    for i, a in enumerate(arrays):
+        if isinstance(a, torch.Tensor):
+            if a.is_cuda or a.dtype != torch.uint8 or not a.is_contiguous():
+                raise TypeError("host shards must be contiguous CPU uint8 tensors or arrays")
+            arr[i], n = a.data_ptr(), a.numel()
+        else:
+            if a.dtype != np.uint8 or not a.flags["C_CONTIGUOUS"]:
+                raise TypeError("host shards must be contiguous uint8 arrays")
+            arr[i], n = a.ctypes.data, a.size
+        if length is not None and n != length:
+            raise Error(ERROR_NAMES.index("IncorrectShardSize"))
+        length = n
+    return arr, (length or 0)
+
+
+class DeviceBuffer:
+    """A shard buffer the library allocated on the device (trs_rs8_reconstruct_alloc)."""
+
+    def __init__(self, ptr: int, nbytes: int):
+        self.ptr, self.nbytes = int(ptr), int(nbytes)
+
+    def to_tensor(self):
+        """A torch copy of the buffer (through the slice kernel with c = 1)."""
+        import torch
+
+        t = torch.empty(self.nbytes, dtype=torch.uint8, device="cuda")
+        _check(lib.trs_gf8_mul_slice(ctypes.c_void_p(t.data_ptr()), self.nbytes,
+                                     ctypes.c_void_p(self.ptr), self.nbytes, 1, 0, _stream()))
+        return t
+
+    def free(self) -> None:
+        if self.ptr:
+            import torch
+
+            torch.cuda.synchronize()
+            _check(lib.trs_device_free(ctypes.c_void_p(self.ptr)))
+            self.ptr = 0
+
+
+def u16_to_split(symbols, shard) -> None:
+    """Interleaved little-endian u16 symbols (a uint8 tensor of 2n bytes) to the
+    split-plane layout [low n | high n] (src/reedsolomon16.rs:4-7)."""
+    if symbols.numel() != shard.numel() or symbols.numel() % 2:
+        raise ValueError("both buffers hold 2 * n_symbols bytes")
+    _check(lib.trs_u16_to_split(ctypes.c_void_p(_dev_ptr(symbols)), symbols.numel() // 2,
+                                ctypes.c_void_p(_dev_ptr(shard)), _stream()))
+
+
+def split_to_u16(shard, symbols) -> None:
+    """The split-plane layout back to interleaved little-endian u16 symbols."""
+    if symbols.numel() != shard.numel() or shard.numel() % 2:
+        raise ValueError("both buffers hold 2 * n_symbols bytes")
+    _check(lib.trs_split_to_u16(ctypes.c_void_p(_dev_ptr(shard)), shard.numel() // 2,
+                                ctypes.c_void_p(_dev_ptr(symbols)), _stream()))
 
 
 def fill_random(t, seed: int) -> None:

@@ -48,6 +48,12 @@ _sig("trs_status_message", ctypes.c_char_p, c_int)
 _sig("trs_version", ctypes.c_char_p)
 _sig("trs_gf8_mul_slice", c_int, c_vp, c_size, c_vp, c_size, ctypes.c_uint8, c_int, c_vp)
 _sig("trs_fill_random", c_int, c_vp, c_size, ctypes.c_uint64, c_vp)
+_sig("trs_rs8_encode_host", c_int, c_vp, c_ptrs, c_ptrs, c_size, c_size)
+_sig("trs_rs8_reconstruct_host", c_int, c_vp, c_ptrs, c_u8p, c_size, c_size, c_int, c_size)
+_sig("trs_rs8_reconstruct_alloc", c_int, c_vp, c_ptrs, c_sizep, c_size, c_int, c_vp)
+_sig("trs_device_free", c_int, c_vp)
+_sig("trs_u16_to_split", c_int, c_vp, c_size, c_vp, c_vp)
+_sig("trs_split_to_u16", c_int, c_vp, c_size, c_vp, c_vp)
 
 for fam, elem in (("rs8", c_u8p), ("rs16", c_u16p)):
     _sig(f"trs_{fam}_new", c_int, c_size, c_size, ctypes.POINTER(c_vp))

@@ -16,6 +16,7 @@
 // disappears.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstdio>
 #include <cstdlib>
@@ -2243,6 +2244,304 @@ int trs_server_take(trs_server *S, uint64_t ticket, double *latency_us, uint64_t
   if (status) *status = it->second.status;
   S->tickets.erase(it);
   return TRS_OK;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// host-side data path and wire formats (no reference counterpart: the
+// reference's buffers are host slices)
+// ============================================================================
+
+namespace {
+
+// Library-owned staging of one host-resident call: S slots, each holding the
+// call's shard rows for one chunk in device memory and, for pageable
+// callers, in pinned host memory; one stream per pipeline stage.  Kept per
+// host thread and grown on demand.
+struct HostPipe {
+  static constexpr int S = 3;
+  void *dev[S] = {};
+  void *pin[S] = {};
+  size_t dev_bytes = 0, pin_bytes = 0;
+  hipStream_t h2d = nullptr, comp = nullptr, d2h = nullptr;
+  hipEvent_t e_h2d[S] = {}, e_comp[S] = {}, e_d2h[S] = {};
+  bool ready = false;
+  ~HostPipe() {
+    for (int s = 0; s < S; ++s) {
+      if (dev[s]) (void)hipFree(dev[s]);
+      if (pin[s]) (void)hipHostFree(pin[s]);
+      if (e_h2d[s]) (void)hipEventDestroy(e_h2d[s]);
+      if (e_comp[s]) (void)hipEventDestroy(e_comp[s]);
+      if (e_d2h[s]) (void)hipEventDestroy(e_d2h[s]);
+    }
+    if (h2d) (void)hipStreamDestroy(h2d);
+    if (comp) (void)hipStreamDestroy(comp);
+    if (d2h) (void)hipStreamDestroy(d2h);
+  }
+  hipError_t setup(size_t bytes, bool staged) {
+    hipError_t e;
+    if (!ready) {
+      for (hipStream_t *st : {&h2d, &comp, &d2h})
+        if ((e = hipStreamCreateWithFlags(st, hipStreamNonBlocking)) != hipSuccess) return e;
+      for (int s = 0; s < S; ++s)
+        for (hipEvent_t *ev : {&e_h2d[s], &e_comp[s], &e_d2h[s]})
+          if ((e = hipEventCreateWithFlags(ev, hipEventDisableTiming)) != hipSuccess) return e;
+      ready = true;
+    }
+    if (bytes > dev_bytes) {
+      for (int s = 0; s < S; ++s) {
+        if (dev[s]) (void)hipFree(dev[s]);
+        dev[s] = nullptr;
+      }
+      dev_bytes = 0;
+      for (int s = 0; s < S; ++s)
+        if ((e = hipMalloc(&dev[s], bytes)) != hipSuccess) return e;
+      dev_bytes = bytes;
+    }
+    if (staged && bytes > pin_bytes) {
+      for (int s = 0; s < S; ++s) {
+        if (pin[s]) (void)hipHostFree(pin[s]);
+        pin[s] = nullptr;
+      }
+      pin_bytes = 0;
+      for (int s = 0; s < S; ++s)
+        if ((e = hipHostMalloc(&pin[s], bytes, hipHostMallocDefault)) != hipSuccess) return e;
+      pin_bytes = bytes;
+    }
+    return hipSuccess;
+  }
+};
+
+HostPipe &host_pipe() {
+  static thread_local HostPipe pipe;
+  return pipe;
+}
+
+// Pinned (page-locked, device-visible) host memory takes the direct copy path.
+bool pinned_host(const void *p) {
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // pageable memory is not known to HIP: clear the query's error
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
+}
+
+constexpr size_t kDefaultHostChunk = size_t(1) << 20;
+
+// One host-resident apply, pipelined over chunks of the shard length: chunk c
+// of the `nin` input rows goes host -> device on one stream, `run` computes on
+// the device rows of its slot on another, and the `nout` output rows come back
+// on a third, so the copies of neighbouring chunks overlap the compute.
+// in_host / out_host are the caller's rows; run(dev_rows, chunk_len, stream)
+// receives nin + nout device row pointers.
+template <typename Run>
+int host_pipeline_body(HostPipe &P, bool direct, const uint8_t *const *in_host, size_t nin,
+                       uint8_t *const *out_host, size_t nout, size_t len, size_t chunk, Run &run);
+
+template <typename Run>
+int host_pipeline(const uint8_t *const *in_host, size_t nin, uint8_t *const *out_host,
+                  size_t nout, size_t len, size_t chunk, Run run) {
+  if (chunk == 0) chunk = kDefaultHostChunk;
+  chunk = std::min(chunk, len);
+  bool direct = true;
+  for (size_t i = 0; i < nin && direct; ++i) direct = pinned_host(in_host[i]);
+  for (size_t j = 0; j < nout && direct; ++j) direct = pinned_host(out_host[j]);
+  HostPipe &P = host_pipe();
+  TRS_TRY(P.setup((nin + nout) * chunk, !direct));
+  const int st = host_pipeline_body(P, direct, in_host, nin, out_host, nout, len, chunk, run);
+  if (st) {  // nothing of this call may still be in flight on return
+    (void)hipStreamSynchronize(P.h2d);
+    (void)hipStreamSynchronize(P.comp);
+    (void)hipStreamSynchronize(P.d2h);
+  }
+  return st;
+}
+
+template <typename Run>
+int host_pipeline_body(HostPipe &P, bool direct, const uint8_t *const *in_host, size_t nin,
+                       uint8_t *const *out_host, size_t nout, size_t len, size_t chunk, Run &run) {
+  const size_t rows = nin + nout;
+  const size_t nchunks = (len + chunk - 1) / chunk;
+  auto drain = [&](size_t c) -> int {  // chunk c's outputs are home once this returns
+    const int s = int(c % HostPipe::S);
+    TRS_TRY(hipEventSynchronize(P.e_d2h[s]));
+    if (!direct) {
+      const size_t off = c * chunk, cl = std::min(chunk, len - off);
+      for (size_t j = 0; j < nout; ++j)
+        std::memcpy(out_host[j] + off, static_cast<uint8_t *>(P.pin[s]) + (nin + j) * chunk, cl);
+    }
+    return TRS_OK;
+  };
+  std::vector<uint8_t *> dev_rows(rows);
+  for (size_t c = 0; c < nchunks; ++c) {
+    const int s = int(c % HostPipe::S);
+    const size_t off = c * chunk, cl = std::min(chunk, len - off);
+    if (c >= size_t(HostPipe::S)) {
+      const int st = drain(c - HostPipe::S);  // frees slot s
+      if (st) return st;
+    }
+    uint8_t *dev = static_cast<uint8_t *>(P.dev[s]);
+    uint8_t *pin = static_cast<uint8_t *>(P.pin[s]);
+    for (size_t r = 0; r < rows; ++r) dev_rows[r] = dev + r * chunk;
+    for (size_t i = 0; i < nin; ++i) {
+      const uint8_t *src = in_host[i] + off;
+      if (!direct) {
+        std::memcpy(pin + i * chunk, src, cl);
+        src = pin + i * chunk;
+      }
+      TRS_TRY(hipMemcpyAsync(dev_rows[i], src, cl, hipMemcpyHostToDevice, P.h2d));
+    }
+    TRS_TRY(hipEventRecord(P.e_h2d[s], P.h2d));
+    TRS_TRY(hipStreamWaitEvent(P.comp, P.e_h2d[s], 0));
+    const int st = run(dev_rows.data(), cl, P.comp);
+    if (st) return st;
+    TRS_TRY(hipEventRecord(P.e_comp[s], P.comp));
+    TRS_TRY(hipStreamWaitEvent(P.d2h, P.e_comp[s], 0));
+    for (size_t j = 0; j < nout; ++j) {
+      uint8_t *dst = direct ? out_host[j] + off : pin + (nin + j) * chunk;
+      TRS_TRY(hipMemcpyAsync(dst, dev_rows[nin + j], cl, hipMemcpyDeviceToHost, P.d2h));
+    }
+    TRS_TRY(hipEventRecord(P.e_d2h[s], P.d2h));
+  }
+  const size_t first = nchunks > size_t(HostPipe::S) ? nchunks - HostPipe::S : 0;
+  for (size_t c = first; c < nchunks; ++c) {
+    const int st = drain(c);
+    if (st) return st;
+  }
+  return TRS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int trs_u16_to_split(const uint16_t *symbols, size_t n_symbols, uint8_t *shard, void *stream) {
+  if (n_symbols && (!symbols || !shard)) return TRS_ERR_INVALID_ARGUMENT;
+  return hip_status(launch_u16_split(reinterpret_cast<const uint8_t *>(symbols), shard, n_symbols,
+                                     true, as_stream(stream)));
+}
+
+int trs_split_to_u16(const uint8_t *shard, size_t n_symbols, uint16_t *symbols, void *stream) {
+  if (n_symbols && (!symbols || !shard)) return TRS_ERR_INVALID_ARGUMENT;
+  return hip_status(launch_u16_split(reinterpret_cast<const uint8_t *>(symbols),
+                                     const_cast<uint8_t *>(shard), n_symbols, false,
+                                     as_stream(stream)));
+}
+
+int trs_rs8_reconstruct_alloc(const trs_rs8 *rs, uint8_t **shards, size_t *lens, size_t n,
+                              int data_only, void *stream) {
+  if (!rs || (n && (!shards || !lens))) return TRS_ERR_INVALID_ARGUMENT;
+  if (n != rs->n) return n < rs->n ? TRS_TOO_FEW_SHARDS : TRS_TOO_MANY_SHARDS;
+  std::vector<uint8_t> present(n);
+  size_t count = 0, len = 0;
+  for (size_t i = 0; i < n; ++i) {
+    present[i] = shards[i] != nullptr;
+    if (!present[i]) continue;
+    if (lens[i] == 0) return TRS_EMPTY_SHARD;
+    if (len && lens[i] != len) return TRS_INCORRECT_SHARD_SIZE;
+    len = lens[i];
+    ++count;
+  }
+  if (count == n) return TRS_OK;
+  if (count < rs->k) return TRS_TOO_FEW_SHARDS_PRESENT;
+  // src/reedsolomon.rs:88-101: a missing Option shard is allocated (zeroed,
+  // then rebuilt); with data_only the missing parity stays None (:842-847)
+  std::vector<size_t> made;
+  std::vector<size_t> all_lens(lens, lens + n);
+  hipStream_t s = as_stream(stream);
+  for (size_t i = 0; i < n; ++i) {
+    if (present[i] || (data_only && i >= rs->k)) continue;
+    void *p = nullptr;  // every byte is rebuilt, so the zero fill is implied
+    hipError_t e = hipMalloc(&p, len);
+    if (e != hipSuccess) {
+      if (p) (void)hipFree(p);
+      for (size_t j : made) {
+        (void)hipFree(shards[j]);
+        shards[j] = nullptr;
+      }
+      return hip_status(e);
+    }
+    shards[i] = static_cast<uint8_t *>(p);
+    all_lens[i] = len;
+    made.push_back(i);
+  }
+  const int st = reconstruct8(rs, rs->k, rs->n, nullptr, shards, all_lens.data(), present.data(),
+                              n, data_only, s, rs->apply_route.load());
+  if (st) {
+    (void)hipStreamSynchronize(s);
+    for (size_t j : made) {
+      (void)hipFree(shards[j]);
+      shards[j] = nullptr;
+    }
+    return st;
+  }
+  for (size_t j : made) lens[j] = len;
+  return TRS_OK;
+}
+
+int trs_device_free(void *p) { return p ? hip_status(hipFree(p)) : TRS_OK; }
+
+int trs_rs8_encode_host(const trs_rs8 *rs, const uint8_t *const *data, uint8_t *const *parity,
+                        size_t len, size_t chunk) {
+  if (!rs || !data || !parity) return TRS_ERR_INVALID_ARGUMENT;
+  if (len == 0) return TRS_EMPTY_SHARD;
+  for (size_t i = 0; i < rs->k; ++i)
+    if (!data[i]) return TRS_ERR_INVALID_ARGUMENT;
+  for (size_t j = 0; j < rs->m; ++j)
+    if (!parity[j]) return TRS_ERR_INVALID_ARGUMENT;
+  const int st = host_pipeline(
+      data, rs->k, parity, rs->m, len, chunk, [&](uint8_t **rows, size_t cl, hipStream_t s) {
+        std::vector<const uint8_t *> ptrs(rows, rows + rs->k + rs->m);
+        return encode8_ptrs(rs, ptrs, cl, s, rs->route);
+      });
+  return st;
+}
+
+int trs_rs8_reconstruct_host(const trs_rs8 *rs, uint8_t *const *shards, const uint8_t *present,
+                             size_t n, size_t len, int data_only, size_t chunk) {
+  if (!rs || (n && (!shards || !present))) return TRS_ERR_INVALID_ARGUMENT;
+  if (n != rs->n) return n < rs->n ? TRS_TOO_FEW_SHARDS : TRS_TOO_MANY_SHARDS;
+  size_t count = 0;
+  for (size_t i = 0; i < n; ++i) count += present[i] != 0;
+  if (count == n) return TRS_OK;
+  if (len == 0) return TRS_EMPTY_SHARD;
+  if (count < rs->k) return TRS_TOO_FEW_SHARDS_PRESENT;
+  // inputs: the first k present shards (the only ones a plan reads); outputs:
+  // the missing data, then (unless data_only) the missing parity
+  std::vector<const uint8_t *> in_host;
+  std::vector<uint8_t *> out_host;
+  std::vector<size_t> in_idx, out_idx;
+  for (size_t i = 0; i < n; ++i) {
+    if (present[i]) {
+      if (in_idx.size() < rs->k) {
+        in_idx.push_back(i);
+        in_host.push_back(shards[i]);
+      }
+    } else if (i < rs->k || !data_only) {
+      out_idx.push_back(i);
+      out_host.push_back(shards[i]);
+    }
+  }
+  for (auto q : in_host)
+    if (!q) return TRS_ERR_INVALID_ARGUMENT;
+  for (auto q : out_host)
+    if (!q) return TRS_ERR_INVALID_ARGUMENT;
+  if (out_idx.empty()) return TRS_OK;  // data_only with every data shard present
+  std::vector<uint8_t> pres(present, present + n);
+  return host_pipeline(
+      in_host.data(), in_host.size(), out_host.data(), out_host.size(), len, chunk,
+      [&](uint8_t **rows, size_t cl, hipStream_t s) {
+        // device view of the stripe: present shards past the first k are never
+        // read by the plan, so they share the first input's row
+        std::vector<uint8_t *> view(n, rows[0]);
+        std::vector<size_t> lens(n, cl);
+        for (size_t r = 0; r < in_idx.size(); ++r) view[in_idx[r]] = rows[r];
+        for (size_t r = 0; r < out_idx.size(); ++r) view[out_idx[r]] = rows[in_idx.size() + r];
+        return reconstruct8(rs, rs->k, rs->n, nullptr, view.data(), lens.data(), pres.data(), n,
+                            data_only, s, rs->apply_route.load());
+      });
 }
 
 }  // extern "C"

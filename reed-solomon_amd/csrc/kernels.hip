@@ -436,6 +436,54 @@ __global__ __launch_bounds__(kBlockThreads) void k_gf16_apply(const Gf16Call a) 
     gf16_body<NT, MODE>(a, a.col0 + rel, a.cols - rel, false);
 }
 
+// Wire-format conversion between interleaved u16 symbols (symbol j in bytes
+// 2j (low) and 2j+1 (high), little-endian) and the split-plane shard layout
+// [low N | high N] (src/reedsolomon16.rs:4-7).  A lane converts 8 symbols;
+// v_perm_b32 picks the even or odd bytes of two words (selector bytes index
+// the 8-byte concatenation, the second operand first).
+__global__ __launch_bounds__(kBlockThreads) void k_u16_to_split(const uint8_t *in, uint8_t *lo,
+                                                               uint8_t *hi, uint64_t n,
+                                                               uint32_t vec_ok) {
+  const uint64_t s0 = (uint64_t(blockIdx.x) * kBlockThreads + threadIdx.x) * 8;
+  if (s0 >= n) return;
+  if (vec_ok && s0 + 8 <= n) {
+    const uint4 v = ld16(in + 2 * s0);
+    const uint32_t l0 = __builtin_amdgcn_perm(v.y, v.x, 0x06040200u);
+    const uint32_t l1 = __builtin_amdgcn_perm(v.w, v.z, 0x06040200u);
+    const uint32_t h0 = __builtin_amdgcn_perm(v.y, v.x, 0x07050301u);
+    const uint32_t h1 = __builtin_amdgcn_perm(v.w, v.z, 0x07050301u);
+    st4(lo + s0, l0);
+    st4(lo + s0 + 4, l1);
+    st4(hi + s0, h0);
+    st4(hi + s0 + 4, h1);
+  } else {
+    for (uint64_t s = s0; s < n && s < s0 + 8; ++s) {
+      st1(lo + s, ld1(in + 2 * s));
+      st1(hi + s, ld1(in + 2 * s + 1));
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlockThreads) void k_split_to_u16(const uint8_t *lo,
+                                                               const uint8_t *hi, uint8_t *out,
+                                                               uint64_t n, uint32_t vec_ok) {
+  const uint64_t s0 = (uint64_t(blockIdx.x) * kBlockThreads + threadIdx.x) * 8;
+  if (s0 >= n) return;
+  if (vec_ok && s0 + 8 <= n) {
+    const uint32_t l0 = ld4(lo + s0), l1 = ld4(lo + s0 + 4);
+    const uint32_t h0 = ld4(hi + s0), h1 = ld4(hi + s0 + 4);
+    st16(out + 2 * s0, make_uint4(__builtin_amdgcn_perm(h0, l0, 0x05010400u),
+                                  __builtin_amdgcn_perm(h0, l0, 0x07030602u),
+                                  __builtin_amdgcn_perm(h1, l1, 0x05010400u),
+                                  __builtin_amdgcn_perm(h1, l1, 0x07030602u)));
+  } else {
+    for (uint64_t s = s0; s < n && s < s0 + 8; ++s) {
+      st1(out + 2 * s, ld1(lo + s));
+      st1(out + 2 * s + 1, ld1(hi + s));
+    }
+  }
+}
+
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
@@ -539,6 +587,24 @@ hipError_t launch_batch8_planes(const BatchJob *jobs, const uint32_t *tile_job,
   if (total_tiles == 0) return hipSuccess;
   hipLaunchKernelGGL(k_batch8_planes, dim3(grid), dim3(kBlockThreads), 0, stream, jobs, tile_job,
                      total_tiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_u16_split(const uint8_t *u16, uint8_t *shard, uint64_t n, bool to_split,
+                            hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  uint8_t *lo = shard, *hi = shard + n;
+  const uintptr_t a = reinterpret_cast<uintptr_t>(u16), l = reinterpret_cast<uintptr_t>(lo),
+                  h = reinterpret_cast<uintptr_t>(hi);
+  const uint32_t vec_ok = (a % 16 == 0 && l % 4 == 0 && h % 4 == 0) ? 1u : 0u;
+  const uint64_t blocks = (n + 8ull * kBlockThreads - 1) / (8ull * kBlockThreads);
+  if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+  if (to_split)
+    hipLaunchKernelGGL(k_u16_to_split, dim3(uint32_t(blocks)), dim3(kBlockThreads), 0, stream, u16,
+                       lo, hi, n, vec_ok);
+  else
+    hipLaunchKernelGGL(k_split_to_u16, dim3(uint32_t(blocks)), dim3(kBlockThreads), 0, stream,
+                       lo, hi, const_cast<uint8_t *>(u16), n, vec_ok);
   return hipGetLastError();
 }
 

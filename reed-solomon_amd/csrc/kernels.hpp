@@ -92,6 +92,10 @@ constexpr int kPlanesOut = 8;  // output rows per planes tile (a multiple of 4; 
 hipError_t launch_batch8_planes(const BatchJob *jobs, const uint32_t *tile_job,
                                 uint64_t total_tiles, uint32_t grid, hipStream_t stream);
 hipError_t launch_fill(uint8_t *dst, uint64_t len, uint64_t seed, hipStream_t stream);
+// n interleaved u16 symbols at `u16` <-> the split-plane shard at `shard`
+// ([low n | high n]); to_split selects the direction.
+hipError_t launch_u16_split(const uint8_t *u16, uint8_t *shard, uint64_t n, bool to_split,
+                            hipStream_t stream);
 
 // int8-MFMA apply (mfma.hip); available for k <= 32.
 hipError_t launch_gf8_mfma(const Gf8MfmaCall &call, hipStream_t stream);
