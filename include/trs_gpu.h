@@ -79,6 +79,10 @@ size_t trs_rs8_total_shard_count(const trs_rs8 *rs);
 /* ReedSolomon::encode_route, src/reedsolomon.rs:239-253: the device route
  * this codec's encode takes at the given shard length. */
 const char *trs_rs8_encode_route(const trs_rs8 *rs, size_t shard_len);
+/* ReedSolomon::encode_route as the reference's x86 build with GFNI answers it
+ * (src/reedsolomon.rs:239-253, src/gf/fft_x86.rs:239-250): "fft-generated",
+ * "fft-staged" or "fused-matrix".  Host only. */
+const char *trs_rs8_reference_route(const trs_rs8 *rs, size_t shard_len);
 /* Host copy of the n x k generator matrix (row-major, n*k bytes). */
 int trs_rs8_generator(const trs_rs8 *rs, uint8_t *out, size_t out_len);
 
@@ -90,10 +94,16 @@ int trs_rs8_encode(const trs_rs8 *rs, uint8_t *const *shards, const size_t *lens
 int trs_rs8_encode_sep(const trs_rs8 *rs, const uint8_t *const *data, const size_t *data_lens,
                        size_t n_data, uint8_t *const *parity, const size_t *parity_lens,
                        size_t n_parity, void *stream);
-/* Encode through a named route (0 = fused table kernel, 1 = bit-sliced
- * constant matrix, 2 = bit-sliced LCH FFT); the bench / test hook that
- * ReedSolomon::encode_forced is in the reference (src/reedsolomon.rs:365-398).
- * TRS_ERR_UNSUPPORTED when the shape has no program for that route. */
+/* Encode through a named route (0 = matrix apply, 1 = bit-sliced constant
+ * matrix, 2 = bit-sliced LCH FFT, 3 = the staged FFT program of
+ * src/fft.rs:393-592 on the tower interpreter, even lengths only); the bench
+ * / test hook that ReedSolomon::encode_forced is in the reference
+ * (src/reedsolomon.rs:365-398).  TRS_ERR_UNSUPPORTED when the shape has no
+ * program for that route or route 3 gets an odd length.  Automatic encodes
+ * of even length from 64 KiB take route 3 for power-of-two k >= 2 without a
+ * generated program when the program has fewer than k*m/10 multiplies, and
+ * up to 1 MiB for such k >= 64 otherwise (measured crossovers; the
+ * reference's rule, src/reedsolomon.rs:204-234, is trs_rs8_reference_route). */
 int trs_rs8_encode_sep_route(const trs_rs8 *rs, int route, const uint8_t *const *data,
                              const size_t *data_lens, size_t n_data, uint8_t *const *parity,
                              const size_t *parity_lens, size_t n_parity, void *stream);
@@ -187,6 +197,22 @@ int trs_fft16_program_info(size_t k, size_t m, size_t *out, size_t n);
 int trs_fft16_lowered_program(size_t k, size_t m, uint32_t *steps, size_t steps_cap,
                               uint32_t *consts, size_t consts_cap, uint32_t *ops, size_t ops_cap,
                               uint32_t *parity, size_t parity_cap, size_t *counts);
+/* The same for the GF(2^8) staged program, src/fft.rs:393-592
+ * (build_staged_program, lower_staged; fields of StagedProgram :134-155).
+ * out[0..n): register_count, mult_count, glue ops, stages, zero registers
+ * (all of src/fft.rs's lowering), reference route (1 when the reference's
+ * x86 build routes the shape to this program, src/reedsolomon.rs:204-234),
+ * device registers, device steps, LDS bytes per 32-column group, and the
+ * multiply count after the device lowering's liveness pass.
+ * TRS_ERR_UNSUPPORTED unless k >= 1, m >= 1, k + m <= 256.  Touches no
+ * device. */
+int trs_fft8_program_info(size_t k, size_t m, size_t *out, size_t n);
+/* Host-only dump of its device lowering, as trs_fft16_lowered_program.  The
+ * constants are bytes c of the subfield, whose tower Karatsuba words are
+ * [c, c, 0]: the interpreter applies them to the low and high planes alike. */
+int trs_fft8_lowered_program(size_t k, size_t m, uint32_t *steps, size_t steps_cap,
+                             uint32_t *consts, size_t consts_cap, uint32_t *ops, size_t ops_cap,
+                             uint32_t *parity, size_t parity_cap, size_t *counts);
 /* Encode of `stripes` stripes of the codec's shape in one call: shards holds
  * stripes * (k + m) pointers (stripe s: data then parity), every shard `len`
  * bytes.  route 0: the codec's route, 1: matrix, 2: tower FFT.  The reference
@@ -357,10 +383,11 @@ int trs_split_to_u16(const uint8_t *shard, size_t n_symbols, uint16_t *symbols, 
  * split-plane, even len), random nonzero coefficients, through `route`
  * (0 = matrix route with runtime coefficients, dispatched by strip length as
  * the codec's single calls are: the table kernel below 128 KiB, the
- * bit-sliced planes kernel above; 1 = bit-sliced program: the generated GF(2^8)
- * program of the shape where it has one, or for the tower the staged LCH FFT
- * encode program of shape (k, nout) on the LDS interpreter, 2 <= k <= 1024,
- * k + nout <= 4096; 2 = int8 MFMA, GF(2^8), k <= 32) in column strips of
+ * bit-sliced planes kernel above; 1 = FFT route: the generated bit-sliced
+ * GF(2^8) program of the shape where it has one, else the staged LCH FFT
+ * encode program of shape (k, nout) on the LDS interpreter -- GF(2^8)
+ * (src/fft.rs:393-592; even len and strip) or the tower (2 <= k <= 1024,
+ * k + nout <= 4096); 2 = int8 MFMA, GF(2^8), k <= 32) in column strips of
  * `strip` bytes (0 = whole shard; one launch per strip).  *ms_out = mean milliseconds of `reps`
  * timed applies after one warm-up.  TRS_ERR_UNSUPPORTED for a route the shape
  * has no kernel for.  The reference tunes by hand (STRIP_BYTES,

@@ -314,7 +314,15 @@ class ReedSolomon(_Codec):
     def encode_route(self, shard_len: int) -> str:
         return lib.trs_rs8_encode_route(self._h, shard_len).decode()
 
-    ROUTES = {"table": 0, "bitsliced-matrix": 1, "bitsliced-fft": 2}
+    def reference_route(self, shard_len: int) -> str:
+        """The route the reference's x86 (GFNI) build takes for this shape and
+        shard length (ReedSolomon::encode_route, src/reedsolomon.rs:239-253):
+        "fft-generated", "fft-staged" or "fused-matrix"."""
+        return lib.trs_rs8_reference_route(self._h, shard_len).decode()
+
+    # "staged-fft": the GF(2^8) staged FFT program (src/fft.rs:393-592) on the
+    # tower interpreter; even shard lengths only.
+    ROUTES = {"table": 0, "bitsliced-matrix": 1, "bitsliced-fft": 2, "staged-fft": 3}
 
     def encode_sep_route(self, route: str, data: Sequence, parity: Sequence) -> None:
         """Encode through a named device route (the reference's encode_forced
@@ -506,6 +514,21 @@ def fft16_program_info(k: int, m: int) -> Optional[dict]:
     _check(rc)
     keys = ("register_count", "mult_count", "glue", "stages", "zero_regs", "profitable",
             "device_regs", "device_steps", "group_bytes")
+    return {key: int(v) for key, v in zip(keys, out)}
+
+
+def fft8_program_info(k: int, m: int) -> Optional[dict]:
+    """Host-side statistics of the GF(2^8) staged FFT program for (k, m)
+    (src/fft.rs:393-592) and of its device lowering; `reference_route` is 1
+    when the reference's x86 build routes the shape to it
+    (src/reedsolomon.rs:204-234).  None outside 1 <= k, 1 <= m, k + m <= 256."""
+    out = (ctypes.c_size_t * 10)()
+    rc = lib.trs_fft8_program_info(k, m, out, 10)
+    if rc == -4:
+        return None
+    _check(rc)
+    keys = ("register_count", "mult_count", "glue", "stages", "zero_regs", "reference_route",
+            "device_regs", "device_steps", "group_bytes", "device_mult_count")
     return {key: int(v) for key, v in zip(keys, out)}
 
 

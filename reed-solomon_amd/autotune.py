@@ -4,8 +4,9 @@ Candidates for one apply shape (field, k inputs, m outputs, shard length L)
 are a device route -- "table" (the matrix route with runtime coefficients:
 the table kernel below 128 KiB strips, the bit-sliced planes kernel above),
 "bitsliced" (the FFT route: the generated bit-sliced LCH program of a GF(2^8)
-shape that has one, or for the tower the staged LCH FFT program on the LDS
-interpreter) or "mfma" (bit-sliced int8 MFMA, GF(2^8), k <= 32) -- times a
+shape that has one, else the staged LCH FFT program on the LDS interpreter --
+GF(2^8) at even lengths, src/fft.rs:393-592, or the tower) or "mfma"
+(bit-sliced int8 MFMA, GF(2^8), k <= 32) -- times a
 column strip size (one launch per strip, 0 = whole shard), the GPU analogue of
 the reference's hand-tuned STRIP_BYTES (src/reedsolomon16.rs:796-804) and of
 its FFT-vs-matrix routing rule (src/fft16.rs:85-106, src/reedsolomon.rs:204-234).
@@ -20,7 +21,8 @@ its FFT-vs-matrix routing rule (src/fft16.rs:85-106, src/reedsolomon.rs:204-234)
   only the top ``top_n``; the exhaustive optimum judges its regret.
 * ``heuristic(shape)`` is the method without the model: the reference's
   routing rules, whole shard -- GF(2^8): the generated program when the shape
-  has one; GF(2^16): the FFT when 9 * multiplies < 2 k m (src/fft16.rs:105).
+  has one, the staged program for the shapes src/reedsolomon.rs:204-234
+  takes; GF(2^16): the FFT when 9 * multiplies < 2 k m (src/fft16.rs:105).
 """
 from __future__ import annotations
 
@@ -77,9 +79,14 @@ def _tower_fft(s: Shape) -> bool:
     return 2 <= s.k <= 1024 and s.k + s.m <= 4096
 
 
+def _byte_staged(s: Shape) -> bool:
+    return s.field == 8 and (s.k, s.m) not in GENERATED and s.length % 2 == 0
+
+
 def candidates(s: Shape) -> List[Candidate]:
     routes = ["table"]
-    if (s.field == 8 and (s.k, s.m) in GENERATED) or (s.field == 16 and _tower_fft(s)):
+    if (s.field == 8 and (s.k, s.m) in GENERATED) or (s.field == 16 and _tower_fft(s)) \
+            or _byte_staged(s):
         routes.append("bitsliced")
     if s.field == 8 and s.k <= 32:
         routes.append("mfma")
@@ -111,7 +118,7 @@ def measure(s: Shape, c: Candidate, cap_bytes: int = 32 << 20, reps: int = 2) ->
     route has no kernel for the shape."""
     route, strip = c
     window = max(64, min(s.length, cap_bytes // (s.k + s.m)))
-    window -= window % (2 if s.field == 16 else 1)
+    window -= window % (2 if s.field == 16 or (route == "bitsliced" and _byte_staged(s)) else 1)
     window = max(2, window)
     ms = ctypes.c_double(0.0)
     rc = lib.trs_tune_apply(s.field, s.k, s.m, window, ROUTES.index(route), strip, reps,
@@ -135,6 +142,12 @@ def exhaustive(s: Shape, **kw) -> Dict[Candidate, float]:
 def heuristic(s: Shape) -> Candidate:
     if s.field == 8 and (s.k, s.m) in GENERATED:
         return ("bitsliced", 0)
+    if _byte_staged(s) and s.length >= 32:
+        from . import fft8_program_info
+
+        info = fft8_program_info(s.k, s.m)
+        if info is not None and info["reference_route"]:
+            return ("bitsliced", 0)
     if s.field == 16 and _tower_fft(s):
         from . import fft16_program_info
 

@@ -399,6 +399,17 @@ struct trs_rs8 {
   DevCoef parity;
   DevBuf parity_frag;  // MFMA A fragments of the parity rows (k <= 32)
   PlanCache plans;
+  // The GF(2^8) staged FFT program (src/fft.rs:393-592) on the tower
+  // interpreter: its constants lie in the subfield, so a shard of even
+  // length L is a split-plane shard of L / 2 columns.  `staged`: the
+  // automatic encode route takes it for even lengths of at least
+  // staged_min_len() and, when `staged_max` is nonzero, at most that.  The
+  // device form is built and uploaded on first use.
+  bool staged = false;
+  size_t staged_max = 0;
+  mutable std::once_flag staged_once;
+  mutable hipError_t staged_status = hipSuccess;
+  mutable Fft16DevicePtr staged_dev;
 };
 
 struct trs_dec8 {
@@ -690,6 +701,51 @@ int trs_fill_random(uint8_t *dst, size_t len, uint64_t seed, void *stream) {
 
 // ---------------------------------------------------------------- RS8
 
+namespace {
+// Shapes whose automatic encode runs the staged FFT program, and up to which
+// shard length (0: any).  The reference takes power-of-two k whose program
+// has fewer than k*m/2 multiplies and fits 160 registers
+// (src/reedsolomon.rs:204-234).  Measured on this GPU the interpreter costs
+// about 1.2 us per program multiply per 4 MiB of shard for power-of-two k
+// (two to four times that for other k: more glue, more steps), the matrix
+// kernels about 0.034 us per k*m product per MiB once they fill the chip,
+// and below 1 MiB the matrix grid leaves the chip part idle when k >= 64.
+// So: power-of-two k that is not a generated shape, at any length when the
+// device multiply count is under k*m/10, else up to 1 MiB when k >= 64.
+// The register limit does not apply (the file lives in LDS).
+bool staged8_gpu_route(size_t k, size_t m, size_t *max_len) {
+  *max_len = 0;
+  if (k < 2 || (k & (k - 1)) || k + m > 256) return false;
+  static const size_t kGenerated[][2] = {{7, 13}, {10, 10}, {14, 14}, {16, 16}, {18, 6}, {32, 32}};
+  for (const auto &g : kGenerated)
+    if (g[0] == k && g[1] == m) return false;
+  const Fft16Program p = build_staged_program8(k, m);
+  if (!fft16_fits_device(p)) return false;
+  if (p.mult_count * 10 < k * m) return true;
+  *max_len = size_t(1) << 20;
+  return k >= 64;
+}
+
+// Shortest shard the automatic route sends to the staged FFT (env
+// TRS_STAGED8_MIN overrides).
+size_t staged_min_len() {
+  static const size_t v = [] {
+    const char *e = std::getenv("TRS_STAGED8_MIN");
+    return e ? size_t(std::strtoull(e, nullptr, 10)) : size_t(64) << 10;
+  }();
+  return v;
+}
+
+// The encode route of an automatic call at this length.
+int auto_route8(const trs_rs8 *rs, size_t len) {
+  if (rs->route == kRouteTable && rs->staged && len % 2 == 0 && len >= staged_min_len() &&
+      (rs->staged_max == 0 || len <= rs->staged_max))
+    return kRouteStaged8;
+  return rs->route;
+}
+
+}  // namespace
+
 int trs_rs8_new(size_t k, size_t m, trs_rs8 **out) {
   if (!out) return TRS_ERR_INVALID_ARGUMENT;
   *out = nullptr;
@@ -709,6 +765,7 @@ int trs_rs8_new(size_t k, size_t m, trs_rs8 **out) {
     rs->route = kRouteBsFft;
   else if (bs_encode_available(kRouteBsMatrix, uint32_t(k), uint32_t(m)))
     rs->route = kRouteBsMatrix;
+  rs->staged = rs->route == kRouteTable && staged8_gpu_route(k, m, &rs->staged_max);
   *out = rs.release();
   return TRS_OK;
 }
@@ -720,7 +777,8 @@ size_t trs_rs8_total_shard_count(const trs_rs8 *rs) { return rs ? rs->n : 0; }
 
 const char *trs_rs8_encode_route(const trs_rs8 *rs, size_t shard_len) {
   if (!rs) return "none";
-  switch (rs->route) {
+  switch (auto_route8(rs, shard_len)) {
+    case kRouteStaged8: return "gpu-staged-fft";
     case kRouteBsFft: return "gpu-bitsliced-fft";
     case kRouteBsMatrix: return "gpu-bitsliced-matrix";
     default:
@@ -730,6 +788,17 @@ const char *trs_rs8_encode_route(const trs_rs8 *rs, size_t shard_len) {
   }
 }
 
+const char *trs_rs8_reference_route(const trs_rs8 *rs, size_t shard_len) {
+  if (!rs) return "none";
+  // src/reedsolomon.rs:239-253 with src/gf/fft_x86.rs:239-250 (GFNI host,
+  // 32-byte strips)
+  if (shard_len >= 32) {
+    if (bs_encode_available(kRouteBsFft, uint32_t(rs->k), uint32_t(rs->m))) return "fft-generated";
+    if (staged8_reference_route(rs->k, rs->m)) return "fft-staged";
+  }
+  return "fused-matrix";
+}
+
 int trs_rs8_generator(const trs_rs8 *rs, uint8_t *out, size_t out_len) {
   if (!rs || !out || out_len != rs->n * rs->k) return TRS_ERR_INVALID_ARGUMENT;
   std::copy(rs->G.a.begin(), rs->G.a.end(), out);
@@ -737,6 +806,31 @@ int trs_rs8_generator(const trs_rs8 *rs, uint8_t *out, size_t out_len) {
 }
 
 namespace {
+int run_staged8(const trs_rs8 *rs, const std::vector<const uint8_t *> &ptrs, size_t len,
+                hipStream_t stream) {
+  if (len % 2) return TRS_ERR_UNSUPPORTED;
+  std::call_once(rs->staged_once, [rs] {
+    rs->staged_status = fft16_device_new(build_staged_program8(rs->k, rs->m), &rs->staged_dev);
+  });
+  if (rs->staged_status == hipErrorInvalidValue) return TRS_ERR_UNSUPPORTED;
+  if (rs->staged_status != hipSuccess) return hip_status(rs->staged_status);
+  const Fft16Device &d = *rs->staged_dev;
+  if (fft16_device_loads(d) + fft16_device_stores(d) != ptrs.size()) return TRS_ERR_UNSUPPORTED;
+  if (len == 0) return TRS_OK;
+  bool al = true;
+  for (auto q : ptrs) al = al && aligned16(q);
+  PtrTable table;
+  int st = table.upload(ptrs, stream);
+  if (st) return st;
+  Fft16Call c{};
+  c.ptrs = static_cast<const uint8_t *const *>(table.d);
+  c.plane = len / 2;
+  c.stripes = 1;
+  c.nstore = uint32_t(fft16_device_stores(d));
+  c.vec_ok = al ? 1u : 0u;
+  return hip_status(launch_fft16(d, c, stream));
+}
+
 // check_piece_count_all + check_equal_lengths (src/reedsolomon.rs:701-722).
 int check_all8(size_t total, const size_t *lens, size_t n) {
   if (n < total) return TRS_TOO_FEW_SHARDS;
@@ -752,6 +846,7 @@ int encode8_ptrs(const trs_rs8 *rs, std::vector<const uint8_t *> &ptrs, size_t l
                  hipStream_t s, int route) {
   for (auto q : ptrs)
     if (!q) return TRS_ERR_INVALID_ARGUMENT;
+  if (route == kRouteStaged8) return run_staged8(rs, ptrs, len, s);
   if (route == kRouteTable || len == 0) {
     if (use_mfma(rs->apply_route.load(), rs->k, len, false) && rs->parity_frag.p)
       return run_mfma(ptrs, rs->k, rs->m, rs->parity_frag.p, len, s);
@@ -779,7 +874,7 @@ int trs_rs8_encode(const trs_rs8 *rs, uint8_t *const *shards, const size_t *lens
   int st = check_all8(rs->n, lens, n);
   if (st) return st;
   std::vector<const uint8_t *> ptrs(shards, shards + n);
-  return encode8_ptrs(rs, ptrs, lens[0], as_stream(stream), rs->route);
+  return encode8_ptrs(rs, ptrs, lens[0], as_stream(stream), auto_route8(rs, lens[0]));
 }
 
 int trs_rs8_encode_sep(const trs_rs8 *rs, const uint8_t *const *data, const size_t *dl,
@@ -797,14 +892,14 @@ int trs_rs8_encode_sep(const trs_rs8 *rs, const uint8_t *const *data, const size
   if (len == 0) return TRS_OK;
   std::vector<const uint8_t *> ptrs(data, data + nd);
   ptrs.insert(ptrs.end(), parity, parity + np);
-  return encode8_ptrs(rs, ptrs, len, as_stream(stream), rs->route);
+  return encode8_ptrs(rs, ptrs, len, as_stream(stream), auto_route8(rs, len));
 }
 
 int trs_rs8_encode_sep_route(const trs_rs8 *rs, int route, const uint8_t *const *data,
                              const size_t *dl, size_t nd, uint8_t *const *parity, const size_t *pl,
                              size_t np, void *stream) {
   if (!rs || (nd && (!data || !dl)) || (np && (!parity || !pl))) return TRS_ERR_INVALID_ARGUMENT;
-  if (route < kRouteTable || route > kRouteBsFft) return TRS_ERR_INVALID_ARGUMENT;
+  if (route < kRouteTable || route > kRouteStaged8) return TRS_ERR_INVALID_ARGUMENT;
   if (nd != rs->k) return nd < rs->k ? TRS_TOO_FEW_DATA_SHARDS : TRS_TOO_MANY_DATA_SHARDS;
   if (np != rs->m) return np < rs->m ? TRS_TOO_FEW_PARITY_SHARDS : TRS_TOO_MANY_PARITY_SHARDS;
   const size_t len = dl[0];
@@ -832,7 +927,7 @@ int trs_rs8_encode_rows_into(const trs_rs8 *rs, const uint8_t *data, size_t data
   // The reference's rows API always takes its matrix route
   // (src/reedsolomon.rs:684-697); every route here is byte-identical, so the
   // rows take the codec's fastest one.
-  return encode8_ptrs(rs, ptrs, row_len, as_stream(stream), rs->route);
+  return encode8_ptrs(rs, ptrs, row_len, as_stream(stream), auto_route8(rs, row_len));
 }
 
 int trs_rs8_verify(const trs_rs8 *rs, const uint8_t *const *shards, const size_t *lens, size_t n,
@@ -1133,12 +1228,11 @@ int trs_fft16_program_info(size_t k, size_t m, size_t *out, size_t n) {
   return TRS_OK;
 }
 
-int trs_fft16_lowered_program(size_t k, size_t m, uint32_t *steps, size_t steps_cap,
-                              uint32_t *consts, size_t consts_cap, uint32_t *ops, size_t ops_cap,
-                              uint32_t *parity, size_t parity_cap, size_t *counts) {
-  if (!counts) return TRS_ERR_INVALID_ARGUMENT;
-  if (k < 2 || k > kMaxFftDataShards || m == 0 || k + m > 4096) return TRS_ERR_UNSUPPORTED;
-  const Fft16Lowered L = fft16_lower(build_staged_program16(k, m));
+namespace {
+// counts[0..5) and, when the buffers hold them, the words of L.
+int dump_lowered(const Fft16Lowered &L, uint32_t *steps, size_t steps_cap, uint32_t *consts,
+                 size_t consts_cap, uint32_t *ops, size_t ops_cap, uint32_t *parity,
+                 size_t parity_cap, size_t *counts) {
   counts[0] = L.steps.size() * 8;
   counts[1] = L.consts.size();
   counts[2] = L.ops.size();
@@ -1153,6 +1247,41 @@ int trs_fft16_lowered_program(size_t k, size_t m, uint32_t *steps, size_t steps_
   std::copy(L.ops.begin(), L.ops.end(), ops);
   std::copy(L.store.begin(), L.store.end(), parity);
   return TRS_OK;
+}
+
+bool fft8_shape_ok(size_t k, size_t m) { return k >= 1 && m >= 1 && k + m <= 256; }
+}  // namespace
+
+int trs_fft16_lowered_program(size_t k, size_t m, uint32_t *steps, size_t steps_cap,
+                              uint32_t *consts, size_t consts_cap, uint32_t *ops, size_t ops_cap,
+                              uint32_t *parity, size_t parity_cap, size_t *counts) {
+  if (!counts) return TRS_ERR_INVALID_ARGUMENT;
+  if (k < 2 || k > kMaxFftDataShards || m == 0 || k + m > 4096) return TRS_ERR_UNSUPPORTED;
+  return dump_lowered(fft16_lower(build_staged_program16(k, m)), steps, steps_cap, consts,
+                      consts_cap, ops, ops_cap, parity, parity_cap, counts);
+}
+
+int trs_fft8_program_info(size_t k, size_t m, size_t *out, size_t n) {
+  if (n && !out) return TRS_ERR_INVALID_ARGUMENT;
+  if (!fft8_shape_ok(k, m)) return TRS_ERR_UNSUPPORTED;
+  const Fft16Program r = build_staged_program8(k, m, false);
+  const Fft16Program p = build_staged_program8(k, m, true);
+  const Fft16Lowered L = fft16_lower(p);
+  const size_t v[10] = {r.register_count, r.mult_count, r.glue.size(),
+                        r.stages.size(), r.zero_regs.size(),
+                        staged8_reference_route(k, m) ? 1u : 0u,
+                        L.regs, L.steps.size(), L.regs * 64, p.mult_count};
+  for (size_t i = 0; i < n && i < 10; ++i) out[i] = v[i];
+  return TRS_OK;
+}
+
+int trs_fft8_lowered_program(size_t k, size_t m, uint32_t *steps, size_t steps_cap,
+                             uint32_t *consts, size_t consts_cap, uint32_t *ops, size_t ops_cap,
+                             uint32_t *parity, size_t parity_cap, size_t *counts) {
+  if (!counts) return TRS_ERR_INVALID_ARGUMENT;
+  if (!fft8_shape_ok(k, m)) return TRS_ERR_UNSUPPORTED;
+  return dump_lowered(fft16_lower(build_staged_program8(k, m)), steps, steps_cap, consts,
+                      consts_cap, ops, ops_cap, parity, parity_cap, counts);
 }
 
 int trs_decode_matrix(int field, size_t k, size_t n, const uint8_t *present, void *out,
@@ -1707,7 +1836,8 @@ struct TuneSpace {
   DevBuf data;
   DevCoef coef;
   DevBuf frag;
-  std::vector<std::pair<uint64_t, Fft16DevicePtr>> fft16;  // (k << 32 | nout) -> program
+  // (field << 48 | k << 24 | nout) -> staged FFT program on the interpreter
+  std::vector<std::pair<uint64_t, Fft16DevicePtr>> fft16;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -1734,11 +1864,18 @@ extern "C" int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len
   if (field == 16 && ((len & 1) || (strip & 1))) return TRS_ERR_INVALID_ARGUMENT;
   if (field == 8 && k + nout > 256) return TRS_ERR_INVALID_ARGUMENT;
   int bs_route = 0;
+  // FFT candidate on the interpreter: the tower program, or a GF(2^8) shape's
+  // staged program (src/fft.rs:393-592) over even lengths and strips when it
+  // has no bit-sliced program
+  bool interp = field == 16 && route == 1;
   if (field == 8 && route == 1) {
     bs_route = bs_encode_available(kRouteBsFft, k, nout)      ? kRouteBsFft
                : bs_encode_available(kRouteBsMatrix, k, nout) ? kRouteBsMatrix
                                                                 : 0;
-    if (!bs_route) return TRS_ERR_UNSUPPORTED;
+    if (!bs_route) {
+      if ((len & 1) || (strip & 1)) return TRS_ERR_UNSUPPORTED;
+      interp = true;
+    }
   } else if (field == 8 && route == 2) {
     if (!mfma_available(k)) return TRS_ERR_UNSUPPORTED;
   } else if (field == 16 && route == 1) {
@@ -1749,13 +1886,14 @@ extern "C" int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len
   TuneSpace &T = tune_space();
   std::lock_guard<std::mutex> lk(T.mu);
   const Fft16Device *prog16 = nullptr;
-  if (field == 16 && route == 1) {
-    const uint64_t key = (uint64_t(k) << 32) | nout;
+  if (interp) {
+    const uint64_t key = (uint64_t(field) << 48) | (uint64_t(k) << 24) | nout;
     for (auto &e : T.fft16)
       if (e.first == key) prog16 = e.second.get();
     if (!prog16) {
       Fft16DevicePtr d;
-      const hipError_t e = fft16_device_new(build_staged_program16(k, nout), &d);
+      const hipError_t e = fft16_device_new(
+          field == 16 ? build_staged_program16(k, nout) : build_staged_program8(k, nout), &d);
       if (e == hipErrorInvalidValue) return TRS_ERR_UNSUPPORTED;
       TRS_TRY(e);
       if (T.fft16.size() >= 64) T.fft16.clear();
@@ -1790,22 +1928,25 @@ extern "C" int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len
   if (st) return st;
   uint8_t *base = static_cast<uint8_t *>(T.data.p);
   const uint64_t sw = (strip == 0 || strip >= len) ? len : strip;
+  // the interpreter and the tower kernel address split planes of len / 2
+  // columns (a GF(2^8) shard on the interpreter pairs byte c with L/2 + c)
+  const bool planes = field == 16 || interp;
   PtrTable table16;
-  if (field == 16) {
-    // route 0: one table; route 1: one table per column strip (pointers at
-    // the strip's column offset in the low plane)
+  if (planes) {
+    // tower matrix: one table; interpreter: one table per column strip
+    // (pointers at the strip's column offset in the low plane)
     std::vector<const uint8_t *> p16;
-    const uint64_t step = route == 1 ? sw / 2 : len / 2;
+    const uint64_t step = interp ? sw / 2 : len / 2;
     for (uint64_t off = 0; off < len / 2; off += step)
       for (uint32_t i = 0; i < k + nout; ++i) p16.push_back(base + uint64_t(i) * len + off);
     if ((st = table16.upload(p16, T.stream))) return st;
   }
   auto run_once = [&]() -> int {
-    for (uint64_t off = 0; off < (field == 16 ? len / 2 : len); off += (field == 16 ? sw / 2 : sw)) {
+    for (uint64_t off = 0; off < (planes ? len / 2 : len); off += (planes ? sw / 2 : sw)) {
       const uint64_t span =
-          std::min<uint64_t>(field == 16 ? sw / 2 : sw, (field == 16 ? len / 2 : len) - off);
+          std::min<uint64_t>(planes ? sw / 2 : sw, (planes ? len / 2 : len) - off);
       hipError_t e = hipSuccess;
-      if (field == 16 && route == 1) {
+      if (interp) {
         Fft16Call c{};
         c.ptrs = static_cast<const uint8_t *const *>(table16.d) + (off / (sw / 2)) * (k + nout);
         c.plane = len / 2;
@@ -2494,7 +2635,7 @@ int trs_rs8_encode_host(const trs_rs8 *rs, const uint8_t *const *data, uint8_t *
   const int st = host_pipeline(
       data, rs->k, parity, rs->m, len, chunk, [&](uint8_t **rows, size_t cl, hipStream_t s) {
         std::vector<const uint8_t *> ptrs(rows, rows + rs->k + rs->m);
-        return encode8_ptrs(rs, ptrs, cl, s, rs->route);
+        return encode8_ptrs(rs, ptrs, cl, s, auto_route8(rs, cl));
       });
   return st;
 }

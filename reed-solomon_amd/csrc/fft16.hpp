@@ -51,6 +51,19 @@ Fft16Program build_staged_program16(size_t k, size_t m);
 // src/fft16.rs:85-106, with the multiply-count rule of x86 hosts with GFNI.
 bool fft16_profitable(const Fft16Program &p, size_t k, size_t m);
 
+// The GF(2^8) staged program (src/fft.rs:393-592) in the same form, its
+// constants bytes of the subfield.  With `liveness` the lowering also drops
+// dead glue and marks dead transform blocks (the tower lowering's passes,
+// never changing the parity); without it the multiply and register counts
+// are src/fft.rs:472-592's.  Requires 1 <= k, k + m <= 256.
+constexpr size_t kMaxFftDataShards8 = 256;
+constexpr size_t kMaxStagedRegisters8 = 160;  // src/fft.rs:130
+Fft16Program build_staged_program8(size_t k, size_t m, bool liveness = true);
+// Whether the reference's x86 build takes the staged route for (k, m)
+// (src/reedsolomon.rs:204-234): power-of-two k >= 2, not a generated shape,
+// at most 160 registers, 2 * multiplies < k * m.
+bool staged8_reference_route(size_t k, size_t m);
+
 // ---- device form ---------------------------------------------------------
 
 // One barrier-separated step.  kind 0: one level of a transform, `count`

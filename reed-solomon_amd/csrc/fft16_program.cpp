@@ -1,8 +1,10 @@
-// Staged tower-FFT program builder (host), restating src/fft16.rs:42-621:
-// WHAT tables, the interpolation / folded-chunk builder, the two base
-// strategies, and the three-pass lowering (liveness, zero normalisation,
-// per-block skip flags).  The device lowering (fft16_device_new) is separate
-// and lives with the kernel in fft16.hip.
+// Staged additive-FFT program builders (host).  The tower builder restates
+// src/fft16.rs:42-621: WHAT tables, the interpolation / folded-chunk builder,
+// the two base strategies, and the three-pass lowering (liveness, zero
+// normalisation, per-block skip flags).  The GF(2^8) builder restates
+// src/fft.rs:28-79 (WHAT) and :393-592 (build_staged_program, lower_staged)
+// on the same builder with the byte field's tables.  The device lowering
+// (fft16_device_new) is separate and lives with the kernel in fft16.hip.
 #include <algorithm>
 #include <mutex>
 
@@ -46,6 +48,39 @@ uint16_t fft16_what(int level, size_t x) { return what(level, x); }
 
 namespace {
 
+// The byte field's WHAT[8][256], same recurrence over GF(2^8)
+// (src/fft.rs:28-79).
+const std::vector<uint8_t> &what8_table() {
+  static std::vector<uint8_t> t;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    const Gf8 &f = gf8();
+    t.assign(size_t(8) << 8, 0);
+    std::vector<uint8_t> w(256), nxt(256);
+    for (uint32_t x = 0; x < 256; ++x) w[x] = uint8_t(x);
+    for (int j = 0; j < 8; ++j) {
+      const uint8_t piv = w[size_t(1) << j];
+      const uint8_t inv = f.inv(piv);
+      for (uint32_t x = 0; x < 256; ++x) t[(size_t(j) << 8) | x] = f.mul[w[x]][inv];
+      for (uint32_t x = 0; x < 256; ++x) nxt[x] = f.mul[w[x]][w[x] ^ piv];
+      w.swap(nxt);
+    }
+  });
+  return t;
+}
+
+// The field a program is built over: its WHAT rows and product.
+struct FieldOps {
+  uint16_t (*what)(int level, size_t x);
+  uint16_t (*mul)(uint16_t a, uint16_t b);
+};
+
+const FieldOps kTower{[](int level, size_t x) { return what(level, x); },
+                      [](uint16_t a, uint16_t b) { return gf16().mul(a, b); }};
+const FieldOps kByte{
+    [](int level, size_t x) { return uint16_t(what8_table()[(size_t(level) << 8) | x]); },
+    [](uint16_t a, uint16_t b) { return uint16_t(gf8().mul[a & 0xff][b & 0xff]); }};
+
 inline int log2_exact(size_t n) {
   int l = 0;
   while ((size_t(1) << l) < n) ++l;
@@ -67,6 +102,10 @@ struct Item {
 };
 
 struct Builder {
+  const FieldOps *f = &kTower;
+  // Folded chunks end in fft_with_zero_tail (tower, src/fft16.rs:341) or a
+  // plain forward transform (byte field, src/fft.rs:449).
+  bool zero_tail = true;
   std::vector<Item> items;
   size_t next = 0;
 
@@ -102,7 +141,7 @@ struct Builder {
       return;
     }
     transform(true, regs, half, beta);
-    const uint16_t c = what(log2_exact(n) - 1, beta);
+    const uint16_t c = f->what(log2_exact(n) - 1, beta);
     const std::vector<uint16_t> scratch = alloc(half);
     for (size_t j = 0; j < half; ++j) glue(kOpXor, scratch[j], regs[j], 0);
     transform(false, scratch.data(), half, beta ^ half);
@@ -118,7 +157,7 @@ struct Builder {
       return;
     }
     const size_t half = size / 2;
-    const uint16_t c = what(log2_exact(size) - 1, beta);
+    const uint16_t c = f->what(log2_exact(size) - 1, beta);
     for (size_t i = 0; i < half; ++i) {
       mul_xor(regs[i], regs[half + i], c);
       glue(kOpXor, regs[half + i], regs[i], 0);
@@ -134,14 +173,13 @@ struct Builder {
     const int tlevels = log2_exact(transform_n);
     const std::vector<uint16_t> chunk = alloc(size);
     const int shift = log2_exact(size);
-    const Gf16 &f = gf16();
     for (size_t j = 0; j < size; ++j) {
       for (size_t h = 0; h < transform_n / size; ++h) {
         const size_t index = h * size + j;
         if (index >= k) continue;
         uint16_t factor = 1;
         for (int level = shift; level < tlevels; ++level)
-          if (index & (size_t(1) << level)) factor = f.mul(factor, what(level, position));
+          if (index & (size_t(1) << level)) factor = f->mul(factor, f->what(level, position));
         if (factor == 0) continue;
         if (factor == 1)
           glue(kOpXor, chunk[j], base[index], 0);
@@ -149,18 +187,24 @@ struct Builder {
           glue(kOpMulXor, chunk[j], base[index], factor);
       }
     }
-    fft_with_zero_tail(chunk.data(), size, position, std::min(k, size));
+    if (zero_tail)
+      fft_with_zero_tail(chunk.data(), size, position, std::min(k, size));
+    else
+      transform(false, chunk.data(), size, position);
     return chunk;
   }
 };
 
-// src/fft16.rs:438-621
-Fft16Program lower(const Builder &b, const std::vector<uint16_t> &parity, size_t k, size_t m) {
+// src/fft16.rs:438-621.  Without `liveness` every item is kept and every
+// block counts as live, which is src/fft.rs:472-592 (same multiply count).
+Fft16Program lower(const Builder &b, const std::vector<uint16_t> &parity, size_t k, size_t m,
+                   bool liveness = true) {
+  const FieldOps &f = *b.f;
   const size_t nreg = b.next;
-  std::vector<uint8_t> live(nreg, 0), keep(b.items.size(), 0);
+  std::vector<uint8_t> live(nreg, liveness ? 0 : 1), keep(b.items.size(), liveness ? 0 : 1);
   std::vector<std::vector<uint8_t>> live_out(b.items.size());
   for (uint16_t r : parity) live[r] = 1;
-  for (size_t i = b.items.size(); i-- > 0;) {
+  for (size_t i = b.items.size(); liveness && i-- > 0;) {
     const Item &it = b.items[i];
     if (!it.transform) {
       if (live[it.op.dst]) {
@@ -212,9 +256,9 @@ Fft16Program lower(const Builder &b, const std::vector<uint16_t> &parity, size_t
         const int level = it.inverse ? step : levels - 1 - step;
         const size_t stride = size_t(1) << level;
         for (size_t block = 0; block < it.size; block += 2 * stride) {
-          const uint16_t c = what(level, it.beta ^ block);
-          bool any = false;
-          for (size_t j = block; j < block + 2 * stride; ++j) any |= alive[j] != 0;
+          const uint16_t c = f.what(level, it.beta ^ block);
+          bool any = !liveness;
+          for (size_t j = block; liveness && j < block + 2 * stride; ++j) any |= alive[j] != 0;
           p.consts.push_back(c);
           p.consts_skip.push_back(!any);
           if (c && any) p.mult_count += stride;
@@ -266,7 +310,49 @@ Fft16Program build_with(size_t k, size_t m, Strategy strategy) {
   return lower(b, parity, k, m);
 }
 
+// src/fft.rs:393-467: interpolation, folded trailing chunks with plain
+// forward transforms, then the base transform last.
+Fft16Program build_byte(size_t k, size_t m, bool liveness) {
+  const size_t n = k + m;
+  const size_t transform_n = next_pow2(k);
+  Builder b;
+  b.f = &kByte;
+  b.zero_tail = false;
+  const std::vector<uint16_t> base = b.alloc(transform_n);
+  b.interpolate(base.data(), transform_n, 0, k);
+  std::vector<uint16_t> parity(m, 0);
+  size_t position = transform_n;
+  while (position < n) {
+    const size_t alignment = position & (~position + 1);
+    const size_t size = std::min({alignment, next_pow2(n - position), transform_n});
+    const std::vector<uint16_t> chunk = b.folded_chunk(base, k, position, size);
+    for (size_t off = 0; off < size; ++off) {
+      const size_t point = position + off;
+      if (point >= k && point < n) parity[point - k] = chunk[off];
+    }
+    position += size;
+  }
+  b.transform(false, base.data(), transform_n, 0);
+  const size_t base_end = std::min(n, transform_n);
+  for (size_t i = k; i < base_end; ++i) parity[i - k] = base[i];
+  return lower(b, parity, k, m, liveness);
+}
+
 }  // namespace
+
+Fft16Program build_staged_program8(size_t k, size_t m, bool liveness) {
+  return build_byte(k, m, liveness);
+}
+
+bool staged8_reference_route(size_t k, size_t m) {
+  static const size_t kGenerated[][2] = {{7, 13}, {10, 10}, {14, 14}, {16, 16}, {18, 6}, {32, 32}};
+  for (const auto &g : kGenerated)
+    if (g[0] == k && g[1] == m) return false;
+  if (k < 2 || (k & (k - 1)) || k > kMaxFftDataShards8 || m == 0) return false;
+  const Fft16Program p = build_byte(k, m, false);
+  if (p.register_count > kMaxStagedRegisters8) return false;
+  return p.mult_count * 2 < k * m;
+}
 
 Fft16Program build_staged_program16(size_t k, size_t m) {
   const size_t n = k + m;

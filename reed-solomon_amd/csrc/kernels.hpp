@@ -108,7 +108,9 @@ hipError_t launch_batch_mfma(const BatchJob *jobs, const uint32_t *tile_job,
 
 // Bit-sliced encode with compile-time coefficients (gen_bitsliced.hip,
 // generated by gf8_codegen.py).  route: kRouteBsMatrix or kRouteBsFft.
-enum EncodeRoute : int { kRouteTable = 0, kRouteBsMatrix = 1, kRouteBsFft = 2 };
+// kRouteStaged8: the GF(2^8) staged FFT program on the tower interpreter
+// (codec.cpp), not a bit-sliced kernel.
+enum EncodeRoute : int { kRouteTable = 0, kRouteBsMatrix = 1, kRouteBsFft = 2, kRouteStaged8 = 3 };
 hipError_t launch_bs_encode(int route, const Gf8Call &call, hipStream_t stream);
 bool bs_encode_available(int route, uint32_t k, uint32_t m);
 uint32_t bs_program_id(int route, uint32_t k, uint32_t m);

@@ -99,19 +99,24 @@ SKIP, ZERO, ONE = 1 << 24, 2 << 24, 4 << 24
 K_XOR, K_COPY, K_MULXOR, K_MULCOPY, K_ZERO = range(5)
 
 
-def _lowered(trs, k, m):
-    lib = trs._lib.lib
+def _lowered_from(fn, k, m):
+    """(steps, consts, ops, parity registers, register count) of a lowered
+    program dump (trs_fft16_lowered_program or trs_fft8_lowered_program)."""
     counts = (ctypes.c_size_t * 5)()
     null = ctypes.POINTER(ctypes.c_uint32)()
-    assert lib.trs_fft16_lowered_program(k, m, null, 0, null, 0, null, 0, null, 0, counts) == 0
+    assert fn(k, m, null, 0, null, 0, null, 0, null, 0, counts) == 0
     bufs = [np.zeros(max(1, int(c)), np.uint32) for c in counts[:4]]
     ptr = [b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) for b in bufs]
-    rc = lib.trs_fft16_lowered_program(k, m, ptr[0], bufs[0].size, ptr[1], bufs[1].size,
-                                       ptr[2], bufs[2].size, ptr[3], bufs[3].size, counts)
+    rc = fn(k, m, ptr[0], bufs[0].size, ptr[1], bufs[1].size, ptr[2], bufs[2].size, ptr[3],
+            bufs[3].size, counts)
     assert rc == 0
     steps = bufs[0][:counts[0]].reshape(-1, 8).astype(np.int64)
     return (steps, bufs[1][:counts[1]].astype(np.int64), bufs[2][:counts[2]].astype(np.int64),
             bufs[3][:counts[3]].astype(np.int64), int(counts[4]))
+
+
+def _lowered(trs, k, m):
+    return _lowered_from(trs._lib.lib.trs_fft16_lowered_program, k, m)
 
 
 def _kara(cw, x):

@@ -1,0 +1,108 @@
+"""GPU parity of the GF(2^8) staged FFT route (src/fft.rs:393-592 on the tower
+interpreter) through the C ABI, against the oracle's generator-matrix encode.
+
+Mirrors the reference's staged_matches_scalar test (src/reedsolomon.rs:997-1021:
+staged and runtime shapes at ragged lengths) with the forced route, adds
+full-size shards, unaligned shard pointers, the automatic route choice and the
+reference-route report (src/reedsolomon.rs:239-253)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from util import rows_to_dev, seeded, to_host
+
+pytestmark = pytest.mark.gpu
+
+# src/reedsolomon.rs:999-1006 plus the reference's staged shapes and wide ones
+SHAPES = [(12, 8), (14, 14), (16, 16), (18, 6), (24, 8), (32, 32), (2, 2), (4, 4), (8, 8),
+          (64, 64), (128, 32), (1, 4), (37, 21), (100, 156), (200, 55), (128, 128)]
+LENGTHS = [2, 16, 32, 34, 100, 1000, 4096, 4098, 65536 + 64]
+
+
+def _forced(trs, rs, k, m, data):
+    length = data.shape[1]
+    dev = rows_to_dev(np.concatenate([data, np.zeros((m, length), np.uint8)]))
+    rs.encode_sep_route("staged-fft", dev[:k], dev[k:])
+    return to_host(dev[k:])
+
+
+@pytest.mark.parametrize("k,m", SHAPES)
+def test_staged_route_matches_oracle(trs, k, m):
+    print(f"staged route k={k} m={m}", flush=True)
+    G = O.build_matrix(8, k, k + m)
+    rs = trs.ReedSolomon(k, m)
+    for length in LENGTHS:
+        data = seeded(k, length, 0x51A6ED + length)
+        assert (_forced(trs, rs, k, m, data) == O.rs8_encode(G, k, data)).all(), (k, m, length)
+
+
+@pytest.mark.parametrize("k,m", [(64, 64), (8, 8), (128, 32)])
+def test_staged_route_full_size(trs, k, m):
+    # 1 MiB and 4 MiB + 32 shards: many workgroups, vector accesses
+    G = O.build_matrix(8, k, k + m)
+    rs = trs.ReedSolomon(k, m)
+    for length in ((1 << 20), (4 << 20) + 32):
+        if k * length > (256 << 20):
+            continue
+        data = seeded(k, length, 0xF00D + k)
+        cols = slice(0, length, 4099)  # the oracle on a column sample keeps the check quick
+        got = _forced(trs, rs, k, m, data)
+        want = O.rs8_encode(G, k, np.ascontiguousarray(data[:, cols]))
+        assert (got[:, cols] == want).all(), (k, m, length)
+        # every column: the matrix route on the GPU is already pinned to the oracle
+        dev = rows_to_dev(np.concatenate([data, np.zeros((m, length), np.uint8)]))
+        rs.encode_sep_route("table", dev[:k], dev[k:])
+        assert (to_host(dev[k:]) == got).all(), (k, m, length)
+
+
+def test_staged_route_unaligned_pointers(trs):
+    import torch
+
+    k, m, length = 64, 64, 8192 + 64
+    G = O.build_matrix(8, k, k + m)
+    rs = trs.ReedSolomon(k, m)
+    data = seeded(k, length, 0xA11)
+    big = torch.zeros((k + m) * (length + 16), dtype=torch.uint8, device="cuda")
+    views = [big[i * (length + 16) + 3: i * (length + 16) + 3 + length] for i in range(k + m)]
+    for i in range(k):
+        views[i].copy_(torch.from_numpy(data[i]))
+    rs.encode_sep_route("staged-fft", views[:k], views[k:])
+    assert (to_host(views[k:]) == O.rs8_encode(G, k, data)).all()
+
+
+def test_staged_route_rejects_odd_length(trs):
+    k, m = 8, 8
+    rs = trs.ReedSolomon(k, m)
+    data = seeded(k, 101, 5)
+    with pytest.raises(trs.Error):
+        _forced(trs, rs, k, m, data)
+
+
+def test_automatic_route_and_reports(trs):
+    k, m = 64, 64
+    G = O.build_matrix(8, k, k + m)
+    rs = trs.ReedSolomon(k, m)
+    assert rs.encode_route(1 << 20) == "gpu-staged-fft"
+    assert rs.encode_route((1 << 20) + 1) != "gpu-staged-fft"
+    assert rs.encode_route(4096) != "gpu-staged-fft"
+    assert rs.reference_route(1 << 20) == "fft-staged"
+    assert rs.reference_route(31) == "fused-matrix"
+    for length in ((1 << 20), (1 << 20) + 1):
+        data = seeded(k, length, 0xB0B + length)
+        dev = rows_to_dev(np.concatenate([data, np.zeros((m, length), np.uint8)]))
+        rs.encode(dev)
+        cols = slice(0, length, 1021)
+        want = O.rs8_encode(G, k, np.ascontiguousarray(data[:, cols]))
+        assert (to_host(dev[k:])[:, cols] == want).all(), length
+    # the device rule: any length when the program's multiplies are under
+    # k*m/10, up to 1 MiB for k >= 64 otherwise, never below k = 64
+    r64_16 = trs.ReedSolomon(64, 16)
+    assert r64_16.encode_route(1 << 20) == "gpu-staged-fft"
+    assert r64_16.encode_route(4 << 20) != "gpu-staged-fft"
+    assert trs.ReedSolomon(128, 128).encode_route(4 << 20) == "gpu-staged-fft"
+    assert trs.ReedSolomon(8, 8).encode_route(1 << 20) != "gpu-staged-fft"
+    assert trs.ReedSolomon(12, 8).encode_route(1 << 20) != "gpu-staged-fft"
+    assert trs.ReedSolomon(16, 16).reference_route(64) == "fft-generated"
+    assert trs.ReedSolomon(12, 8).reference_route(1 << 20) == "fused-matrix"
+    assert trs.ReedSolomon(128, 128).reference_route(1 << 20) == "fused-matrix"
+    assert trs.ReedSolomon(128, 32).reference_route(1 << 20) == "fft-staged"
