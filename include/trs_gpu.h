@@ -100,9 +100,9 @@ int trs_rs8_encode_sep(const trs_rs8 *rs, const uint8_t *const *data, const size
  * / test hook that ReedSolomon::encode_forced is in the reference
  * (src/reedsolomon.rs:365-398).  TRS_ERR_UNSUPPORTED when the shape has no
  * program for that route or route 3 gets an odd length.  Automatic encodes
- * of even length from 64 KiB take route 3 for power-of-two k >= 2 without a
- * generated program when the program has fewer than k*m/10 multiplies, and
- * up to 1 MiB for such k >= 64 otherwise (measured crossovers; the
+ * of even length take route 3 when k > 32: for k = 64 or 128 at any length
+ * when the program has fewer than k*m/10 multiplies, else up to 1 MiB; for
+ * other k up to 256 KiB (measured crossovers against the matrix route; the
  * reference's rule, src/reedsolomon.rs:204-234, is trs_rs8_reference_route). */
 int trs_rs8_encode_sep_route(const trs_rs8 *rs, int route, const uint8_t *const *data,
                              const size_t *data_lens, size_t n_data, uint8_t *const *parity,

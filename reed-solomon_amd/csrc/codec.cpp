@@ -382,6 +382,30 @@ struct PtrTable {
   }
 };
 
+// One stripe of a device program: ptrs = its loaded then stored shards,
+// `plane` columns.  Up to kFftInlinePtrs pointers travel in the kernel
+// arguments; more go through a device table.
+int launch_fft16_one(const Fft16Device &d, const std::vector<const uint8_t *> &ptrs,
+                     size_t nstore, uint64_t plane, hipStream_t stream) {
+  bool al = true;
+  for (auto q : ptrs) al = al && aligned16(q);
+  Fft16Call c{};
+  c.plane = plane;
+  c.stripes = 1;
+  c.nstore = uint32_t(nstore);
+  c.vec_ok = al ? 1u : 0u;
+  if (ptrs.size() <= kFftInlinePtrs) {
+    Fft16Ptrs P{};
+    std::copy(ptrs.begin(), ptrs.end(), P.p);
+    return hip_status(launch_fft16(d, c, stream, &P));
+  }
+  PtrTable table;
+  int st = table.upload(ptrs, stream);
+  if (st) return st;
+  c.ptrs = static_cast<const uint8_t *const *>(table.d);
+  return hip_status(launch_fft16(d, c, stream));
+}
+
 }  // namespace
 }  // namespace trs
 
@@ -401,11 +425,9 @@ struct trs_rs8 {
   PlanCache plans;
   // The GF(2^8) staged FFT program (src/fft.rs:393-592) on the tower
   // interpreter: its constants lie in the subfield, so a shard of even
-  // length L is a split-plane shard of L / 2 columns.  `staged`: the
-  // automatic encode route takes it for even lengths of at least
-  // staged_min_len() and, when `staged_max` is nonzero, at most that.  The
+  // length L is a split-plane shard of L / 2 columns.  The automatic encode
+  // route takes it for even lengths up to `staged_max` (0: never).  The
   // device form is built and uploaded on first use.
-  bool staged = false;
   size_t staged_max = 0;
   mutable std::once_flag staged_once;
   mutable hipError_t staged_status = hipSuccess;
@@ -702,45 +724,35 @@ int trs_fill_random(uint8_t *dst, size_t len, uint64_t seed, void *stream) {
 // ---------------------------------------------------------------- RS8
 
 namespace {
-// Shapes whose automatic encode runs the staged FFT program, and up to which
-// shard length (0: any).  The reference takes power-of-two k whose program
-// has fewer than k*m/2 multiplies and fits 160 registers
-// (src/reedsolomon.rs:204-234).  Measured on this GPU the interpreter costs
-// about 1.2 us per program multiply per 4 MiB of shard for power-of-two k
-// (two to four times that for other k: more glue, more steps), the matrix
-// kernels about 0.034 us per k*m product per MiB once they fill the chip,
-// and below 1 MiB the matrix grid leaves the chip part idle when k >= 64.
-// So: power-of-two k that is not a generated shape, at any length when the
-// device multiply count is under k*m/10, else up to 1 MiB when k >= 64.
-// The register limit does not apply (the file lives in LDS).
-bool staged8_gpu_route(size_t k, size_t m, size_t *max_len) {
-  *max_len = 0;
-  if (k < 2 || (k & (k - 1)) || k + m > 256) return false;
-  static const size_t kGenerated[][2] = {{7, 13}, {10, 10}, {14, 14}, {16, 16}, {18, 6}, {32, 32}};
-  for (const auto &g : kGenerated)
-    if (g[0] == k && g[1] == m) return false;
-  const Fft16Program p = build_staged_program8(k, m);
-  if (!fft16_fits_device(p)) return false;
-  if (p.mult_count * 10 < k * m) return true;
-  *max_len = size_t(1) << 20;
-  return k >= 64;
-}
-
-// Shortest shard the automatic route sends to the staged FFT (env
-// TRS_STAGED8_MIN overrides).
-size_t staged_min_len() {
-  static const size_t v = [] {
-    const char *e = std::getenv("TRS_STAGED8_MIN");
-    return e ? size_t(std::strtoull(e, nullptr, 10)) : size_t(64) << 10;
+// Longest even shard length at which the automatic encode of (k, m) runs the
+// staged FFT program (0: never).  The reference takes power-of-two k whose
+// program has fewer than k*m/2 multiplies and fits 160 registers
+// (src/reedsolomon.rs:204-234); its register limit does not apply here (the
+// file lives in LDS).  Measured on this GPU at full size against the
+// automatic matrix route (51 shapes, 16 KiB - 4 MiB): for k > 32 the matrix
+// kernels leave most of the chip idle up to a few hundred KiB (the MFMA route
+// covers k <= 32) and the program is 1.1-4x faster through 256 KiB; beyond,
+// the interpreter's cost per multiply (about 1.2 us per 4 MiB for
+// power-of-two k, two to four times that for other k) decides, and only
+// k = 64, 128 stay ahead up to 1 MiB, and at any length when the program
+// has fewer than k*m/10 multiplies.  Generated shapes keep their bit-sliced
+// programs.  Env TRS_STAGED8=0 turns the route off.
+size_t staged8_gpu_max(size_t k, size_t m) {
+  static const bool off = [] {
+    const char *e = std::getenv("TRS_STAGED8");
+    return e && e[0] == '0';
   }();
-  return v;
+  if (off || k <= 32 || k + m > 256) return 0;
+  const Fft16Program p = build_staged_program8(k, m);
+  if (!fft16_fits_device(p)) return 0;
+  if ((k & (k - 1)) == 0)
+    return p.mult_count * 10 < k * m ? SIZE_MAX : size_t(1) << 20;
+  return size_t(256) << 10;
 }
 
 // The encode route of an automatic call at this length.
 int auto_route8(const trs_rs8 *rs, size_t len) {
-  if (rs->route == kRouteTable && rs->staged && len % 2 == 0 && len >= staged_min_len() &&
-      (rs->staged_max == 0 || len <= rs->staged_max))
-    return kRouteStaged8;
+  if (rs->route == kRouteTable && len % 2 == 0 && len <= rs->staged_max) return kRouteStaged8;
   return rs->route;
 }
 
@@ -765,7 +777,7 @@ int trs_rs8_new(size_t k, size_t m, trs_rs8 **out) {
     rs->route = kRouteBsFft;
   else if (bs_encode_available(kRouteBsMatrix, uint32_t(k), uint32_t(m)))
     rs->route = kRouteBsMatrix;
-  rs->staged = rs->route == kRouteTable && staged8_gpu_route(k, m, &rs->staged_max);
+  if (rs->route == kRouteTable) rs->staged_max = staged8_gpu_max(k, m);
   *out = rs.release();
   return TRS_OK;
 }
@@ -817,18 +829,7 @@ int run_staged8(const trs_rs8 *rs, const std::vector<const uint8_t *> &ptrs, siz
   const Fft16Device &d = *rs->staged_dev;
   if (fft16_device_loads(d) + fft16_device_stores(d) != ptrs.size()) return TRS_ERR_UNSUPPORTED;
   if (len == 0) return TRS_OK;
-  bool al = true;
-  for (auto q : ptrs) al = al && aligned16(q);
-  PtrTable table;
-  int st = table.upload(ptrs, stream);
-  if (st) return st;
-  Fft16Call c{};
-  c.ptrs = static_cast<const uint8_t *const *>(table.d);
-  c.plane = len / 2;
-  c.stripes = 1;
-  c.nstore = uint32_t(fft16_device_stores(d));
-  c.vec_ok = al ? 1u : 0u;
-  return hip_status(launch_fft16(d, c, stream));
+  return launch_fft16_one(d, ptrs, fft16_device_stores(d), len / 2, stream);
 }
 
 // check_piece_count_all + check_equal_lengths (src/reedsolomon.rs:701-722).
@@ -1093,6 +1094,7 @@ int run_fft16(const trs_rs16 *rs, const std::vector<const uint8_t *> &ptrs, size
   int st = fft16_device_for(rs, &d);
   if (st) return st;
   if (len == 0 || stripes == 0) return TRS_OK;
+  if (stripes == 1) return launch_fft16_one(*d, ptrs, fft16_device_stores(*d), len / 2, stream);
   bool al = true;
   for (auto q : ptrs) al = al && aligned16(q);
   PtrTable table;
@@ -1112,18 +1114,7 @@ int run_fft16_program(const Fft16Device &d, const std::vector<const uint8_t *> &
                       size_t nstore, size_t len, hipStream_t stream) {
   if (len == 0 || nstore == 0) return TRS_OK;
   if (ptrs.size() != fft16_device_loads(d) + nstore) return TRS_ERR_INVALID_ARGUMENT;
-  bool al = true;
-  for (auto q : ptrs) al = al && aligned16(q);
-  PtrTable table;
-  int st = table.upload(ptrs, stream);
-  if (st) return st;
-  Fft16Call c{};
-  c.ptrs = static_cast<const uint8_t *const *>(table.d);
-  c.plane = len / 2;
-  c.stripes = 1;
-  c.nstore = uint32_t(nstore);
-  c.vec_ok = al ? 1u : 0u;
-  return hip_status(launch_fft16(d, c, stream));
+  return launch_fft16_one(d, ptrs, nstore, len / 2, stream);
 }
 
 int encode16_ptrs(const trs_rs16 *rs, const std::vector<const uint8_t *> &ptrs, size_t len,

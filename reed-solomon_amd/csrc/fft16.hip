@@ -561,6 +561,15 @@ __global__ __launch_bounds__(T) void k_fft16(Fft16Args a) {
   run_program<T>(a, g0, a.ptrs + uint64_t(stripe) * (a.nload + a.nstore), rf);
 }
 
+// One stripe whose shard pointers travel in the kernel arguments (no device
+// table to allocate and copy per call).
+template <int T>
+__global__ __launch_bounds__(T) void k_fft16_inline(Fft16Args a, Fft16Ptrs P) {
+  extern __shared__ uint4 rf[];
+  const uint32_t wg = xcd_order(blockIdx.x, a.total);
+  run_program<T>(a, wg << a.gshift, P.p, rf);
+}
+
 // Many programs (one per job, e.g. one decode pattern per stripe) in one
 // launch: workgroup w runs job wg_job[w] on its column block w - first.
 template <int T>
@@ -761,9 +770,12 @@ hipError_t raise_lds_limit(size_t lds) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess || raised == dev) return e;
-  const void *kernels[6] = {reinterpret_cast<const void *>(&k_fft16<1024>),
+  const void *kernels[9] = {reinterpret_cast<const void *>(&k_fft16<1024>),
                             reinterpret_cast<const void *>(&k_fft16<512>),
                             reinterpret_cast<const void *>(&k_fft16<256>),
+                            reinterpret_cast<const void *>(&k_fft16_inline<1024>),
+                            reinterpret_cast<const void *>(&k_fft16_inline<512>),
+                            reinterpret_cast<const void *>(&k_fft16_inline<256>),
                             reinterpret_cast<const void *>(&k_fft16_jobs<1024>),
                             reinterpret_cast<const void *>(&k_fft16_jobs<512>),
                             reinterpret_cast<const void *>(&k_fft16_jobs<256>)};
@@ -784,8 +796,11 @@ uint32_t choose_threads(size_t lds, uint64_t widest) {
 }
 }  // namespace
 
-hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t stream) {
+hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t stream,
+                        const Fft16Ptrs *inline_ptrs) {
   if (call.plane == 0 || call.stripes == 0) return hipSuccess;
+  if (inline_ptrs && (call.stripes != 1 || d.nload + call.nstore > kFftInlinePtrs))
+    return hipErrorInvalidValue;
   const uint64_t cols = call.cols ? call.cols : call.plane;
   const uint64_t groups = (cols + kGroupCols - 1) / kGroupCols;
   const uint32_t gs = choose_gshift(d, groups);
@@ -814,12 +829,21 @@ hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t
   const uint32_t threads = choose_threads(lds, uint64_t(d.max_count) << gs);
   hipError_t e = raise_lds_limit(lds);
   if (e != hipSuccess) return e;
-  if (threads == 1024)
+  if (inline_ptrs) {
+    const Fft16Ptrs &P = *inline_ptrs;
+    if (threads == 1024)
+      hipLaunchKernelGGL(k_fft16_inline<1024>, dim3(uint32_t(total)), dim3(1024), lds, stream, a, P);
+    else if (threads == 512)
+      hipLaunchKernelGGL(k_fft16_inline<512>, dim3(uint32_t(total)), dim3(512), lds, stream, a, P);
+    else
+      hipLaunchKernelGGL(k_fft16_inline<256>, dim3(uint32_t(total)), dim3(256), lds, stream, a, P);
+  } else if (threads == 1024) {
     hipLaunchKernelGGL(k_fft16<1024>, dim3(uint32_t(total)), dim3(1024), lds, stream, a);
-  else if (threads == 512)
+  } else if (threads == 512) {
     hipLaunchKernelGGL(k_fft16<512>, dim3(uint32_t(total)), dim3(512), lds, stream, a);
-  else
+  } else {
     hipLaunchKernelGGL(k_fft16<256>, dim3(uint32_t(total)), dim3(256), lds, stream, a);
+  }
   return hipGetLastError();
 }
 

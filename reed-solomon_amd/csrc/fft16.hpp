@@ -138,7 +138,14 @@ Fft16Lowered fft16_decode_lowered(size_t n, const std::vector<uint32_t> &valid,
 hipError_t fft16_device_upload(const Fft16Lowered &L, Fft16DevicePtr *out);
 size_t fft16_device_loads(const Fft16Device &d);
 size_t fft16_device_stores(const Fft16Device &d);
-hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t stream);
+// Shard pointers of one stripe passed in the kernel arguments: call.ptrs is
+// ignored, nload + nstore <= kFftInlinePtrs, stripes == 1.
+constexpr size_t kFftInlinePtrs = 256;
+struct Fft16Ptrs {
+  const uint8_t *p[kFftInlinePtrs];
+};
+hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t stream,
+                        const Fft16Ptrs *inline_ptrs = nullptr);
 
 // One stripe of one program; launch_fft16_jobs runs many in one launch.
 struct Fft16JobSpec {

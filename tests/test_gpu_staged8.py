@@ -84,7 +84,7 @@ def test_automatic_route_and_reports(trs):
     rs = trs.ReedSolomon(k, m)
     assert rs.encode_route(1 << 20) == "gpu-staged-fft"
     assert rs.encode_route((1 << 20) + 1) != "gpu-staged-fft"
-    assert rs.encode_route(4096) != "gpu-staged-fft"
+    assert rs.encode_route(4096) == "gpu-staged-fft"
     assert rs.reference_route(1 << 20) == "fft-staged"
     assert rs.reference_route(31) == "fused-matrix"
     for length in ((1 << 20), (1 << 20) + 1):
@@ -94,14 +94,20 @@ def test_automatic_route_and_reports(trs):
         cols = slice(0, length, 1021)
         want = O.rs8_encode(G, k, np.ascontiguousarray(data[:, cols]))
         assert (to_host(dev[k:])[:, cols] == want).all(), length
-    # the device rule: any length when the program's multiplies are under
-    # k*m/10, up to 1 MiB for k >= 64 otherwise, never below k = 64
+    # the device rule (even lengths, k > 32): k = 64, 128 at any length when
+    # the program's multiplies are under k*m/10, else up to 1 MiB; other k up
+    # to 256 KiB; never for k <= 32
     r64_16 = trs.ReedSolomon(64, 16)
     assert r64_16.encode_route(1 << 20) == "gpu-staged-fft"
     assert r64_16.encode_route(4 << 20) != "gpu-staged-fft"
     assert trs.ReedSolomon(128, 128).encode_route(4 << 20) == "gpu-staged-fft"
+    r100 = trs.ReedSolomon(100, 156)
+    assert r100.encode_route(256 << 10) == "gpu-staged-fft"
+    assert r100.encode_route((256 << 10) + 2) != "gpu-staged-fft"
+    assert trs.ReedSolomon(33, 16).encode_route(16 << 10) == "gpu-staged-fft"
     assert trs.ReedSolomon(8, 8).encode_route(1 << 20) != "gpu-staged-fft"
-    assert trs.ReedSolomon(12, 8).encode_route(1 << 20) != "gpu-staged-fft"
+    assert trs.ReedSolomon(12, 8).encode_route(64 << 10) != "gpu-staged-fft"
+    assert trs.ReedSolomon(31, 19).encode_route(64 << 10) != "gpu-staged-fft"
     assert trs.ReedSolomon(16, 16).reference_route(64) == "fft-generated"
     assert trs.ReedSolomon(12, 8).reference_route(1 << 20) == "fused-matrix"
     assert trs.ReedSolomon(128, 128).reference_route(1 << 20) == "fused-matrix"
