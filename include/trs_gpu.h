@@ -96,14 +96,14 @@ int trs_rs8_encode_sep(const trs_rs8 *rs, const uint8_t *const *data, const size
                        size_t n_parity, void *stream);
 /* Encode through a named route (0 = matrix apply, 1 = bit-sliced constant
  * matrix, 2 = bit-sliced LCH FFT, 3 = the staged FFT program of
- * src/fft.rs:393-592 on the tower interpreter, even lengths only); the bench
- * / test hook that ReedSolomon::encode_forced is in the reference
+ * src/fft.rs:393-592 on the byte-plane LDS interpreter); the bench / test
+ * hook that ReedSolomon::encode_forced is in the reference
  * (src/reedsolomon.rs:365-398).  TRS_ERR_UNSUPPORTED when the shape has no
- * program for that route or route 3 gets an odd length.  Automatic encodes
- * of even length take route 3 when k > 32: for k = 64 or 128 at any length
- * when the program has fewer than k*m/10 multiplies, else up to 1 MiB; for
- * other k up to 256 KiB (measured crossovers against the matrix route; the
- * reference's rule, src/reedsolomon.rs:204-234, is trs_rs8_reference_route). */
+ * program for that route.  Automatic encodes take route 3 when k > 32: at any
+ * length when the program has fewer than 0.13 k*m multiplies (k = 64, 128)
+ * or 0.078 k*m (other k), else up to 1 MiB (k = 64, 128) or 256 KiB
+ * (measured crossovers against the matrix route; the reference's rule,
+ * src/reedsolomon.rs:204-234, is trs_rs8_reference_route). */
 int trs_rs8_encode_sep_route(const trs_rs8 *rs, int route, const uint8_t *const *data,
                              const size_t *data_lens, size_t n_data, uint8_t *const *parity,
                              const size_t *parity_lens, size_t n_parity, void *stream);
@@ -202,14 +202,15 @@ int trs_fft16_lowered_program(size_t k, size_t m, uint32_t *steps, size_t steps_
  * out[0..n): register_count, mult_count, glue ops, stages, zero registers
  * (all of src/fft.rs's lowering), reference route (1 when the reference's
  * x86 build routes the shape to this program, src/reedsolomon.rs:204-234),
- * device registers, device steps, LDS bytes per 32-column group, and the
- * multiply count after the device lowering's liveness pass.
+ * device registers, device steps, LDS bytes per 32-byte column group (8 bit
+ * planes per register), and the multiply count after the device lowering's
+ * liveness pass.
  * TRS_ERR_UNSUPPORTED unless k >= 1, m >= 1, k + m <= 256.  Touches no
  * device. */
 int trs_fft8_program_info(size_t k, size_t m, size_t *out, size_t n);
 /* Host-only dump of its device lowering, as trs_fft16_lowered_program.  The
- * constants are bytes c of the subfield, whose tower Karatsuba words are
- * [c, c, 0]: the interpreter applies them to the low and high planes alike. */
+ * constants are bytes c of the subfield, written as their tower Karatsuba
+ * words [c, c, 0]; the byte-plane interpreter reads the low byte. */
 int trs_fft8_lowered_program(size_t k, size_t m, uint32_t *steps, size_t steps_cap,
                              uint32_t *consts, size_t consts_cap, uint32_t *ops, size_t ops_cap,
                              uint32_t *parity, size_t parity_cap, size_t *counts);

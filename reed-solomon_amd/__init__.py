@@ -321,7 +321,7 @@ class ReedSolomon(_Codec):
         return lib.trs_rs8_reference_route(self._h, shard_len).decode()
 
     # "staged-fft": the GF(2^8) staged FFT program (src/fft.rs:393-592) on the
-    # tower interpreter; even shard lengths only.
+    # byte-plane LDS interpreter.
     ROUTES = {"table": 0, "bitsliced-matrix": 1, "bitsliced-fft": 2, "staged-fft": 3}
 
     def encode_sep_route(self, route: str, data: Sequence, parity: Sequence) -> None:

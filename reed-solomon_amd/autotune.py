@@ -5,7 +5,7 @@ are a device route -- "table" (the matrix route with runtime coefficients:
 the table kernel below 128 KiB strips, the bit-sliced planes kernel above),
 "bitsliced" (the FFT route: the generated bit-sliced LCH program of a GF(2^8)
 shape that has one, else the staged LCH FFT program on the LDS interpreter --
-GF(2^8) at even lengths, src/fft.rs:393-592, or the tower) or "mfma"
+GF(2^8) on byte planes, src/fft.rs:393-592, or the tower) or "mfma"
 (bit-sliced int8 MFMA, GF(2^8), k <= 32) -- times a
 column strip size (one launch per strip, 0 = whole shard), the GPU analogue of
 the reference's hand-tuned STRIP_BYTES (src/reedsolomon16.rs:796-804) and of
@@ -80,7 +80,7 @@ def _tower_fft(s: Shape) -> bool:
 
 
 def _byte_staged(s: Shape) -> bool:
-    return s.field == 8 and (s.k, s.m) not in GENERATED and s.length % 2 == 0
+    return s.field == 8 and (s.k, s.m) not in GENERATED
 
 
 def candidates(s: Shape) -> List[Candidate]:
@@ -118,7 +118,7 @@ def measure(s: Shape, c: Candidate, cap_bytes: int = 32 << 20, reps: int = 2) ->
     route has no kernel for the shape."""
     route, strip = c
     window = max(64, min(s.length, cap_bytes // (s.k + s.m)))
-    window -= window % (2 if s.field == 16 or (route == "bitsliced" and _byte_staged(s)) else 1)
+    window -= window % (2 if s.field == 16 else 1)
     window = max(2, window)
     ms = ctypes.c_double(0.0)
     rc = lib.trs_tune_apply(s.field, s.k, s.m, window, ROUTES.index(route), strip, reps,

@@ -386,7 +386,7 @@ struct PtrTable {
 // `plane` columns.  Up to kFftInlinePtrs pointers travel in the kernel
 // arguments; more go through a device table.
 int launch_fft16_one(const Fft16Device &d, const std::vector<const uint8_t *> &ptrs,
-                     size_t nstore, uint64_t plane, hipStream_t stream) {
+                     size_t nstore, uint64_t plane, hipStream_t stream, bool byte_planes = false) {
   bool al = true;
   for (auto q : ptrs) al = al && aligned16(q);
   Fft16Call c{};
@@ -397,13 +397,13 @@ int launch_fft16_one(const Fft16Device &d, const std::vector<const uint8_t *> &p
   if (ptrs.size() <= kFftInlinePtrs) {
     Fft16Ptrs P{};
     std::copy(ptrs.begin(), ptrs.end(), P.p);
-    return hip_status(launch_fft16(d, c, stream, &P));
+    return hip_status(launch_fft16(d, c, stream, &P, byte_planes));
   }
   PtrTable table;
   int st = table.upload(ptrs, stream);
   if (st) return st;
   c.ptrs = static_cast<const uint8_t *const *>(table.d);
-  return hip_status(launch_fft16(d, c, stream));
+  return hip_status(launch_fft16(d, c, stream, nullptr, byte_planes));
 }
 
 }  // namespace
@@ -423,11 +423,11 @@ struct trs_rs8 {
   DevCoef parity;
   DevBuf parity_frag;  // MFMA A fragments of the parity rows (k <= 32)
   PlanCache plans;
-  // The GF(2^8) staged FFT program (src/fft.rs:393-592) on the tower
-  // interpreter: its constants lie in the subfield, so a shard of even
-  // length L is a split-plane shard of L / 2 columns.  The automatic encode
-  // route takes it for even lengths up to `staged_max` (0: never).  The
-  // device form is built and uploaded on first use.
+  // The GF(2^8) staged FFT program (src/fft.rs:393-592) on the LDS
+  // interpreter with byte-plane registers (its constants all lie in the
+  // subfield).  The automatic encode route takes it for lengths up to
+  // `staged_max` (0: never).  The device form is built and uploaded on first
+  // use.
   size_t staged_max = 0;
   mutable std::once_flag staged_once;
   mutable hipError_t staged_status = hipSuccess;
@@ -724,19 +724,19 @@ int trs_fill_random(uint8_t *dst, size_t len, uint64_t seed, void *stream) {
 // ---------------------------------------------------------------- RS8
 
 namespace {
-// Longest even shard length at which the automatic encode of (k, m) runs the
+// Longest shard length at which the automatic encode of (k, m) runs the
 // staged FFT program (0: never).  The reference takes power-of-two k whose
 // program has fewer than k*m/2 multiplies and fits 160 registers
 // (src/reedsolomon.rs:204-234); its register limit does not apply here (the
 // file lives in LDS).  Measured on this GPU at full size against the
 // automatic matrix route (51 shapes, 16 KiB - 4 MiB): for k > 32 the matrix
 // kernels leave most of the chip idle up to a few hundred KiB (the MFMA route
-// covers k <= 32) and the program is 1.1-4x faster through 256 KiB; beyond,
-// the interpreter's cost per multiply (about 1.2 us per 4 MiB for
-// power-of-two k, two to four times that for other k) decides, and only
-// k = 64, 128 stay ahead up to 1 MiB, and at any length when the program
-// has fewer than k*m/10 multiplies.  Generated shapes keep their bit-sliced
-// programs.  Env TRS_STAGED8=0 turns the route off.
+// covers k <= 32) and the program is 1.1-4x faster through 256 KiB.  Beyond,
+// the ratio of program multiplies to k*m decides: the program stays ahead at
+// any length below 0.13 for k = 64, 128 (few, wide steps) and below 0.078
+// for other k (more glue, more steps); k = 64, 128 above that stay ahead up
+// to 1 MiB.  Generated shapes keep their bit-sliced programs.  Env
+// TRS_STAGED8=0 turns the route off.
 size_t staged8_gpu_max(size_t k, size_t m) {
   static const bool off = [] {
     const char *e = std::getenv("TRS_STAGED8");
@@ -745,14 +745,15 @@ size_t staged8_gpu_max(size_t k, size_t m) {
   if (off || k <= 32 || k + m > 256) return 0;
   const Fft16Program p = build_staged_program8(k, m);
   if (!fft16_fits_device(p)) return 0;
-  if ((k & (k - 1)) == 0)
-    return p.mult_count * 10 < k * m ? SIZE_MAX : size_t(1) << 20;
-  return size_t(256) << 10;
+  const bool pow2 = (k & (k - 1)) == 0;
+  if ((pow2 && p.mult_count * 100 < 13 * k * m) || p.mult_count * 1000 < 78 * k * m)
+    return SIZE_MAX;
+  return pow2 ? size_t(1) << 20 : size_t(256) << 10;
 }
 
 // The encode route of an automatic call at this length.
 int auto_route8(const trs_rs8 *rs, size_t len) {
-  if (rs->route == kRouteTable && len % 2 == 0 && len <= rs->staged_max) return kRouteStaged8;
+  if (rs->route == kRouteTable && len <= rs->staged_max) return kRouteStaged8;
   return rs->route;
 }
 
@@ -820,7 +821,6 @@ int trs_rs8_generator(const trs_rs8 *rs, uint8_t *out, size_t out_len) {
 namespace {
 int run_staged8(const trs_rs8 *rs, const std::vector<const uint8_t *> &ptrs, size_t len,
                 hipStream_t stream) {
-  if (len % 2) return TRS_ERR_UNSUPPORTED;
   std::call_once(rs->staged_once, [rs] {
     rs->staged_status = fft16_device_new(build_staged_program8(rs->k, rs->m), &rs->staged_dev);
   });
@@ -829,7 +829,7 @@ int run_staged8(const trs_rs8 *rs, const std::vector<const uint8_t *> &ptrs, siz
   const Fft16Device &d = *rs->staged_dev;
   if (fft16_device_loads(d) + fft16_device_stores(d) != ptrs.size()) return TRS_ERR_UNSUPPORTED;
   if (len == 0) return TRS_OK;
-  return launch_fft16_one(d, ptrs, fft16_device_stores(d), len / 2, stream);
+  return launch_fft16_one(d, ptrs, fft16_device_stores(d), len, stream, true);
 }
 
 // check_piece_count_all + check_equal_lengths (src/reedsolomon.rs:701-722).
@@ -1261,7 +1261,7 @@ int trs_fft8_program_info(size_t k, size_t m, size_t *out, size_t n) {
   const size_t v[10] = {r.register_count, r.mult_count, r.glue.size(),
                         r.stages.size(), r.zero_regs.size(),
                         staged8_reference_route(k, m) ? 1u : 0u,
-                        L.regs, L.steps.size(), L.regs * 64, p.mult_count};
+                        L.regs, L.steps.size(), L.regs * 32, p.mult_count};
   for (size_t i = 0; i < n && i < 10; ++i) out[i] = v[i];
   return TRS_OK;
 }
@@ -1856,17 +1856,14 @@ extern "C" int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len
   if (field == 8 && k + nout > 256) return TRS_ERR_INVALID_ARGUMENT;
   int bs_route = 0;
   // FFT candidate on the interpreter: the tower program, or a GF(2^8) shape's
-  // staged program (src/fft.rs:393-592) over even lengths and strips when it
-  // has no bit-sliced program
+  // staged program (src/fft.rs:393-592) over byte planes when it has no
+  // bit-sliced program
   bool interp = field == 16 && route == 1;
   if (field == 8 && route == 1) {
     bs_route = bs_encode_available(kRouteBsFft, k, nout)      ? kRouteBsFft
                : bs_encode_available(kRouteBsMatrix, k, nout) ? kRouteBsMatrix
                                                                 : 0;
-    if (!bs_route) {
-      if ((len & 1) || (strip & 1)) return TRS_ERR_UNSUPPORTED;
-      interp = true;
-    }
+    interp = bs_route == 0;
   } else if (field == 8 && route == 2) {
     if (!mfma_available(k)) return TRS_ERR_UNSUPPORTED;
   } else if (field == 16 && route == 1) {
@@ -1919,33 +1916,33 @@ extern "C" int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len
   if (st) return st;
   uint8_t *base = static_cast<uint8_t *>(T.data.p);
   const uint64_t sw = (strip == 0 || strip >= len) ? len : strip;
-  // the interpreter and the tower kernel address split planes of len / 2
-  // columns (a GF(2^8) shard on the interpreter pairs byte c with L/2 + c)
-  const bool planes = field == 16 || interp;
+  // tower kernels address split planes of len / 2 columns; GF(2^8) kernels
+  // (the byte-plane interpreter included) one column per byte
+  const bool halves = field == 16;
+  const uint64_t ncols = halves ? len / 2 : len, scols = halves ? sw / 2 : sw;
   PtrTable table16;
-  if (planes) {
+  if (halves || interp) {
     // tower matrix: one table; interpreter: one table per column strip
     // (pointers at the strip's column offset in the low plane)
     std::vector<const uint8_t *> p16;
-    const uint64_t step = interp ? sw / 2 : len / 2;
-    for (uint64_t off = 0; off < len / 2; off += step)
+    const uint64_t step = interp ? scols : ncols;
+    for (uint64_t off = 0; off < ncols; off += step)
       for (uint32_t i = 0; i < k + nout; ++i) p16.push_back(base + uint64_t(i) * len + off);
     if ((st = table16.upload(p16, T.stream))) return st;
   }
   auto run_once = [&]() -> int {
-    for (uint64_t off = 0; off < (planes ? len / 2 : len); off += (planes ? sw / 2 : sw)) {
-      const uint64_t span =
-          std::min<uint64_t>(planes ? sw / 2 : sw, (planes ? len / 2 : len) - off);
+    for (uint64_t off = 0; off < ncols; off += scols) {
+      const uint64_t span = std::min<uint64_t>(scols, ncols - off);
       hipError_t e = hipSuccess;
       if (interp) {
         Fft16Call c{};
-        c.ptrs = static_cast<const uint8_t *const *>(table16.d) + (off / (sw / 2)) * (k + nout);
-        c.plane = len / 2;
+        c.ptrs = static_cast<const uint8_t *const *>(table16.d) + (off / scols) * (k + nout);
+        c.plane = ncols;
         c.cols = span;
         c.stripes = 1;
         c.nstore = nout;
-        c.vec_ok = ((len / 2) % 16 == 0 && off % 16 == 0) ? 1u : 0u;
-        e = launch_fft16(*prog16, c, T.stream);
+        c.vec_ok = ((!halves || ncols % 16 == 0) && len % 16 == 0 && off % 16 == 0) ? 1u : 0u;
+        e = launch_fft16(*prog16, c, T.stream, nullptr, !halves);
       } else if (field == 16) {
         Gf16Call c{};
         c.ptrs = static_cast<const uint8_t *const *>(table16.d);

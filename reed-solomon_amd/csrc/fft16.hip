@@ -504,14 +504,20 @@ __device__ __forceinline__ void tower_mul_acc(uint32_t cw, const uint32_t (&x)[1
   if (a2) gf8_mul_acc(a2, x + 8, out);
 }
 
-// Register slot x (= register * G + group) holds 4 quads of 16 bytes; quad q
-// sits at position (q + x/2) mod 4 so that lanes reading consecutive slots
-// spread over all banks.
-__device__ __forceinline__ void rf_load(const uint4 *rf, uint32_t x, uint32_t (&v)[16]) {
-  const uint32_t sw = x >> 1;
+// Register slot x (= register * G + group) holds P/4 quads of 16 bytes (P
+// planes: 16 for tower symbols, 8 for bytes).  Tower: quad q sits at position
+// (q + x/2) mod 4; bytes: (q + x/4) mod 2; either way lanes reading
+// consecutive slots spread over all banks.
+template <int P>
+__device__ __forceinline__ uint32_t rf_quad(uint32_t x, int q) {
+  return P == 16 ? x * 4 + ((q + (x >> 1)) & 3) : x * 2 + ((q + (x >> 2)) & 1);
+}
+
+template <int P>
+__device__ __forceinline__ void rf_load(const uint4 *rf, uint32_t x, uint32_t (&v)[P]) {
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const uint4 t = rf[x * 4 + ((q + sw) & 3)];
+  for (int q = 0; q < P / 4; ++q) {
+    const uint4 t = rf[rf_quad<P>(x, q)];
     v[4 * q] = t.x;
     v[4 * q + 1] = t.y;
     v[4 * q + 2] = t.z;
@@ -519,11 +525,31 @@ __device__ __forceinline__ void rf_load(const uint4 *rf, uint32_t x, uint32_t (&
   }
 }
 
-__device__ __forceinline__ void rf_store(uint4 *rf, uint32_t x, const uint32_t (&v)[16]) {
-  const uint32_t sw = x >> 1;
+template <int P>
+__device__ __forceinline__ void rf_store(uint4 *rf, uint32_t x, const uint32_t (&v)[P]) {
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
-    rf[x * 4 + ((q + sw) & 3)] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+  for (int q = 0; q < P / 4; ++q)
+    rf[rf_quad<P>(x, q)] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+}
+
+// out ^= c * x for a per-lane constant word: the tower's Karatsuba word
+// (P = 16) or a byte (P = 8, the word's low byte).
+template <int P>
+__device__ __forceinline__ void field_mul_acc(uint32_t cw, const uint32_t (&x)[P],
+                                              uint32_t (&out)[P]) {
+  if constexpr (P == 16)
+    tower_mul_acc(cw, x, out);
+  else
+    gf8_mul_acc(cw & 0xffu, x, out);
+}
+
+template <int P>
+__device__ __forceinline__ void field_mul_acc_uniform(uint32_t cw, const uint32_t (&x)[P],
+                                                      uint32_t (&out)[P]) {
+  if constexpr (P == 16)
+    tower_mul_acc_uniform(cw, x, out);
+  else
+    gf8_mul_acc_uniform(cw & 0xffu, x, out);
 }
 
 struct Fft16Args {
@@ -548,26 +574,28 @@ __device__ __forceinline__ uint32_t xcd_order(uint32_t wg, uint32_t total) {
   return (total & 7u) == 0 ? (wg & 7u) * (total >> 3) + (wg >> 3) : wg;
 }
 
-template <int T>
+template <int T, int P>
 __device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
                                             const uint8_t *const *ptr, uint4 *rf);
 
-template <int T>
+// PL = planes per register: 16 (tower symbols of split-plane shards) or 8
+// (bytes of GF(2^8) shards: a program whose constants are all bytes).
+template <int T, int PL>
 __global__ __launch_bounds__(T) void k_fft16(Fft16Args a) {
   extern __shared__ uint4 rf[];
   const uint32_t wg = xcd_order(blockIdx.x, a.total);
   const uint32_t stripe = wg / a.cblocks;
   const uint32_t g0 = (wg - stripe * a.cblocks) << a.gshift;
-  run_program<T>(a, g0, a.ptrs + uint64_t(stripe) * (a.nload + a.nstore), rf);
+  run_program<T, PL>(a, g0, a.ptrs + uint64_t(stripe) * (a.nload + a.nstore), rf);
 }
 
 // One stripe whose shard pointers travel in the kernel arguments (no device
 // table to allocate and copy per call).
-template <int T>
+template <int T, int PL>
 __global__ __launch_bounds__(T) void k_fft16_inline(Fft16Args a, Fft16Ptrs P) {
   extern __shared__ uint4 rf[];
   const uint32_t wg = xcd_order(blockIdx.x, a.total);
-  run_program<T>(a, wg << a.gshift, P.p, rf);
+  run_program<T, PL>(a, wg << a.gshift, P.p, rf);
 }
 
 // Many programs (one per job, e.g. one decode pattern per stripe) in one
@@ -578,36 +606,38 @@ __global__ __launch_bounds__(T) void k_fft16_jobs(const Fft16Args *jobs, const u
   extern __shared__ uint4 rf[];
   const uint32_t wg = xcd_order(blockIdx.x, total);
   const Fft16Args a = ld_const_val(jobs + ld_const(wg_job + wg));  // scalar loads
-  run_program<T>(a, (wg - a.total) << a.gshift, a.ptrs, rf);
+  run_program<T, 16>(a, (wg - a.total) << a.gshift, a.ptrs, rf);
 }
 
-template <int T>
+template <int T, int P>
 __device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
                                             const uint8_t *const *ptr, uint4 *rf) {
   const uint32_t G = 1u << a.gshift;
 
   for (uint32_t q = threadIdx.x; q < (a.nload << a.gshift); q += T) {
     const uint32_t slot = q >> a.gshift, g = q & (G - 1);
-    uint32_t v[16];
+    uint32_t v[P];
     const uint32_t cg = g0 + g;
     if (cg < a.groups) {
       const uint64_t col = uint64_t(cg) * kGroupCols;
       const uint64_t rem = a.cols - col;
       const bool full = a.vec_ok && rem >= kGroupCols;
       const uint8_t *s = ptr[slot];
-      uint32_t lo[8], hi[8];
+      uint32_t lo[8];
       load_planes(s + col, full, rem, lo);
-      load_planes(s + a.plane + col, full, rem, hi);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        v[j] = lo[j];
-        v[8 + j] = hi[j];
+      for (int j = 0; j < 8; ++j) v[j] = lo[j];
+      if constexpr (P == 16) {
+        uint32_t hi[8];
+        load_planes(s + a.plane + col, full, rem, hi);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[8 + j] = hi[j];
       }
     } else {
 #pragma unroll
-      for (int j = 0; j < 16; ++j) v[j] = 0;
+      for (int j = 0; j < P; ++j) v[j] = 0;
     }
-    rf_store(rf, (a.load[slot] << a.gshift) + g, v);
+    rf_store<P>(rf, (a.load[slot] << a.gshift) + g, v);
   }
   __syncthreads();
 
@@ -626,31 +656,31 @@ __device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
         const uint32_t lo = st.a + (block << (sl + 1)) + (o & smask);
         const uint32_t hi = lo + (1u << sl);
         const uint32_t xl = (lo << a.gshift) + g, xh = (hi << a.gshift) + g;
-        uint32_t L[16], H[16];
-        rf_load(rf, xl, L);
-        rf_load(rf, xh, H);
+        uint32_t L[P], H[P];
+        rf_load<P>(rf, xl, L);
+        rf_load<P>(rf, xh, H);
         if (st.d) {
 #pragma unroll
-          for (int j = 0; j < 16; ++j) H[j] ^= L[j];
+          for (int j = 0; j < P; ++j) H[j] ^= L[j];
         }
         if (cw & kConstOne) {
 #pragma unroll
-          for (int j = 0; j < 16; ++j) L[j] ^= H[j];
+          for (int j = 0; j < P; ++j) L[j] ^= H[j];
         } else if (!(cw & kConstZero)) {
           // One block spans the wave at strides >= 64 (one column group per
           // workgroup): then the constant sits in a scalar register.
           const uint32_t cw0 = __builtin_amdgcn_readfirstlane(cw);
           if (__builtin_amdgcn_ballot_w64(cw != cw0) == 0)
-            tower_mul_acc_uniform(cw0, H, L);
+            field_mul_acc_uniform<P>(cw0, H, L);
           else
-            tower_mul_acc(cw, H, L);
+            field_mul_acc<P>(cw, H, L);
         }
         if (!st.d) {
 #pragma unroll
-          for (int j = 0; j < 16; ++j) H[j] ^= L[j];
+          for (int j = 0; j < P; ++j) H[j] ^= L[j];
         }
-        rf_store(rf, xl, L);
-        rf_store(rf, xh, H);
+        rf_store<P>(rf, xl, L);
+        rf_store<P>(rf, xh, H);
       }
     } else {
       for (uint32_t q = threadIdx.x; q < n; q += T) {
@@ -659,26 +689,26 @@ __device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
         const uint32_t kind = op.y & 0xffu;
         const uint32_t xd = ((op.x & 0xffffu) << a.gshift) + g;
         const uint32_t xs = ((op.x >> 16) << a.gshift) + g;
-        uint32_t D[16], S[16];
+        uint32_t D[P], S[P];
         if (kind == kOpZero) {
 #pragma unroll
-          for (int j = 0; j < 16; ++j) D[j] = 0;
+          for (int j = 0; j < P; ++j) D[j] = 0;
         } else {
-          rf_load(rf, xs, S);
+          rf_load<P>(rf, xs, S);
           if (kind == kOpXor || kind == kOpMulXor) {
-            rf_load(rf, xd, D);
+            rf_load<P>(rf, xd, D);
           } else {
 #pragma unroll
-            for (int j = 0; j < 16; ++j) D[j] = 0;
+            for (int j = 0; j < P; ++j) D[j] = 0;
           }
           if (kind == kOpXor || kind == kOpCopy) {
 #pragma unroll
-            for (int j = 0; j < 16; ++j) D[j] ^= S[j];
+            for (int j = 0; j < P; ++j) D[j] ^= S[j];
           } else {
-            tower_mul_acc(op.y >> 8, S, D);
+            field_mul_acc<P>(op.y >> 8, S, D);
           }
         }
-        rf_store(rf, xd, D);
+        rf_store<P>(rf, xd, D);
       }
     }
     __syncthreads();
@@ -688,20 +718,22 @@ __device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
     const uint32_t j = q >> a.gshift, g = q & (G - 1);
     const uint32_t cg = g0 + g;
     if (cg >= a.groups) continue;
-    uint32_t v[16];
-    rf_load(rf, (a.store[j] << a.gshift) + g, v);
+    uint32_t v[P];
+    rf_load<P>(rf, (a.store[j] << a.gshift) + g, v);
     const uint64_t col = uint64_t(cg) * kGroupCols;
     const uint64_t rem = a.cols - col;
     const bool full = a.vec_ok && rem >= kGroupCols;
     uint8_t *d = const_cast<uint8_t *>(ptr[a.nload + j]);
-    uint32_t lo[8], hi[8];
+    uint32_t lo[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      lo[i] = v[i];
-      hi[i] = v[8 + i];
-    }
+    for (int i = 0; i < 8; ++i) lo[i] = v[i];
     store_planes(d + col, full, rem, lo);
-    store_planes(d + a.plane + col, full, rem, hi);
+    if constexpr (P == 16) {
+      uint32_t hi[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) hi[i] = v[8 + i];
+      store_planes(d + a.plane + col, full, rem, hi);
+    }
   }
 }
 
@@ -752,9 +784,9 @@ hipError_t fft16_device_new(const Fft16Program &p, Fft16DevicePtr *out) {
 }
 
 namespace {
-uint32_t choose_gshift(const Fft16Device &d, uint64_t groups) {
+uint32_t choose_gshift(const Fft16Device &d, uint64_t groups, uint32_t reg_bytes = kRegBytes) {
   uint32_t gs = 0;
-  while ((1u << (gs + 1)) <= kMaxGroupsPerWg && (d.regs * kRegBytes << (gs + 1)) <= kLdsTarget &&
+  while ((1u << (gs + 1)) <= kMaxGroupsPerWg && (d.regs * reg_bytes << (gs + 1)) <= kLdsTarget &&
          (uint64_t(1) << (gs + 1)) <= groups)
     ++gs;
   return gs;
@@ -770,15 +802,21 @@ hipError_t raise_lds_limit(size_t lds) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess || raised == dev) return e;
-  const void *kernels[9] = {reinterpret_cast<const void *>(&k_fft16<1024>),
-                            reinterpret_cast<const void *>(&k_fft16<512>),
-                            reinterpret_cast<const void *>(&k_fft16<256>),
-                            reinterpret_cast<const void *>(&k_fft16_inline<1024>),
-                            reinterpret_cast<const void *>(&k_fft16_inline<512>),
-                            reinterpret_cast<const void *>(&k_fft16_inline<256>),
-                            reinterpret_cast<const void *>(&k_fft16_jobs<1024>),
-                            reinterpret_cast<const void *>(&k_fft16_jobs<512>),
-                            reinterpret_cast<const void *>(&k_fft16_jobs<256>)};
+  const void *kernels[15] = {reinterpret_cast<const void *>(&k_fft16<1024, 16>),
+                             reinterpret_cast<const void *>(&k_fft16<512, 16>),
+                             reinterpret_cast<const void *>(&k_fft16<256, 16>),
+                             reinterpret_cast<const void *>(&k_fft16_inline<1024, 16>),
+                             reinterpret_cast<const void *>(&k_fft16_inline<512, 16>),
+                             reinterpret_cast<const void *>(&k_fft16_inline<256, 16>),
+                             reinterpret_cast<const void *>(&k_fft16<1024, 8>),
+                             reinterpret_cast<const void *>(&k_fft16<512, 8>),
+                             reinterpret_cast<const void *>(&k_fft16<256, 8>),
+                             reinterpret_cast<const void *>(&k_fft16_inline<1024, 8>),
+                             reinterpret_cast<const void *>(&k_fft16_inline<512, 8>),
+                             reinterpret_cast<const void *>(&k_fft16_inline<256, 8>),
+                             reinterpret_cast<const void *>(&k_fft16_jobs<1024>),
+                             reinterpret_cast<const void *>(&k_fft16_jobs<512>),
+                             reinterpret_cast<const void *>(&k_fft16_jobs<256>)};
   for (const void *k : kernels)
     if ((e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(kLdsMax))) !=
         hipSuccess)
@@ -796,14 +834,38 @@ uint32_t choose_threads(size_t lds, uint64_t widest) {
 }
 }  // namespace
 
+namespace {
+template <int PL>
+void launch_planes(uint32_t threads, const Fft16Args &a, const Fft16Ptrs *inline_ptrs, size_t lds,
+                   hipStream_t stream) {
+  const dim3 grid(a.total);
+  if (inline_ptrs) {
+    const Fft16Ptrs &P = *inline_ptrs;
+    if (threads == 1024)
+      hipLaunchKernelGGL((k_fft16_inline<1024, PL>), grid, dim3(1024), lds, stream, a, P);
+    else if (threads == 512)
+      hipLaunchKernelGGL((k_fft16_inline<512, PL>), grid, dim3(512), lds, stream, a, P);
+    else
+      hipLaunchKernelGGL((k_fft16_inline<256, PL>), grid, dim3(256), lds, stream, a, P);
+  } else if (threads == 1024) {
+    hipLaunchKernelGGL((k_fft16<1024, PL>), grid, dim3(1024), lds, stream, a);
+  } else if (threads == 512) {
+    hipLaunchKernelGGL((k_fft16<512, PL>), grid, dim3(512), lds, stream, a);
+  } else {
+    hipLaunchKernelGGL((k_fft16<256, PL>), grid, dim3(256), lds, stream, a);
+  }
+}
+}  // namespace
+
 hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t stream,
-                        const Fft16Ptrs *inline_ptrs) {
+                        const Fft16Ptrs *inline_ptrs, bool byte_planes) {
   if (call.plane == 0 || call.stripes == 0) return hipSuccess;
   if (inline_ptrs && (call.stripes != 1 || d.nload + call.nstore > kFftInlinePtrs))
     return hipErrorInvalidValue;
   const uint64_t cols = call.cols ? call.cols : call.plane;
   const uint64_t groups = (cols + kGroupCols - 1) / kGroupCols;
-  const uint32_t gs = choose_gshift(d, groups);
+  const uint32_t reg_bytes = byte_planes ? kRegBytes / 2 : kRegBytes;
+  const uint32_t gs = choose_gshift(d, groups, reg_bytes);
   const uint64_t cblocks = (groups + (1u << gs) - 1) >> gs;
   const uint64_t total = cblocks * call.stripes;
   if (groups > 0xffffffffu || total > 0x7fffffffu) return hipErrorInvalidValue;
@@ -824,26 +886,16 @@ hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t
   a.groups = uint32_t(groups);
   a.cblocks = uint32_t(cblocks);
   a.total = uint32_t(total);
-  a.vec_ok = (call.vec_ok && call.plane % 16 == 0) ? 1u : 0u;  // pointers carry any column offset
-  const size_t lds = (d.regs * kRegBytes) << gs;
+  // pointers carry any column offset; the high plane sits `plane` bytes on
+  a.vec_ok = (call.vec_ok && (byte_planes || call.plane % 16 == 0)) ? 1u : 0u;
+  const size_t lds = (d.regs * reg_bytes) << gs;
   const uint32_t threads = choose_threads(lds, uint64_t(d.max_count) << gs);
   hipError_t e = raise_lds_limit(lds);
   if (e != hipSuccess) return e;
-  if (inline_ptrs) {
-    const Fft16Ptrs &P = *inline_ptrs;
-    if (threads == 1024)
-      hipLaunchKernelGGL(k_fft16_inline<1024>, dim3(uint32_t(total)), dim3(1024), lds, stream, a, P);
-    else if (threads == 512)
-      hipLaunchKernelGGL(k_fft16_inline<512>, dim3(uint32_t(total)), dim3(512), lds, stream, a, P);
-    else
-      hipLaunchKernelGGL(k_fft16_inline<256>, dim3(uint32_t(total)), dim3(256), lds, stream, a, P);
-  } else if (threads == 1024) {
-    hipLaunchKernelGGL(k_fft16<1024>, dim3(uint32_t(total)), dim3(1024), lds, stream, a);
-  } else if (threads == 512) {
-    hipLaunchKernelGGL(k_fft16<512>, dim3(uint32_t(total)), dim3(512), lds, stream, a);
-  } else {
-    hipLaunchKernelGGL(k_fft16<256>, dim3(uint32_t(total)), dim3(256), lds, stream, a);
-  }
+  if (byte_planes)
+    launch_planes<8>(threads, a, inline_ptrs, lds, stream);
+  else
+    launch_planes<16>(threads, a, inline_ptrs, lds, stream);
   return hipGetLastError();
 }
 

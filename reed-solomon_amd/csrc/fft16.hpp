@@ -144,8 +144,13 @@ constexpr size_t kFftInlinePtrs = 256;
 struct Fft16Ptrs {
   const uint8_t *p[kFftInlinePtrs];
 };
+// byte_planes: the program's constants are all bytes (a GF(2^8) program,
+// build_staged_program8) and the shards are plain byte shards: one byte per
+// column, call.plane = shard length, registers of 8 bit planes.  Otherwise
+// tower symbols of split-plane shards (16 planes, high plane `plane` bytes
+// after the low).
 hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t stream,
-                        const Fft16Ptrs *inline_ptrs = nullptr);
+                        const Fft16Ptrs *inline_ptrs = nullptr, bool byte_planes = false);
 
 // One stripe of one program; launch_fft16_jobs runs many in one launch.
 struct Fft16JobSpec {

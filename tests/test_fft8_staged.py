@@ -3,11 +3,11 @@
 * The library's C++ builder, lowered for the device interpreter, replays to
   the generator-matrix parity of the oracle on the reference's 21 program
   shapes (src/fft.rs:979-1005, test program_matches_matrix :1008-1041) and
-  on the staged-route shapes, from a register file of stale values.  The
-  constants are bytes of the subfield, so a byte shard of even length L runs
-  on the tower interpreter as a split-plane shard of L / 2 columns: the
-  replay packs byte i and byte L/2 + i into one symbol, and the parity must
-  come out byte-identical on both halves.
+  on the staged-route shapes, from a register file of stale values.  Every
+  constant is a byte of the subfield (Karatsuba word [c, c, 0]), which is
+  what lets the byte-plane kernel use the word's low byte alone: the replay
+  runs the tower arithmetic on byte inputs and the high byte of every output
+  must stay zero.
 * The multiply and register counts of src/fft.rs's lowering are pinned by
   their closed form for k = m = 2^j: W_l vanishes exactly on [0, 2^l), so the
   base transforms at beta 0 skip one block per level and the chunk at beta k
@@ -42,13 +42,12 @@ def _lowered8(trs, k, m):
 
 
 def _byte_parity_by_replay(trs, k, m, data):
-    """Runs the lowered byte program over (k, L) data bytes, L even, as the
-    GPU does: symbol column i = byte i (low plane) | byte L/2 + i (high)."""
+    """Runs the lowered byte program over (k, L) data bytes with the tower
+    arithmetic; a byte constant's product never leaves the low byte."""
     steps, consts, ops, parity, regs = _lowered8(trs, k, m)
-    half = data.shape[1] // 2
-    sym = data[:, :half].astype(np.int64) | (data[:, half:].astype(np.int64) << 8)
-    out = replay(steps, consts, ops, np.arange(k), parity, regs, sym)
-    return np.concatenate([out & 0xFF, out >> 8], axis=1).astype(np.uint8)
+    out = replay(steps, consts, ops, np.arange(k), parity, regs, data.astype(np.int64))
+    assert (out >> 8 == 0).all()
+    return out.astype(np.uint8)
 
 
 @pytest.mark.parametrize("k,m", REFERENCE_SHAPES + EXTRA_SHAPES)
@@ -107,6 +106,6 @@ def test_reference_route_examples(trs):
 
 def test_program_info_group_bytes(trs):
     info = _info(trs, 64, 64)
-    assert info["group_bytes"] == 64 * info["device_regs"]
+    assert info["group_bytes"] == 32 * info["device_regs"]
     assert info["device_regs"] <= info["register_count"]
     assert math.isfinite(info["device_steps"]) and info["device_steps"] > 0

@@ -63,12 +63,11 @@ def test_tune_apply_routes(trs):
     assert A.measure(A.Shape(16, 8, 4100, 4096), ("bitsliced", 0)) is None
     assert A.measure(A.Shape(8, 40, 5, 4096), ("mfma", 0)) is None
     # GF(2^8) shapes without a generated program: the staged FFT program on
-    # the interpreter, even lengths and strips only
+    # the byte-plane interpreter, any length and strip
     s8 = A.Shape(8, 64, 64, 1 << 20)
     assert {c[0] for c in A.candidates(s8)} == {"table", "bitsliced"}
     assert A.measure(s8, ("bitsliced", 65536)) > 0
-    assert A.measure(A.Shape(8, 64, 64, 4097), ("bitsliced", 0)) is None
-    assert {c[0] for c in A.candidates(A.Shape(8, 64, 64, 4097))} == {"table"}
+    assert A.measure(A.Shape(8, 64, 64, 4097), ("bitsliced", 1000)) > 0
     assert A.heuristic(s8) == ("bitsliced", 0)
     assert A.heuristic(A.Shape(8, 12, 8, 1 << 20)) == ("table", 0)
 

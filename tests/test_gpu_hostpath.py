@@ -16,10 +16,12 @@ def _stripe(k, m, length, seed):
     return G, data, O.rs8_encode(G, k, data)
 
 
-@pytest.mark.parametrize("k,m", [(10, 4), (7, 13), (20, 10)])
+# (48, 16): k > 32, so chunks up to 256 KiB take the staged FFT route
+@pytest.mark.parametrize("k,m", [(10, 4), (7, 13), (20, 10), (48, 16)])
 @pytest.mark.parametrize("pinned", [False, True])
 def test_encode_host(trs, k, m, pinned):
-    for length, chunk in ((1, 0), (1000, 0), (50_000, 4096), (300_000, 65536 + 64)):
+    for length, chunk in ((1, 0), (1000, 0), (50_000, 4096), (50_001, 4096),
+                          (300_000, 65536 + 64)):
         G, data, want = _stripe(k, m, length, 11 + length)
         if pinned:
             d = [torch.from_numpy(data[i].copy()).pin_memory() for i in range(k)]

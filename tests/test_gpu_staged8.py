@@ -1,5 +1,5 @@
-"""GPU parity of the GF(2^8) staged FFT route (src/fft.rs:393-592 on the tower
-interpreter) through the C ABI, against the oracle's generator-matrix encode.
+"""GPU parity of the GF(2^8) staged FFT route (src/fft.rs:393-592 on the
+byte-plane LDS interpreter) through the C ABI, against the oracle's generator-matrix encode.
 
 Mirrors the reference's staged_matches_scalar test (src/reedsolomon.rs:997-1021:
 staged and runtime shapes at ragged lengths) with the forced route, adds
@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 # src/reedsolomon.rs:999-1006 plus the reference's staged shapes and wide ones
 SHAPES = [(12, 8), (14, 14), (16, 16), (18, 6), (24, 8), (32, 32), (2, 2), (4, 4), (8, 8),
           (64, 64), (128, 32), (1, 4), (37, 21), (100, 156), (200, 55), (128, 128)]
-LENGTHS = [2, 16, 32, 34, 100, 1000, 4096, 4098, 65536 + 64]
+LENGTHS = [1, 2, 3, 16, 31, 32, 33, 34, 100, 1000, 1001, 4096, 4098, 4099, 65536 + 64]
 
 
 def _forced(trs, rs, k, m, data):
@@ -70,12 +70,19 @@ def test_staged_route_unaligned_pointers(trs):
     assert (to_host(views[k:]) == O.rs8_encode(G, k, data)).all()
 
 
-def test_staged_route_rejects_odd_length(trs):
-    k, m = 8, 8
+def test_staged_route_unaligned_odd_length(trs):
+    import torch
+
+    k, m, length = 40, 24, 12345
+    G = O.build_matrix(8, k, k + m)
     rs = trs.ReedSolomon(k, m)
-    data = seeded(k, 101, 5)
-    with pytest.raises(trs.Error):
-        _forced(trs, rs, k, m, data)
+    data = seeded(k, length, 0xB11)
+    big = torch.zeros((k + m) * (length + 16), dtype=torch.uint8, device="cuda")
+    views = [big[i * (length + 16) + 5: i * (length + 16) + 5 + length] for i in range(k + m)]
+    for i in range(k):
+        views[i].copy_(torch.from_numpy(data[i]))
+    rs.encode_sep_route("staged-fft", views[:k], views[k:])
+    assert (to_host(views[k:]) == O.rs8_encode(G, k, data)).all()
 
 
 def test_automatic_route_and_reports(trs):
@@ -83,7 +90,7 @@ def test_automatic_route_and_reports(trs):
     G = O.build_matrix(8, k, k + m)
     rs = trs.ReedSolomon(k, m)
     assert rs.encode_route(1 << 20) == "gpu-staged-fft"
-    assert rs.encode_route((1 << 20) + 1) != "gpu-staged-fft"
+    assert rs.encode_route((1 << 20) + 1) == "gpu-staged-fft"
     assert rs.encode_route(4096) == "gpu-staged-fft"
     assert rs.reference_route(1 << 20) == "fft-staged"
     assert rs.reference_route(31) == "fused-matrix"
@@ -94,16 +101,18 @@ def test_automatic_route_and_reports(trs):
         cols = slice(0, length, 1021)
         want = O.rs8_encode(G, k, np.ascontiguousarray(data[:, cols]))
         assert (to_host(dev[k:])[:, cols] == want).all(), length
-    # the device rule (even lengths, k > 32): k = 64, 128 at any length when
-    # the program's multiplies are under k*m/10, else up to 1 MiB; other k up
-    # to 256 KiB; never for k <= 32
+    # the device rule (k > 32): any length when the program's multiplies are
+    # under 0.13 k*m (k = 64, 128) or 0.078 k*m (other k); else up to 1 MiB
+    # (k = 64, 128) or 256 KiB; never for k <= 32
     r64_16 = trs.ReedSolomon(64, 16)
     assert r64_16.encode_route(1 << 20) == "gpu-staged-fft"
     assert r64_16.encode_route(4 << 20) != "gpu-staged-fft"
     assert trs.ReedSolomon(128, 128).encode_route(4 << 20) == "gpu-staged-fft"
-    r100 = trs.ReedSolomon(100, 156)
-    assert r100.encode_route(256 << 10) == "gpu-staged-fft"
-    assert r100.encode_route((256 << 10) + 2) != "gpu-staged-fft"
+    assert trs.ReedSolomon(100, 156).encode_route(4 << 20) == "gpu-staged-fft"
+    r200 = trs.ReedSolomon(200, 55)
+    assert r200.encode_route(256 << 10) == "gpu-staged-fft"
+    assert r200.encode_route((256 << 10) + 1) != "gpu-staged-fft"
+    assert r200.encode_route((256 << 10) - 1) == "gpu-staged-fft"
     assert trs.ReedSolomon(33, 16).encode_route(16 << 10) == "gpu-staged-fft"
     assert trs.ReedSolomon(8, 8).encode_route(1 << 20) != "gpu-staged-fft"
     assert trs.ReedSolomon(12, 8).encode_route(64 << 10) != "gpu-staged-fft"
