@@ -134,6 +134,14 @@ int trs_rs8_reconstruct_rows(const trs_rs8 *rs, uint8_t *rows, size_t rows_len, 
  * kernel.  Every route is byte-identical; no
  * reference counterpart (the reference has one CPU kernel per ISA). */
 int trs_rs8_set_apply_route(trs_rs8 *rs, int route);
+/* Reconstruct route of this codec (single calls and decoders prepared after
+ * the call): 0 = automatic, 1 = matrix plans (the reference's decode,
+ * src/reedsolomon.rs:829-891), 2 = the additive-FFT erasure decoder over
+ * GF(2^8) on the byte-plane interpreter (formal-derivative method; no
+ * reference counterpart).  Automatic takes the decoder for k > 32 up to
+ * 64 KiB shards, and up to 256 KiB from 4 rebuilt shards (measured
+ * crossovers), else the matrix plans.  Byte-identical either way. */
+int trs_rs8_set_decode_route(trs_rs8 *rs, int route);
 /* ReedSolomon::prepare_decode, src/reedsolomon.rs:484-507. */
 int trs_rs8_prepare_decode(const trs_rs8 *rs, const uint8_t *present, size_t n_present,
                            trs_dec8 **out);
@@ -240,6 +248,13 @@ int trs_fft16_decode_program(size_t k, size_t n, const uint8_t *present, uint32_
                              size_t steps_cap, uint32_t *consts, size_t consts_cap, uint32_t *ops,
                              size_t ops_cap, uint32_t *load, size_t load_cap, uint32_t *store,
                              size_t store_cap, size_t *counts);
+/* The same for the GF(2^8) codec's FFT erasure decoder (byte field, n <= 256;
+ * constants lowered as the Karatsuba words [c, c, 0] of subfield bytes).
+ * TRS_ERR_UNSUPPORTED past 256 shards. */
+int trs_fft8_decode_program(size_t k, size_t n, const uint8_t *present, uint32_t *steps,
+                            size_t steps_cap, uint32_t *consts, size_t consts_cap, uint32_t *ops,
+                            size_t ops_cap, uint32_t *load, size_t load_cap, uint32_t *store,
+                            size_t store_cap, size_t *counts);
 /* Reconstruct route of the tower codec: 0 auto, 1 matrix (the reference's
  * decode plans, src/reedsolomon16.rs:595-640), 2 additive-FFT erasure decoder
  * (no reference counterpart; byte-identical, since the erased symbols are

@@ -334,6 +334,12 @@ class ReedSolomon(_Codec):
                                             pl, len(parity), _stream()))
 
     APPLY_ROUTES = {"auto": 0, "table": 1, "mfma": 2}
+    DECODE_ROUTES = ("auto", "matrix", "fft")
+
+    def set_decode_route(self, route: str) -> None:
+        """Reconstruct route: "matrix" (the reference's decode plans), "fft"
+        (the additive-FFT erasure decoder over GF(2^8)) or "auto"."""
+        _check(lib.trs_rs8_set_decode_route(self._h, self.DECODE_ROUTES.index(route)))
 
     def set_apply_route(self, route: str) -> None:
         """Device route for matrix applies (reconstruct; encode of shapes

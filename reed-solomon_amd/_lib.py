@@ -71,6 +71,7 @@ for fam, elem in (("rs8", c_u8p), ("rs16", c_u16p)):
 
 _sig("trs_rs8_encode_route", ctypes.c_char_p, c_vp, c_size)
 _sig("trs_rs8_reference_route", ctypes.c_char_p, c_vp, c_size)
+_sig("trs_rs8_set_decode_route", c_int, c_vp, c_int)
 _sig("trs_rs16_encode_route", ctypes.c_char_p, c_vp)
 _sig("trs_rs8_encode_sep_route", c_int, c_vp, c_int, c_ptrs, c_sizep, c_size, c_ptrs, c_sizep,
      c_size, c_vp)
@@ -92,6 +93,8 @@ _sig("trs_fft16_lowered_program", c_int, c_size, c_size, c_u32p, c_size, c_u32p,
 _sig("trs_fft8_program_info", c_int, c_size, c_size, ctypes.POINTER(c_size), c_size)
 _sig("trs_fft8_lowered_program", c_int, c_size, c_size, c_u32p, c_size, c_u32p, c_size, c_u32p,
      c_size, c_u32p, c_size, ctypes.POINTER(c_size))
+_sig("trs_fft8_decode_program", c_int, c_size, c_size, c_u8p, c_u32p, c_size, c_u32p, c_size,
+     c_u32p, c_size, c_u32p, c_size, c_u32p, c_size, ctypes.POINTER(c_size))
 _sig("trs_fft16_decode_program", c_int, c_size, c_size, c_u8p, c_u32p, c_size, c_u32p, c_size,
      c_u32p, c_size, c_u32p, c_size, c_u32p, c_size, ctypes.POINTER(c_size))
 for fam in ("dec8", "dec16"):

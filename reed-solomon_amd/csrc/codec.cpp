@@ -235,11 +235,28 @@ bool use_fft_decode(int route, size_t n, size_t k, size_t nout) {
   return k * nout >= (size_t(1) << lg) * lg;
 }
 
-int plan_fft16(Plan &p, size_t n, const Fft16Device **out) {
-  std::call_once(p.fft_once, [&p, n] {
+// The plan's FFT erasure decoder (tower, or byte field for GF(2^8) plans),
+// built on first use.
+// Decode route for a GF(2^8) reconstruct (route as trs_rs8_set_decode_route).
+// Automatic takes the FFT decoder where it measured faster than the matrix
+// apply, per call at full size (14 shapes x 1..m erasures x 16 KiB - 4 MiB):
+// for k > 32 up to 64 KiB (the matrix kernels leave the chip mostly idle
+// there; 1.4-8x), and up to 256 KiB from 4 outputs; k <= 32 stays on the
+// matrix route (MFMA), and so do longer shards, where the decoder's two
+// N-point transforms cost more than the k * nout products per column.
+bool use_fft_decode8(int route, size_t n, size_t k, size_t nout, size_t len) {
+  if (route == 1 || nout == 0 || n > 256) return false;
+  if (route == 2) return true;
+  return k > 32 && (len <= (size_t(64) << 10) || (len <= (size_t(256) << 10) && nout >= 4));
+}
+
+int plan_fft16(Plan &p, size_t n, const Fft16Device **out, bool byte_field = false) {
+  std::call_once(p.fft_once, [&p, n, byte_field] {
     std::vector<uint32_t> outputs(p.missing_data);
     outputs.insert(outputs.end(), p.missing_parity.begin(), p.missing_parity.end());
-    p.fft_status = fft16_device_upload(fft16_decode_lowered(n, p.valid, outputs), &p.fft);
+    p.fft_status = fft16_device_upload(byte_field ? fft8_decode_lowered(n, p.valid, outputs)
+                                                  : fft16_decode_lowered(n, p.valid, outputs),
+                                       &p.fft);
   });
   if (p.fft_status == hipErrorInvalidValue) return TRS_ERR_UNSUPPORTED;
   if (p.fft_status != hipSuccess) return hip_status(p.fft_status);
@@ -429,6 +446,9 @@ struct trs_rs8 {
   // `staged_max` (0: never).  The device form is built and uploaded on first
   // use.
   size_t staged_max = 0;
+  // reconstruct: 0 auto, 1 matrix plans, 2 the FFT erasure decoder on the
+  // byte-plane interpreter
+  std::atomic<int> decode_route{0};
   mutable std::once_flag staged_once;
   mutable hipError_t staged_status = hipSuccess;
   mutable Fft16DevicePtr staged_dev;
@@ -436,7 +456,8 @@ struct trs_rs8 {
 
 struct trs_dec8 {
   size_t k = 0, n = 0;
-  int apply_route = 0;  // the codec's apply route when the decoder was prepared
+  int apply_route = 0;   // the codec's apply route when the decoder was prepared
+  int decode_route = 0;  // and its reconstruct route
   std::shared_ptr<Plan> plan;
 };
 
@@ -953,7 +974,7 @@ namespace {
 // (src/reedsolomon.rs:615-641, :771-796).  `plan` non-null = prepared.
 int reconstruct8(const trs_rs8 *rs, size_t k, size_t total, std::shared_ptr<Plan> plan,
                  uint8_t *const *shards, const size_t *lens, const uint8_t *present, size_t n,
-                 int data_only, hipStream_t s, int apply_route) {
+                 int data_only, hipStream_t s, int apply_route, int dec_decode_route = 0) {
   if (n && (!shards || !lens || !present)) return TRS_ERR_INVALID_ARGUMENT;
   if (n != total) return n < total ? TRS_TOO_FEW_SHARDS : TRS_TOO_MANY_SHARDS;
   Mask mask;
@@ -972,11 +993,23 @@ int reconstruct8(const trs_rs8 *rs, size_t k, size_t total, std::shared_ptr<Plan
   std::vector<const uint8_t *> ptrs;
   size_t nout = 0;
   if ((st = gather(*plan, shards, lens, k, len, data_only != 0, ptrs, nout))) return st;
+  if (nout && use_fft_decode8(rs ? rs->decode_route.load() : dec_decode_route, total, k, nout,
+                              len)) {
+    const Fft16Device *d = nullptr;
+    if ((st = plan_fft16(*plan, total, &d, true))) return st;
+    return launch_fft16_one(*d, ptrs, nout, len, s, true);
+  }
   if (use_mfma(apply_route, k, len, false) && plan->frag.p)
     return run_mfma(ptrs, k, nout, plan->frag.p, len, s);
   return run_gf8(ptrs, k, nout, plan->coef, len, kWrite, nullptr, s);
 }
 }  // namespace
+
+int trs_rs8_set_decode_route(trs_rs8 *rs, int route) {
+  if (!rs || route < 0 || route > 2) return TRS_ERR_INVALID_ARGUMENT;
+  rs->decode_route.store(route);
+  return TRS_OK;
+}
 
 int trs_rs8_set_apply_route(trs_rs8 *rs, int route) {
   if (!rs || route < 0 || route > 2) return TRS_ERR_INVALID_ARGUMENT;
@@ -1023,6 +1056,7 @@ int trs_rs8_prepare_decode(const trs_rs8 *rs, const uint8_t *present, size_t n, 
   d->k = rs->k;
   d->n = rs->n;
   d->apply_route = rs->apply_route.load();
+  d->decode_route = rs->decode_route.load();
   d->plan = plan;
   *out = d.release();
   return TRS_OK;
@@ -1032,7 +1066,7 @@ int trs_dec8_reconstruct(const trs_dec8 *dec, uint8_t *const *shards, const size
                          const uint8_t *present, size_t n, int data_only, void *stream) {
   if (!dec) return TRS_ERR_INVALID_ARGUMENT;
   return reconstruct8(nullptr, dec->k, dec->n, dec->plan, shards, lens, present, n, data_only,
-                      as_stream(stream), dec->apply_route);
+                      as_stream(stream), dec->apply_route, dec->decode_route);
 }
 
 void trs_dec8_free(trs_dec8 *dec) { delete dec; }
@@ -1301,18 +1335,20 @@ int trs_decode_matrix(int field, size_t k, size_t n, const uint8_t *present, voi
   return TRS_OK;
 }
 
-int trs_fft16_decode_program(size_t k, size_t n, const uint8_t *present, uint32_t *steps,
-                             size_t steps_cap, uint32_t *consts, size_t consts_cap, uint32_t *ops,
-                             size_t ops_cap, uint32_t *load, size_t load_cap, uint32_t *store,
-                             size_t store_cap, size_t *counts) {
+namespace {
+int decode_program_dump(int field, size_t k, size_t n, const uint8_t *present, uint32_t *steps,
+                        size_t steps_cap, uint32_t *consts, size_t consts_cap, uint32_t *ops,
+                        size_t ops_cap, uint32_t *load, size_t load_cap, uint32_t *store,
+                        size_t store_cap, size_t *counts) {
   if (!counts || !present || k == 0 || n <= k) return TRS_ERR_INVALID_ARGUMENT;
-  if (n > kMaxDecodePoints) return TRS_ERR_UNSUPPORTED;
+  if (n > (field == 8 ? 256 : kMaxDecodePoints)) return TRS_ERR_UNSUPPORTED;
   Plan p;
   split_indices(make_mask(present, n), k, n, p);
   if (p.valid.size() < k) return TRS_TOO_FEW_SHARDS_PRESENT;
   std::vector<uint32_t> outputs(p.missing_data);
   outputs.insert(outputs.end(), p.missing_parity.begin(), p.missing_parity.end());
-  const Fft16Lowered L = fft16_decode_lowered(n, p.valid, outputs);
+  const Fft16Lowered L = field == 8 ? fft8_decode_lowered(n, p.valid, outputs)
+                                    : fft16_decode_lowered(n, p.valid, outputs);
   const size_t c[6] = {L.steps.size() * 8, L.consts.size(), L.ops.size(), L.load.size(),
                        L.store.size(), L.regs};
   std::copy(c, c + 6, counts);
@@ -1326,6 +1362,23 @@ int trs_fft16_decode_program(size_t k, size_t n, const uint8_t *present, uint32_
   std::copy(L.load.begin(), L.load.end(), load);
   std::copy(L.store.begin(), L.store.end(), store);
   return TRS_OK;
+}
+}  // namespace
+
+int trs_fft16_decode_program(size_t k, size_t n, const uint8_t *present, uint32_t *steps,
+                             size_t steps_cap, uint32_t *consts, size_t consts_cap, uint32_t *ops,
+                             size_t ops_cap, uint32_t *load, size_t load_cap, uint32_t *store,
+                             size_t store_cap, size_t *counts) {
+  return decode_program_dump(16, k, n, present, steps, steps_cap, consts, consts_cap, ops,
+                             ops_cap, load, load_cap, store, store_cap, counts);
+}
+
+int trs_fft8_decode_program(size_t k, size_t n, const uint8_t *present, uint32_t *steps,
+                            size_t steps_cap, uint32_t *consts, size_t consts_cap, uint32_t *ops,
+                            size_t ops_cap, uint32_t *load, size_t load_cap, uint32_t *store,
+                            size_t store_cap, size_t *counts) {
+  return decode_program_dump(8, k, n, present, steps, steps_cap, consts, consts_cap, ops,
+                             ops_cap, load, load_cap, store, store_cap, counts);
 }
 
 int trs_rs16_reconstruct_stripes(const trs_rs16 *rs, uint8_t *const *shards,

@@ -264,36 +264,55 @@ Fft16Lowered fft16_lower(const Fft16Program &program) {
 
 namespace {
 
+// The field a decoder is built over: the tower (2^16 points) or the byte
+// field (its subfield, 2^8 points; constants < 256 lower to the Karatsuba
+// words [c, c, 0] the byte-plane kernel reads the low byte of).
+struct TowerField {
+  static constexpr uint32_t kOrder = 65535, kLevels = 11;
+  static uint16_t mul(uint16_t a, uint16_t b) { return gf16().mul(a, b); }
+  static uint16_t inv(uint16_t a) { return gf16().inv(a); }
+  static uint32_t log(uint32_t x) { return gf16().log[x]; }
+  static uint16_t exp(uint32_t i) { return gf16().exp[i]; }
+  static uint16_t what(int level, size_t x) { return fft16_what(level, x); }
+};
+
+struct ByteField {
+  static constexpr uint32_t kOrder = 255, kLevels = 8;
+  static uint16_t mul(uint16_t a, uint16_t b) { return gf8().mul[a & 0xff][b & 0xff]; }
+  static uint16_t inv(uint16_t a) { return gf8().inv(uint8_t(a)); }
+  static uint32_t log(uint32_t x) { return gf8().log[x]; }
+  static uint16_t exp(uint32_t i) { return gf8().exp[i]; }
+  static uint16_t what(int level, size_t x) { return fft8_what(level, x); }
+};
+
 // Formal-derivative constants of the normalised basis: D_j is the linear
 // coefficient of W_j / W_j(2^j), i.e. prod_{0<u<2^j} u / prod_{2^j<=u<2^(j+1)} u
 // (the linear coefficient of the subspace polynomial W_j is the product of the
 // nonzero points of its subspace).
+template <class F>
 const std::vector<uint16_t> &derivative_consts() {
   static std::vector<uint16_t> d;
   static std::once_flag once;
   std::call_once(once, [] {
-    const Gf16 &f = gf16();
-    for (int j = 0; j < 11; ++j) {
+    for (uint32_t j = 0; j < F::kLevels; ++j) {
       uint16_t num = 1, den = 1;
-      for (uint32_t u = 1; u < (1u << j); ++u) num = f.mul(num, uint16_t(u));
-      for (uint32_t u = 1u << j; u < (2u << j); ++u) den = f.mul(den, uint16_t(u));
-      d.push_back(f.mul(num, f.inv(den)));
+      for (uint32_t u = 1; u < (1u << j); ++u) num = F::mul(num, uint16_t(u));
+      for (uint32_t u = 1u << j; u < (2u << j); ++u) den = F::mul(den, uint16_t(u));
+      d.push_back(F::mul(num, F::inv(den)));
     }
   });
   return d;
 }
 
-}  // namespace
-
-Fft16Lowered fft16_decode_lowered(size_t n, const std::vector<uint32_t> &valid,
-                                  const std::vector<uint32_t> &outputs) {
+template <class F>
+Fft16Lowered decode_lowered(size_t n, const std::vector<uint32_t> &valid,
+                            const std::vector<uint32_t> &outputs) {
   size_t N = 2;
   int levels = 1;
   while (N < n) {
     N <<= 1;
     ++levels;
   }
-  const Gf16 &f = gf16();
   Fft16Lowered L;
   L.regs = N;
   std::vector<uint8_t> known(N, 0);
@@ -307,14 +326,14 @@ Fft16Lowered fft16_decode_lowered(size_t n, const std::vector<uint32_t> &valid,
     for (uint32_t r : erased) z.push_back(encode_op({kOpZero, uint16_t(r), uint16_t(r), 0}));
     glue_step(L, z);
   }
-  // log Lambda(x) = sum over e of log(x ^ e), mod 2^16 - 1
+  // log Lambda(x) = sum over e of log(x ^ e), mod the multiplicative order
   auto log_prod = [&](uint32_t x, bool skip_self) {
     uint64_t acc = 0;
     for (uint32_t e : erased) {
       if (skip_self && e == x) continue;
-      acc += f.log[x ^ e];
+      acc += F::log(x ^ e);
     }
-    return uint32_t(acc % 65535u);
+    return uint32_t(acc % F::kOrder);
   };
   // 2. known values times Lambda(x)
   {
@@ -324,7 +343,7 @@ Fft16Lowered fft16_decode_lowered(size_t n, const std::vector<uint32_t> &valid,
     });
     std::vector<OpWords> mul;
     for (size_t i = 0; i < valid.size(); ++i) {
-      const uint16_t c = f.exp[lg[i]];
+      const uint16_t c = F::exp(lg[i]);
       if (c != 1) mul.push_back(encode_op({kOpMulCopy, uint16_t(valid[i]), uint16_t(valid[i]), c}));
     }
     glue_step(L, mul);
@@ -339,7 +358,7 @@ Fft16Lowered fft16_decode_lowered(size_t n, const std::vector<uint32_t> &valid,
       const int level = inverse ? step : levels - 1 - step;
       const size_t stride = size_t(1) << level;
       for (size_t block = 0; block < N; block += 2 * stride) {
-        consts.push_back(fft16_what(level, block));
+        consts.push_back(F::what(level, block));
         bool any = live == nullptr;
         for (size_t j = block; !any && j < block + 2 * stride; ++j) any = (*live)[j] != 0;
         skip.push_back(!any);
@@ -355,18 +374,18 @@ Fft16Lowered fft16_decode_lowered(size_t n, const std::vector<uint32_t> &valid,
   // overwritten (blocks [i - w, i) ^= [i, i + w) for i = 1 .. N-1, w the lowest
   // set bit of i), scale back by 1 / s_t.
   {
-    const auto &D = derivative_consts();
+    const auto &D = derivative_consts<F>();
     std::vector<uint16_t> sc(N, 1);
     for (size_t t = 1; t < N; ++t) {
       int j = 0;
       while (!((t >> j) & 1)) ++j;
-      sc[t] = f.mul(sc[t & (t - 1)], D[j]);
+      sc[t] = F::mul(sc[t & (t - 1)], D[j]);
     }
     std::vector<OpWords> pre, post;
     for (size_t t = 0; t < N; ++t) {
       if (sc[t] == 1) continue;
       pre.push_back(encode_op({kOpMulCopy, uint16_t(t), uint16_t(t), sc[t]}));
-      post.push_back(encode_op({kOpMulCopy, uint16_t(t), uint16_t(t), f.inv(sc[t])}));
+      post.push_back(encode_op({kOpMulCopy, uint16_t(t), uint16_t(t), F::inv(sc[t])}));
     }
     glue_step(L, pre);
     std::vector<Fft16Op> ops;
@@ -391,7 +410,7 @@ Fft16Lowered fft16_decode_lowered(size_t n, const std::vector<uint32_t> &valid,
     });
     std::vector<OpWords> mul;
     for (size_t i = 0; i < outputs.size(); ++i) {
-      const uint16_t c = f.exp[(65535u - lg[i]) % 65535u];
+      const uint16_t c = F::exp((F::kOrder - lg[i]) % F::kOrder);
       if (c != 1) mul.push_back(encode_op({kOpMulCopy, uint16_t(outputs[i]), uint16_t(outputs[i]), c}));
     }
     glue_step(L, mul);
@@ -399,6 +418,18 @@ Fft16Lowered fft16_decode_lowered(size_t n, const std::vector<uint32_t> &valid,
   L.load = valid;
   L.store = outputs;
   return L;
+}
+
+}  // namespace
+
+Fft16Lowered fft16_decode_lowered(size_t n, const std::vector<uint32_t> &valid,
+                                  const std::vector<uint32_t> &outputs) {
+  return decode_lowered<TowerField>(n, valid, outputs);
+}
+
+Fft16Lowered fft8_decode_lowered(size_t n, const std::vector<uint32_t> &valid,
+                                 const std::vector<uint32_t> &outputs) {
+  return decode_lowered<ByteField>(n, valid, outputs);
 }
 
 namespace {

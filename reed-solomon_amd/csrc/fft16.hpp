@@ -120,6 +120,8 @@ bool fft16_fits_device(const Fft16Program &p);
 
 // WHAT[level][x] (src/fft16.rs:42-67), level < 11.
 uint16_t fft16_what(int level, size_t x);
+// The byte field's WHAT[level][x] (src/fft.rs:28-79), level < 8, x < 256.
+uint16_t fft8_what(int level, size_t x);
 
 // Erasure decoding by the additive FFT (formal-derivative method; no reference
 // counterpart, the reference decodes through matrix plans).  The codeword of
@@ -135,6 +137,10 @@ uint16_t fft16_what(int level, size_t x);
 constexpr size_t kMaxDecodePoints = 2048;
 Fft16Lowered fft16_decode_lowered(size_t n, const std::vector<uint32_t> &valid,
                                   const std::vector<uint32_t> &outputs);
+// The same decoder over GF(2^8) (n <= 256 points; no reference counterpart),
+// for the byte-plane interpreter (launch_fft16 with byte_planes).
+Fft16Lowered fft8_decode_lowered(size_t n, const std::vector<uint32_t> &valid,
+                                 const std::vector<uint32_t> &outputs);
 hipError_t fft16_device_upload(const Fft16Lowered &L, Fft16DevicePtr *out);
 size_t fft16_device_loads(const Fft16Device &d);
 size_t fft16_device_stores(const Fft16Device &d);

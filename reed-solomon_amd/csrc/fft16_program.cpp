@@ -69,6 +69,12 @@ const std::vector<uint8_t> &what8_table() {
   return t;
 }
 
+}  // namespace
+
+uint16_t fft8_what(int level, size_t x) { return what8_table()[(size_t(level) << 8) | x]; }
+
+namespace {
+
 // The field a program is built over: its WHAT rows and product.
 struct FieldOps {
   uint16_t (*what)(int level, size_t x);

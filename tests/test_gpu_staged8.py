@@ -121,3 +121,79 @@ def test_automatic_route_and_reports(trs):
     assert trs.ReedSolomon(12, 8).reference_route(1 << 20) == "fused-matrix"
     assert trs.ReedSolomon(128, 128).reference_route(1 << 20) == "fused-matrix"
     assert trs.ReedSolomon(128, 32).reference_route(1 << 20) == "fft-staged"
+
+
+# ---- FFT erasure decoder over GF(2^8) (reconstruct route "fft") -----------
+
+@pytest.mark.parametrize("k,m", [(4, 2), (10, 4), (7, 13), (32, 32), (64, 64), (100, 156),
+                                 (200, 55), (128, 128)])
+def test_fft_decode_route_matches_oracle(trs, k, m):
+    print(f"fft decode k={k} m={m}", flush=True)
+    n = k + m
+    G = O.build_matrix(8, k, n)
+    rs = trs.ReedSolomon(k, m)
+    rs.set_decode_route("fft")
+    rng = np.random.default_rng(k * 7 + m)
+    for length in (1, 33, 4096, 65536 + 3):
+        data = seeded(k, length, 0xDEC0 + length)
+        full = np.concatenate([data, O.rs8_encode(G, k, data)])
+        for data_only in (False, True):
+            e = int(rng.integers(1, m + 1))
+            erased = set(rng.choice(n, e, replace=False).tolist())
+            shards = rows_to_dev(full)
+            for i in erased:
+                shards[i] = None
+            (rs.reconstruct_data if data_only else rs.reconstruct)(shards)
+            for i in range(n):
+                if data_only and i >= k and i in erased:
+                    assert shards[i] is None
+                else:
+                    assert (shards[i].cpu().numpy() == full[i]).all(), \
+                        (k, m, length, sorted(erased), i)
+
+
+def test_fft_decode_prepared_and_full_size(trs):
+    k, m, length = 64, 64, (1 << 20) + 7
+    n = k + m
+    G = O.build_matrix(8, k, n)
+    rs = trs.ReedSolomon(k, m)
+    rs.set_decode_route("fft")
+    data = seeded(k, length, 0xFEED)
+    dev = rows_to_dev(np.concatenate([data, np.zeros((m, length), np.uint8)]))
+    rs.encode(dev)
+    ref = to_host(dev)
+    cols = slice(0, length, 977)
+    assert (ref[k:, cols] == O.rs8_encode(G, k, np.ascontiguousarray(data[:, cols]))).all()
+    erased = set(range(0, n, 2))  # 64 erasures, data and parity
+    present = [i not in erased for i in range(n)]
+    dec = rs.prepare_decode(present)
+    shards = list(dev)
+    for i in erased:
+        shards[i] = None
+    dec.reconstruct(shards)
+    assert (to_host(shards) == ref).all()
+    # the matrix route gives the same bytes
+    rs.set_decode_route("matrix")
+    for i in erased:
+        shards[i] = None
+    rs.reconstruct(shards)
+    assert (to_host(shards) == ref).all()
+
+
+def test_automatic_decode_route_matches_oracle(trs):
+    # k > 32 at 16 KiB / 200 KiB with >= 4 outputs: the automatic route takes
+    # the FFT decoder; 1 MiB stays on the matrix plans -- every case byte-exact
+    k, m = 48, 16
+    n = k + m
+    G = O.build_matrix(8, k, n)
+    rs = trs.ReedSolomon(k, m)
+    rng = np.random.default_rng(5)
+    for length, e in ((16 << 10, 1), (16 << 10, 16), (200 << 10, 5), (1 << 20, 9)):
+        data = seeded(k, length, 0xA070 + length + e)
+        full = np.concatenate([data, O.rs8_encode(G, k, data)])
+        erased = set(rng.choice(n, e, replace=False).tolist())
+        shards = rows_to_dev(full)
+        for i in erased:
+            shards[i] = None
+        rs.reconstruct(shards)
+        assert (to_host(shards) == full).all(), (length, sorted(erased))
