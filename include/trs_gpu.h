@@ -138,9 +138,10 @@ int trs_rs8_set_apply_route(trs_rs8 *rs, int route);
  * the call): 0 = automatic, 1 = matrix plans (the reference's decode,
  * src/reedsolomon.rs:829-891), 2 = the additive-FFT erasure decoder over
  * GF(2^8) on the byte-plane interpreter (formal-derivative method; no
- * reference counterpart).  Automatic takes the decoder for k > 32 up to
- * 64 KiB shards, and up to 256 KiB from 4 rebuilt shards (measured
- * crossovers), else the matrix plans.  Byte-identical either way. */
+ * reference counterpart).  Automatic takes the decoder for k > 32 below
+ * 128 KiB shards, and up to 256 KiB when 100 N < 130 k + 45 e (N the
+ * power of two >= k + m, e the rebuilt shards; measured crossovers), else
+ * the matrix plans.  Byte-identical either way. */
 int trs_rs8_set_decode_route(trs_rs8 *rs, int route);
 /* ReedSolomon::prepare_decode, src/reedsolomon.rs:484-507. */
 int trs_rs8_prepare_decode(const trs_rs8 *rs, const uint8_t *present, size_t n_present,

@@ -239,15 +239,22 @@ bool use_fft_decode(int route, size_t n, size_t k, size_t nout) {
 // built on first use.
 // Decode route for a GF(2^8) reconstruct (route as trs_rs8_set_decode_route).
 // Automatic takes the FFT decoder where it measured faster than the matrix
-// apply, per call at full size (14 shapes x 1..m erasures x 16 KiB - 4 MiB):
-// for k > 32 up to 64 KiB (the matrix kernels leave the chip mostly idle
-// there; 1.4-8x), and up to 256 KiB from 4 outputs; k <= 32 stays on the
-// matrix route (MFMA), and so do longer shards, where the decoder's two
-// N-point transforms cost more than the k * nout products per column.
+// apply, per call at full size (14 shapes x 1..m erasures x 16 KiB - 4 MiB),
+// only for k > 32 (MFMA covers k <= 32):
+// - below 128 KiB, where the matrix route is the table kernel and leaves the
+//   chip mostly idle (the decoder is 1.4-8x faster);
+// - up to 256 KiB when the decoder's N-point transforms (N = 2^ceil(log2 n),
+//   about 0.55 N us there) undercut the planes kernel (about 0.72 k +
+//   0.25 nout us): 100 N < 130 k + 45 nout;
+// longer shards stay on the matrix route.
 bool use_fft_decode8(int route, size_t n, size_t k, size_t nout, size_t len) {
   if (route == 1 || nout == 0 || n > 256) return false;
   if (route == 2) return true;
-  return k > 32 && (len <= (size_t(64) << 10) || (len <= (size_t(256) << 10) && nout >= 4));
+  if (k <= 32) return false;
+  if (len < (size_t(128) << 10)) return true;
+  size_t N = 2;
+  while (N < n) N <<= 1;
+  return len <= (size_t(256) << 10) && 100 * N < 130 * k + 45 * nout;
 }
 
 int plan_fft16(Plan &p, size_t n, const Fft16Device **out, bool byte_field = false) {

@@ -815,9 +815,14 @@ hipError_t fft16_device_new(const Fft16Program &p, Fft16DevicePtr *out) {
 }
 
 namespace {
-uint32_t choose_gshift(const Fft16Device &d, uint64_t groups, uint32_t reg_bytes = kRegBytes) {
+// Column groups per workgroup: as many as fit `target` bytes of LDS (up to 8
+// and the shard's groups).  Tower programs aim at 64 KiB (two workgroups per
+// CU); byte-plane programs at 32 KiB, which measured 5-14 % faster on
+// (100,156) and (200,55) at 1-4 MiB than 64 KiB (16 KiB: up to 40 % slower).
+uint32_t choose_gshift(const Fft16Device &d, uint64_t groups, uint32_t reg_bytes = kRegBytes,
+                       size_t target = kLdsTarget) {
   uint32_t gs = 0;
-  while ((1u << (gs + 1)) <= kMaxGroupsPerWg && (d.regs * reg_bytes << (gs + 1)) <= kLdsTarget &&
+  while ((1u << (gs + 1)) <= kMaxGroupsPerWg && (d.regs * reg_bytes << (gs + 1)) <= target &&
          (uint64_t(1) << (gs + 1)) <= groups)
     ++gs;
   return gs;
@@ -896,7 +901,7 @@ hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t
   const uint64_t cols = call.cols ? call.cols : call.plane;
   const uint64_t groups = (cols + kGroupCols - 1) / kGroupCols;
   const uint32_t reg_bytes = byte_planes ? kRegBytes / 2 : kRegBytes;
-  const uint32_t gs = choose_gshift(d, groups, reg_bytes);
+  const uint32_t gs = choose_gshift(d, groups, reg_bytes, byte_planes ? kLdsTarget / 2 : kLdsTarget);
   const uint64_t cblocks = (groups + (1u << gs) - 1) >> gs;
   const uint64_t total = cblocks * call.stripes;
   if (groups > 0xffffffffu || total > 0x7fffffffu) return hipErrorInvalidValue;
