@@ -139,9 +139,9 @@ int trs_rs8_set_apply_route(trs_rs8 *rs, int route);
  * src/reedsolomon.rs:829-891), 2 = the additive-FFT erasure decoder over
  * GF(2^8) on the byte-plane interpreter (formal-derivative method; no
  * reference counterpart).  Automatic takes the decoder for k > 32 below
- * 128 KiB shards, and up to 256 KiB when 100 N < 130 k + 45 e (N the
- * power of two >= k + m, e the rebuilt shards; measured crossovers), else
- * the matrix plans.  Byte-identical either way. */
+ * 128 KiB shards, up to 256 KiB when 100 N < 170 k + 60 e, and beyond when
+ * k e > 7 N log2 N (N the power of two >= k + m, e the rebuilt shards;
+ * measured crossovers), else the matrix plans.  Byte-identical either way. */
 int trs_rs8_set_decode_route(trs_rs8 *rs, int route);
 /* ReedSolomon::prepare_decode, src/reedsolomon.rs:484-507. */
 int trs_rs8_prepare_decode(const trs_rs8 *rs, const uint8_t *present, size_t n_present,

@@ -244,17 +244,23 @@ bool use_fft_decode(int route, size_t n, size_t k, size_t nout) {
 // - below 128 KiB, where the matrix route is the table kernel and leaves the
 //   chip mostly idle (the decoder is 1.4-8x faster);
 // - up to 256 KiB when the decoder's N-point transforms (N = 2^ceil(log2 n),
-//   about 0.55 N us there) undercut the planes kernel (about 0.72 k +
-//   0.25 nout us): 100 N < 130 k + 45 nout;
-// longer shards stay on the matrix route.
+//   about 0.47 N us there) undercut the planes kernel (about 0.72 k +
+//   0.25 nout us): 100 N < 170 k + 60 nout;
+// - beyond, when k * nout products per column outweigh the transforms
+//   (about 0.9 N log2 N against 0.133 k nout us per 4 MiB):
+//   k nout > 7 N log2 N.
 bool use_fft_decode8(int route, size_t n, size_t k, size_t nout, size_t len) {
   if (route == 1 || nout == 0 || n > 256) return false;
   if (route == 2) return true;
   if (k <= 32) return false;
   if (len < (size_t(128) << 10)) return true;
-  size_t N = 2;
-  while (N < n) N <<= 1;
-  return len <= (size_t(256) << 10) && 100 * N < 130 * k + 45 * nout;
+  size_t N = 2, lg = 1;
+  while (N < n) {
+    N <<= 1;
+    ++lg;
+  }
+  if (len <= (size_t(256) << 10)) return 100 * N < 170 * k + 60 * nout;
+  return k * nout > 7 * N * lg;
 }
 
 int plan_fft16(Plan &p, size_t n, const Fft16Device **out, bool byte_field = false) {

@@ -176,6 +176,7 @@ void coalesce_and_compact(Fft16Program &p) {
 std::pair<uint32_t, uint32_t> encode_op(const Fft16Op &op) {
   uint32_t w = 0;
   if (op.kind == kOpMulXor || op.kind == kOpMulCopy) w = kara_word(op.c) & 0xffffffu;
+  if (op.kind == kOpDeriv) w = op.c;  // the transform's level count
   return {uint32_t(op.dst) | (uint32_t(op.src) << 16), uint32_t(op.kind) | (w << 8)};
 }
 
@@ -370,9 +371,10 @@ Fft16Lowered decode_lowered(size_t n, const std::vector<uint32_t> &valid,
   // 4. coefficient t += sum over zero bits j of t of D_j * coefficient (t | 2^j)
   // (the added Q itself vanishes on E).  With s_t = prod_{j in t} D_j and
   // u_t = s_t q_t this is s_t Q'_t = sum_j u_(t | 2^j): scale by s_t, XOR the
-  // derivative in place in an order that reads every source before it is
-  // overwritten (blocks [i - w, i) ^= [i, i + w) for i = 1 .. N-1, w the lowest
-  // set bit of i), scale back by 1 / s_t.
+  // derivative in place, scale back by 1 / s_t.  u_t reads only registers
+  // with one more set bit, so updating t in order of increasing popcount
+  // reads every source before it is overwritten, and each popcount layer is
+  // one step of independent kOpDeriv ops (levels + 1 steps in all).
   {
     const auto &D = derivative_consts<F>();
     std::vector<uint16_t> sc(N, 1);
@@ -388,13 +390,13 @@ Fft16Lowered decode_lowered(size_t n, const std::vector<uint32_t> &valid,
       post.push_back(encode_op({kOpMulCopy, uint16_t(t), uint16_t(t), F::inv(sc[t])}));
     }
     glue_step(L, pre);
-    std::vector<Fft16Op> ops;
-    ops.reserve(size_t(levels) * N / 2);
-    for (size_t i = 1; i < N; ++i) {
-      const size_t w = i & (~i + 1);
-      for (size_t r = 0; r < w; ++r) ops.push_back({kOpXor, uint16_t(i - w + r), uint16_t(i + r), 0});
+    for (int pc = 0; pc < levels; ++pc) {
+      std::vector<OpWords> layer;
+      for (size_t t = 0; t < N; ++t)
+        if (__builtin_popcountll(t) == pc)
+          layer.push_back(encode_op({kOpDeriv, uint16_t(t), uint16_t(t), uint16_t(levels)}));
+      glue_step(L, layer);
     }
-    schedule_glue(L, ops.data(), ops.size());
     glue_step(L, post);
   }
   // 5. coefficients to values, pruned to the blocks that reach an output
@@ -721,7 +723,16 @@ __device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
         const uint32_t xd = ((op.x & 0xffffu) << a.gshift) + g;
         const uint32_t xs = ((op.x >> 16) << a.gshift) + g;
         uint32_t D[P], S[P];
-        if (kind == kOpZero) {
+        if (kind == kOpDeriv) {
+          // dst ^= registers dst | 2^j for the clear bits j below the level count
+          const uint32_t t = op.x & 0xffffu;
+          rf_load<P>(rf, xd, D);
+          for (uint32_t zb = ~t & ((1u << (op.y >> 8)) - 1u); zb; zb &= zb - 1u) {
+            rf_load<P>(rf, ((t | (zb & (~zb + 1u))) << a.gshift) + g, S);
+#pragma unroll
+            for (int j = 0; j < P; ++j) D[j] ^= S[j];
+          }
+        } else if (kind == kOpZero) {
 #pragma unroll
           for (int j = 0; j < P; ++j) D[j] = 0;
         } else {

@@ -20,7 +20,16 @@ namespace trs {
 constexpr size_t kMaxFftDataShards = 1024;
 constexpr size_t kMaxStagedRegisters16 = 2048;
 
-enum Fft16OpKind : uint8_t { kOpXor = 0, kOpCopy = 1, kOpMulXor = 2, kOpMulCopy = 3, kOpZero = 4 };
+// kOpDeriv (decoder programs only): dst ^= XOR of the registers dst | 2^j
+// for every bit j < c clear in dst (one layer of the formal derivative).
+enum Fft16OpKind : uint8_t {
+  kOpXor = 0,
+  kOpCopy = 1,
+  kOpMulXor = 2,
+  kOpMulCopy = 3,
+  kOpZero = 4,
+  kOpDeriv = 5
+};
 
 struct Fft16Op {
   uint8_t kind;

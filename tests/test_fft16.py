@@ -96,7 +96,7 @@ def test_c3_program_coalesces_to_1024_registers(trs):
 # ---- replay of the device lowering ----------------------------------------
 
 SKIP, ZERO, ONE = 1 << 24, 2 << 24, 4 << 24
-K_XOR, K_COPY, K_MULXOR, K_MULCOPY, K_ZERO = range(5)
+K_XOR, K_COPY, K_MULXOR, K_MULCOPY, K_ZERO, K_DERIV = range(6)
 
 
 def _lowered_from(fn, k, m):
@@ -158,8 +158,21 @@ def replay(steps, consts, ops, load, store, regs, inputs):
             dst, src, knd, cw = w0 & 0xFFFF, w0 >> 16, w1 & 0xFF, (w1 >> 8)[:, None]
             # independent: distinct destinations, no destination read by another op
             assert len(set(dst)) == len(dst)
-            reads_other = (knd != K_ZERO) & (src != dst)
+            reads_other = (knd != K_ZERO) & (knd != K_DERIV) & (src != dst)
             assert not (set(dst) & set(src[reads_other]))
+            deriv = knd == K_DERIV
+            if deriv.any():
+                # dst ^= registers dst | 2^j for the clear bits j below the level count
+                assert deriv.all()
+                srcs = [[t | (1 << j) for j in range(int(c)) if not (t >> j) & 1]
+                        for t, c in zip(dst, cw[:, 0])]
+                assert not (set(dst) & {x for xs in srcs for x in xs})
+                val = np.zeros((len(dst), R.shape[1]), np.int64)
+                for i, xs in enumerate(srcs):
+                    for x in xs:
+                        val[i] ^= R[x]
+                R[dst] ^= val
+                continue
             S = R[src]
             val = np.where((knd == K_MULXOR)[:, None] | (knd == K_MULCOPY)[:, None],
                            _kara(cw, S), S)
