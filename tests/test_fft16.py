@@ -252,5 +252,9 @@ def test_fft_decoder_c3_pattern(trs):
     present = [i not in erased for i in range(k + m)]
     steps, consts, ops, load, store, regs = _decode_program(trs, k, k + m, present)
     assert regs == 2048
+    # zero, scale by Lambda, 11 inverse levels, pre-scale, the formal derivative
+    # as one step per popcount layer 0..10, post-scale, 11 forward levels,
+    # scale by 1 / Lambda'
+    assert len(steps) == 1 + 1 + 11 + 1 + 11 + 1 + 11 + 1
     got = replay(steps, consts, ops, load, store, regs, cw[load])
     assert (got == cw[store]).all()
