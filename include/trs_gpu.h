@@ -336,8 +336,9 @@ size_t trs_batch8_size(const trs_batch8 *b);
  * in-flight work stay within hbm_budget_bytes (0 = unlimited), and launches
  * it, so requests arriving while others run start as soon as a slot and budget
  * free up.  window_us: how long the dispatcher waits for more arrivals before
- * launching a partial batch.  Shard buffers must stay valid until the ticket
- * completes. */
+ * launching a partial batch while other slots have work in flight (with every
+ * slot idle a request launches at once).  Shard buffers must stay valid until
+ * the ticket completes. */
 typedef struct trs_server trs_server;
 int trs_server_new(uint32_t slots, uint64_t hbm_budget_bytes, uint32_t max_batch,
                    uint32_t window_us, trs_server **out);

@@ -2149,8 +2149,13 @@ void server_dispatch(trs_server *S) {
       return false;
     });
     if (S->stop) return;
-    // short batching window: let a burst of arrivals join this batch
-    if (S->window_us && S->pending.size() < S->max_batch) {
+    // Short batching window, only while other work is in flight: then the
+    // GPU is busy anyway and a burst of arrivals can join this batch.  With
+    // every slot idle the request goes out at once (the window would only
+    // add its length to the latency).
+    bool busy = false;
+    for (auto &sl : S->slots) busy = busy || sl->busy;
+    if (S->window_us && busy && S->pending.size() < S->max_batch) {
       lk.unlock();
       std::this_thread::sleep_for(std::chrono::microseconds(S->window_us));
       lk.lock();
