@@ -263,6 +263,19 @@ bool use_fft_decode8(int route, size_t n, size_t k, size_t nout, size_t len) {
   return k * nout > 7 * N * lg;
 }
 
+// The same choice for a batched reconstruct, where the work list fills the
+// chip whatever the shard length: the full-size rule, k nout > 7 N log2 N.
+bool use_fft_decode8_batched(int route, size_t n, size_t k, size_t nout) {
+  if (route == 1 || nout == 0 || n > 256) return false;
+  if (route == 2) return true;
+  size_t N = 2, lg = 1;
+  while (N < n) {
+    N <<= 1;
+    ++lg;
+  }
+  return k > 32 && k * nout > 7 * N * lg;
+}
+
 int plan_fft16(Plan &p, size_t n, const Fft16Device **out, bool byte_field = false) {
   std::call_once(p.fft_once, [&p, n, byte_field] {
     std::vector<uint32_t> outputs(p.missing_data);
@@ -853,14 +866,23 @@ int trs_rs8_generator(const trs_rs8 *rs, uint8_t *out, size_t out_len) {
 }
 
 namespace {
-int run_staged8(const trs_rs8 *rs, const std::vector<const uint8_t *> &ptrs, size_t len,
-                hipStream_t stream) {
+// The codec's staged program on the device, built and uploaded once.
+int staged8_device(const trs_rs8 *rs, const Fft16Device **out) {
   std::call_once(rs->staged_once, [rs] {
     rs->staged_status = fft16_device_new(build_staged_program8(rs->k, rs->m), &rs->staged_dev);
   });
   if (rs->staged_status == hipErrorInvalidValue) return TRS_ERR_UNSUPPORTED;
   if (rs->staged_status != hipSuccess) return hip_status(rs->staged_status);
-  const Fft16Device &d = *rs->staged_dev;
+  *out = rs->staged_dev.get();
+  return TRS_OK;
+}
+
+int run_staged8(const trs_rs8 *rs, const std::vector<const uint8_t *> &ptrs, size_t len,
+                hipStream_t stream) {
+  const Fft16Device *dp = nullptr;
+  const int dst = staged8_device(rs, &dp);
+  if (dst) return dst;
+  const Fft16Device &d = *dp;
   if (fft16_device_loads(d) + fft16_device_stores(d) != ptrs.size()) return TRS_ERR_UNSUPPORTED;
   if (len == 0) return TRS_OK;
   return launch_fft16_one(d, ptrs, fft16_device_stores(d), len, stream, true);
@@ -1604,6 +1626,9 @@ struct trs_batch8 {
     uint32_t k = 0, nout = 0;
     uint32_t prog = 0;  // generated bit-sliced program (encode of generated shapes), 0 = table
     const void *frag = nullptr;  // MFMA A fragments: the job runs on the int8-MFMA kernel
+    // staged FFT program or FFT erasure decoder: the job runs on the byte-plane
+    // interpreter (one launch for all such jobs)
+    const Fft16Device *fft = nullptr;
   };
   // One device work list per kernel: table-kernel jobs and generated-program jobs.
   struct List {
@@ -1615,6 +1640,8 @@ struct trs_batch8 {
   std::vector<Req> reqs;
   DevBuf ptrs;
   List table, mfma;
+  DevBuf fft_blob;  // job table of the interpreter jobs
+  Fft16JobsPlan fft_plan;
   // Runtime-coefficient jobs run on the bit-sliced tile (k_batch8_planes)
   // unless TRS_BATCH8_TILE=table selects the packed-byte table tile.
   bool planes = true;
@@ -1654,6 +1681,7 @@ void trs_batch8_clear(trs_batch8 *b) {
   b->drop_graph();
   b->table.njobs = b->mfma.njobs = 0;
   b->table.total_tiles = b->mfma.total_tiles = 0;
+  b->fft_plan = Fft16JobsPlan{};
   for (auto &L : b->bs) {
     L.njobs = 0;
     L.total_tiles = 0;
@@ -1684,6 +1712,10 @@ int trs_batch8_add_encode(trs_batch8 *b, const trs_rs8 *rs, uint8_t *const *shar
   r.k = uint32_t(rs->k);
   r.nout = uint32_t(rs->m);
   if (rs->route != kRouteTable) r.prog = bs_program_id(rs->route, r.k, r.nout);
+  // the staged FFT where it stays ahead at any length (its full-chip rule)
+  if (rs->route == kRouteTable && rs->staged_max == SIZE_MAX &&
+      (st = staged8_device(rs, &r.fft)))
+    return st;
   b->reqs.push_back(std::move(r));
   b->prepared = false;
   return TRS_OK;
@@ -1711,7 +1743,9 @@ int trs_batch8_add_reconstruct(trs_batch8 *b, const trs_rs8 *rs, uint8_t *const 
   r.len = len;
   r.k = uint32_t(rs->k);
   r.nout = uint32_t(nout);
-  if (use_mfma(rs->apply_route.load(), rs->k, len, true) && plan->frag.p) {
+  if (use_fft_decode8_batched(rs->decode_route.load(), rs->n, rs->k, nout)) {
+    if ((st = plan_fft16(*plan, rs->n, &r.fft, true))) return st;
+  } else if (use_mfma(rs->apply_route.load(), rs->k, len, true) && plan->frag.p) {
     bool al = true;
     for (auto q : r.ptrs) al = al && (reinterpret_cast<uintptr_t>(q) & 3) == 0;
     if (al) r.frag = plan->frag.p;
@@ -1740,6 +1774,7 @@ int trs_batch8_prepare(trs_batch8 *b) {
   if (const char *e = std::getenv("TRS_BATCH8_TILE")) b->planes = std::strcmp(e, "table") != 0;
   const uint64_t table_chunk = b->planes ? kPlanesChunkBytes : kChunkBytes;
   const auto *dptr = static_cast<const uint8_t *const *>(b->ptrs.p);
+  std::vector<Fft16JobSpec> fjobs;
   for (const auto &r : b->reqs) {
     BatchJob J{};
     J.ptrs = dptr + hp.size();
@@ -1747,6 +1782,10 @@ int trs_batch8_prepare(trs_batch8 *b) {
     for (auto q : r.ptrs) {
       hp.push_back(q);
       al = al && aligned16(q);
+    }
+    if (r.fft) {
+      fjobs.push_back({r.fft, J.ptrs, r.len, r.nout, al ? 1u : 0u});
+      continue;
     }
     J.coef = r.coef->ptr();
     J.len = r.len;
@@ -1777,6 +1816,12 @@ int trs_batch8_prepare(trs_batch8 *b) {
   }
   if (!hp.empty())
     TRS_TRY(hipMemcpy(b->ptrs.p, hp.data(), hp.size() * sizeof(void *), hipMemcpyHostToDevice));
+  TRS_TRY(fft16_jobs_plan(fjobs, true, &b->fft_plan));
+  if (b->fft_plan.total) {
+    TRS_TRY(b->fft_blob.ensure(b->fft_plan.blob.size()));
+    TRS_TRY(hipMemcpy(b->fft_blob.p, b->fft_plan.blob.data(), b->fft_plan.blob.size(),
+                      hipMemcpyHostToDevice));
+  }
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -1836,6 +1881,10 @@ int batch8_launch(trs_batch8 *b, hipStream_t s) {
     const int st = hip_status(launch_batch_mfma(static_cast<const BatchJob *>(b->mfma.jobs.p),
                                                 static_cast<const uint32_t *>(b->mfma.tiles.p),
                                                 b->mfma.total_tiles, b->mfma.grid, s));
+    if (st) return st;
+  }
+  if (b->fft_plan.total) {
+    const int st = hip_status(fft16_jobs_launch(b->fft_plan, b->fft_blob.p, s));
     if (st) return st;
   }
   if (b->table.total_tiles)

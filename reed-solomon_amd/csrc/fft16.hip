@@ -633,13 +633,13 @@ __global__ __launch_bounds__(T) void k_fft16_inline(Fft16Args a, Fft16Ptrs P) {
 
 // Many programs (one per job, e.g. one decode pattern per stripe) in one
 // launch: workgroup w runs job wg_job[w] on its column block w - first.
-template <int T>
+template <int T, int PL>
 __global__ __launch_bounds__(T) void k_fft16_jobs(const Fft16Args *jobs, const uint32_t *wg_job,
                                                   uint32_t total) {
   extern __shared__ uint4 rf[];
   const uint32_t wg = xcd_order(blockIdx.x, total);
   const Fft16Args a = ld_const_val(jobs + ld_const(wg_job + wg));  // scalar loads
-  run_program<T, 16>(a, (wg - a.total) << a.gshift, a.ptrs, rf);
+  run_program<T, PL>(a, (wg - a.total) << a.gshift, a.ptrs, rf);
 }
 
 template <int T, int P>
@@ -849,7 +849,7 @@ hipError_t raise_lds_limit(size_t lds) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess || raised == dev) return e;
-  const void *kernels[15] = {reinterpret_cast<const void *>(&k_fft16<1024, 16>),
+  const void *kernels[18] = {reinterpret_cast<const void *>(&k_fft16<1024, 16>),
                              reinterpret_cast<const void *>(&k_fft16<512, 16>),
                              reinterpret_cast<const void *>(&k_fft16<256, 16>),
                              reinterpret_cast<const void *>(&k_fft16_inline<1024, 16>),
@@ -861,9 +861,12 @@ hipError_t raise_lds_limit(size_t lds) {
                              reinterpret_cast<const void *>(&k_fft16_inline<1024, 8>),
                              reinterpret_cast<const void *>(&k_fft16_inline<512, 8>),
                              reinterpret_cast<const void *>(&k_fft16_inline<256, 8>),
-                             reinterpret_cast<const void *>(&k_fft16_jobs<1024>),
-                             reinterpret_cast<const void *>(&k_fft16_jobs<512>),
-                             reinterpret_cast<const void *>(&k_fft16_jobs<256>)};
+                             reinterpret_cast<const void *>(&k_fft16_jobs<1024, 16>),
+                             reinterpret_cast<const void *>(&k_fft16_jobs<512, 16>),
+                             reinterpret_cast<const void *>(&k_fft16_jobs<256, 16>),
+                             reinterpret_cast<const void *>(&k_fft16_jobs<1024, 8>),
+                             reinterpret_cast<const void *>(&k_fft16_jobs<512, 8>),
+                             reinterpret_cast<const void *>(&k_fft16_jobs<256, 8>)};
   for (const void *k : kernels)
     if ((e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(kLdsMax))) !=
         hipSuccess)
@@ -946,17 +949,21 @@ hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t
   return hipGetLastError();
 }
 
-hipError_t launch_fft16_jobs(const std::vector<Fft16JobSpec> &jobs, hipStream_t stream) {
+hipError_t fft16_jobs_plan(const std::vector<Fft16JobSpec> &jobs, bool byte_planes,
+                           Fft16JobsPlan *plan) {
+  *plan = Fft16JobsPlan{};
+  plan->byte_planes = byte_planes;
+  const uint32_t reg_bytes = byte_planes ? kRegBytes / 2 : kRegBytes;
+  const size_t target = byte_planes ? kLdsTarget / 2 : kLdsTarget;
   std::vector<Fft16Args> table;
   std::vector<uint32_t> wg_job;
-  size_t lds = 0;
   uint64_t widest = 0;
   for (const Fft16JobSpec &j : jobs) {
     const Fft16Device &d = *j.prog;
     if (j.plane == 0 || j.nstore == 0) continue;
     if (j.nstore > d.nstore) return hipErrorInvalidValue;
     const uint64_t groups = (j.plane + kGroupCols - 1) / kGroupCols;
-    const uint32_t gs = choose_gshift(d, groups);
+    const uint32_t gs = choose_gshift(d, groups, reg_bytes, target);
     const uint64_t cblocks = (groups + (1u << gs) - 1) >> gs;
     if (groups > 0xffffffffu || wg_job.size() + cblocks > 0x7fffffffu) return hipErrorInvalidValue;
     Fft16Args a{};
@@ -975,36 +982,57 @@ hipError_t launch_fft16_jobs(const std::vector<Fft16JobSpec> &jobs, hipStream_t 
     a.groups = uint32_t(groups);
     a.cblocks = uint32_t(cblocks);
     a.total = uint32_t(wg_job.size());
-    a.vec_ok = (j.vec_ok && j.plane % 16 == 0) ? 1u : 0u;
+    a.vec_ok = (j.vec_ok && (byte_planes || j.plane % 16 == 0)) ? 1u : 0u;
     wg_job.insert(wg_job.end(), cblocks, uint32_t(table.size()));
     table.push_back(a);
-    lds = std::max(lds, (d.regs * kRegBytes) << gs);
+    plan->lds = std::max(plan->lds, (d.regs * reg_bytes) << gs);
     widest = std::max<uint64_t>(widest, uint64_t(d.max_count) << gs);
   }
   if (table.empty()) return hipSuccess;
-  hipError_t e = raise_lds_limit(lds);
-  if (e != hipSuccess) return e;
   const size_t tb = table.size() * sizeof(Fft16Args), mb = wg_job.size() * sizeof(uint32_t);
-  const size_t off = (tb + 255) & ~size_t(255);
-  void *mem = nullptr;
-  if ((e = hipMallocAsync(&mem, off + mb, stream)) != hipSuccess) return e;
-  std::vector<uint8_t> host(off + mb);
-  std::memcpy(host.data(), table.data(), tb);
-  std::memcpy(host.data() + off, wg_job.data(), mb);
-  e = hipMemcpyAsync(mem, host.data(), off + mb, hipMemcpyHostToDevice, stream);
-  if (e == hipSuccess) {
-    const auto *jt = static_cast<const Fft16Args *>(mem);
-    const auto *wj = reinterpret_cast<const uint32_t *>(static_cast<uint8_t *>(mem) + off);
-    const uint32_t total = uint32_t(wg_job.size());
-    const uint32_t threads = choose_threads(lds, widest);
-    if (threads == 1024)
-      hipLaunchKernelGGL(k_fft16_jobs<1024>, dim3(total), dim3(1024), lds, stream, jt, wj, total);
-    else if (threads == 512)
-      hipLaunchKernelGGL(k_fft16_jobs<512>, dim3(total), dim3(512), lds, stream, jt, wj, total);
-    else
-      hipLaunchKernelGGL(k_fft16_jobs<256>, dim3(total), dim3(256), lds, stream, jt, wj, total);
-    e = hipGetLastError();
+  plan->map_offset = (tb + 255) & ~size_t(255);
+  plan->blob.assign(plan->map_offset + mb, 0);
+  std::memcpy(plan->blob.data(), table.data(), tb);
+  std::memcpy(plan->blob.data() + plan->map_offset, wg_job.data(), mb);
+  plan->total = uint32_t(wg_job.size());
+  plan->threads = choose_threads(plan->lds, widest);
+  return hipSuccess;
+}
+
+hipError_t fft16_jobs_launch(const Fft16JobsPlan &plan, const void *dev_blob, hipStream_t stream) {
+  if (plan.total == 0) return hipSuccess;
+  hipError_t e = raise_lds_limit(plan.lds);
+  if (e != hipSuccess) return e;
+  const auto *jt = static_cast<const Fft16Args *>(dev_blob);
+  const auto *wj =
+      reinterpret_cast<const uint32_t *>(static_cast<const uint8_t *>(dev_blob) + plan.map_offset);
+  const dim3 grid(plan.total);
+  const uint32_t total = plan.total;
+  const size_t lds = plan.lds;
+#define TRS_JOBS(T, PL) \
+  hipLaunchKernelGGL((k_fft16_jobs<T, PL>), grid, dim3(T), lds, stream, jt, wj, total)
+  if (plan.byte_planes) {
+    if (plan.threads == 1024) TRS_JOBS(1024, 8);
+    else if (plan.threads == 512) TRS_JOBS(512, 8);
+    else TRS_JOBS(256, 8);
+  } else {
+    if (plan.threads == 1024) TRS_JOBS(1024, 16);
+    else if (plan.threads == 512) TRS_JOBS(512, 16);
+    else TRS_JOBS(256, 16);
   }
+#undef TRS_JOBS
+  return hipGetLastError();
+}
+
+hipError_t launch_fft16_jobs(const std::vector<Fft16JobSpec> &jobs, hipStream_t stream,
+                             bool byte_planes) {
+  Fft16JobsPlan plan;
+  hipError_t e = fft16_jobs_plan(jobs, byte_planes, &plan);
+  if (e != hipSuccess || plan.total == 0) return e;
+  void *mem = nullptr;
+  if ((e = hipMallocAsync(&mem, plan.blob.size(), stream)) != hipSuccess) return e;
+  e = hipMemcpyAsync(mem, plan.blob.data(), plan.blob.size(), hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = fft16_jobs_launch(plan, mem, stream);
   const hipError_t f = hipFreeAsync(mem, stream);
   return e != hipSuccess ? e : f;
 }

@@ -175,6 +175,21 @@ struct Fft16JobSpec {
   uint32_t nstore;
   uint32_t vec_ok;
 };
-hipError_t launch_fft16_jobs(const std::vector<Fft16JobSpec> &jobs, hipStream_t stream);
+hipError_t launch_fft16_jobs(const std::vector<Fft16JobSpec> &jobs, hipStream_t stream,
+                             bool byte_planes = false);
+
+// The same launch in two parts, for callers that upload once and launch
+// many times (batch work lists, graph replays): the host builds `blob` (job
+// table and workgroup -> job map), the caller copies it to device memory,
+// and fft16_jobs_launch launches the kernel over that copy.
+struct Fft16JobsPlan {
+  std::vector<uint8_t> blob;
+  size_t map_offset = 0, lds = 0;
+  uint32_t total = 0, threads = 0;
+  bool byte_planes = false;
+};
+hipError_t fft16_jobs_plan(const std::vector<Fft16JobSpec> &jobs, bool byte_planes,
+                           Fft16JobsPlan *plan);
+hipError_t fft16_jobs_launch(const Fft16JobsPlan &plan, const void *dev_blob, hipStream_t stream);
 
 }  // namespace trs

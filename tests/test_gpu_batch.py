@@ -125,3 +125,47 @@ def test_batch_graph_replays_match_eager(trs):
     torch.cuda.synchronize()
     for shards, full, _ in cases:
         assert (to_host(shards) == full).all()
+
+
+def test_batch_fft_jobs_match_oracle(trs):
+    """Shapes with k > 32 whose staged program or FFT decoder the batch runs
+    on the byte-plane interpreter (one launch for all of them), mixed with
+    matrix-route jobs, eager and as graph replays."""
+    import torch
+
+    rng = np.random.default_rng(33)
+    codecs = {s: trs.ReedSolomon(*s) for s in [(64, 64), (100, 156), (128, 128), (48, 16)]}
+    batch = trs.Batch()
+    cases = []
+    plan = [((64, 64), "enc", 4096), ((64, 64), "enc", 65536 + 5), ((100, 156), "enc", 1000),
+            ((128, 128), "rec", 3000), ((128, 128), "rec", 777), ((48, 16), "enc", 5000),
+            ((64, 64), "rec", 4096), ((128, 128), "rec-data", 2048)]
+    for n, ((k, m), kind, length) in enumerate(plan):
+        data = seeded(k, length, 900 + n)
+        full = np.concatenate([data, O.rs8_encode(O.build_matrix(8, k, k + m), k, data)])
+        if kind == "enc":
+            shards = rows_to_dev(np.concatenate([data, np.zeros((m, length), np.uint8)]))
+            batch.add_encode(codecs[(k, m)], shards)
+            wipe, check = list(range(k, k + m)), list(range(k + m))
+        else:
+            e = 120 if n % 2 else 10  # 120 of 256: past the batched FFT-decoder threshold
+            wipe = rng.choice(k + m, e, replace=False).tolist()
+            cur = full.copy()
+            cur[wipe] = 0
+            shards = rows_to_dev(cur)
+            data_only = kind == "rec-data"
+            batch.add_reconstruct(codecs[(k, m)], shards, [i not in wipe for i in range(k + m)],
+                                  data_only=data_only)
+            check = [i for i in range(k + m) if not (data_only and i >= k and i in wipe)]
+        cases.append((shards, full, wipe, check))
+    batch.prepare()
+    for graph in (False, True, True):
+        batch.set_graph(graph)
+        for shards, _, wipe, _ in cases:
+            for i in wipe:
+                shards[i].zero_()
+        batch.run()
+        torch.cuda.synchronize()
+        for shards, full, _, check in cases:
+            got = to_host(shards)
+            assert (got[check] == full[check]).all(), graph
