@@ -190,6 +190,89 @@ def host_leg(trs, torch, O, L: int = 32 << 20, reps: int = 3):
     return out
 
 
+def wide_leg(trs, torch, reps: int = 20):
+    """GF(2^8) shapes with k > 32 (SURVEY (f) 4 and (f) 1 over GF(2^8)): per
+    call, the staged FFT program (src/fft.rs:393-592) against the matrix route
+    for encode, and the FFT erasure decoder against the matrix plans for a
+    reconstruct of m erasures; batched, a work list of such stripes on the
+    interpreter list.  Each pair of routes must produce the same bytes."""
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out, exact = {}, True
+    for (k, m), L in (((64, 64), 64 << 10), ((200, 55), 64 << 10), ((128, 128), 1 << 20)):
+        rs = trs.ReedSolomon(k, m)
+        buf = torch.empty((k + m, L), dtype=torch.uint8, device="cuda")
+        trs.fill_random(buf[:k].reshape(-1), SEED ^ (k << 8) ^ m)
+        rows = [buf[i] for i in range(k + m)]
+        enc = {}
+        for route in ("table", "staged-fft"):
+            ms = timed(lambda: rs.encode_sep_route(route, rows[:k], rows[k:]))
+            enc[route] = (ms, buf[k:].clone())
+        exact = exact and bool(torch.equal(enc["table"][1], enc["staged-fft"][1]))
+        full = buf.clone()
+        erased = list(range(0, k + m, (k + m) // m))[:m]
+        present = [i not in erased for i in range(k + m)]
+        rec = {}
+        lib = trs._lib.lib
+        p, l = trs._ptrs(rows)
+        flags = trs._flags(present)
+        for route in ("matrix", "fft"):
+            rs.set_decode_route(route)
+            dec = rs.prepare_decode(present)
+
+            def run():
+                st = lib.trs_dec8_reconstruct(dec._h, p, l, flags, k + m, 0, trs._stream())
+                assert st == 0, st
+
+            rec[route] = timed(run)
+            for i in erased:
+                rows[i].zero_()
+            run()
+            torch.cuda.synchronize()
+            exact = exact and bool(torch.equal(buf, full))
+        rs.set_decode_route("auto")
+        mb = (k + m) * L
+        out[f"{k}_{m}_{L >> 10}KiB"] = {
+            "encode_gbs": {"matrix": round(mb / (enc["table"][0] * 1e6), 1),
+                           "staged_fft": round(mb / (enc["staged-fft"][0] * 1e6), 1)},
+            "encode_route_auto": rs.encode_route(L),
+            "encode_route_reference": rs.reference_route(L),
+            "reconstruct_gbs": {"matrix": round((k + m) * L / (rec["matrix"] * 1e6), 1),
+                                "fft_decoder": round((k + m) * L / (rec["fft"] * 1e6), 1)},
+        }
+        del buf, rows, full
+        torch.cuda.empty_cache()
+    # batched: 256 stripes of (64,64) x 64 KiB through the interpreter work list
+    k, m, L, S = 64, 64, 64 << 10, 256
+    rs = trs.ReedSolomon(k, m)
+    buf = torch.empty((S, k + m, L), dtype=torch.uint8, device="cuda")
+    trs.fill_random(buf[:, :k].reshape(-1), SEED ^ 0x51DE)
+    b = trs.Batch()
+    for st in range(S):
+        b.add_encode(rs, [buf[st, i] for i in range(k + m)])
+    b.prepare()
+    b.set_graph(True)
+    ms = timed(b.run)
+    one = [buf[7, i].clone() for i in range(k + m)]
+    rs.encode_sep_route("table", one[:k], one[k:])
+    exact = exact and all(bool(torch.equal(one[i], buf[7, i])) for i in range(k, k + m))
+    out["batched_64_64_x256_64KiB_encode_gbs"] = round(S * (k + m) * L / (ms * 1e6), 1)
+    out["byte_exact"] = exact
+    del buf, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def c1_leg(trs, torch, O, stripes: int = 256, L: int = 65536):
     """RS8 (10,4), `stripes` stripes of 14 x L bytes resident in HBM, seeded
     bytes: batched encode, then per stripe erase 4 of 14 shards chosen
@@ -572,6 +655,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=800)
     ap.add_argument("--no-tower", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-wide", action="store_true",
+                    help="skip the k > 32 leg (staged FFT and FFT decoder vs matrix)")
     ap.add_argument("--no-host", action="store_true",
                     help="skip the host-resident (PCIe-inclusive) leg")
     ap.add_argument("--counters", action="store_true",
@@ -750,6 +835,9 @@ def main() -> int:
     host = None
     if not args.no_host:
         host = host_leg(trs, torch, O)
+    wide = None
+    if not args.no_wide:
+        wide = wide_leg(trs, torch)
 
     served = None
     if not args.no_serve:
@@ -810,6 +898,7 @@ def main() -> int:
         "batched_vs_per_call": round(value / percall_gbs, 2),
         "c1": c1,
         "host_path": host,
+        "wide": wide,
         "tower": tower,
         "served": served,
         "byte_exact": {"batched_reconstruct": rec_exact, "batched_encode": enc_exact,
