@@ -666,7 +666,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "stripe-parallel sharding across GPUs is described in DESIGN.md", file=sys.stderr)
+              "across GPUs the stripes would shard with no exchange, each GPU running this "
+              "path on its share", file=sys.stderr)
         return 2
 
     import torch
