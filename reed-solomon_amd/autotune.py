@@ -118,6 +118,8 @@ def measure(s: Shape, c: Candidate, cap_bytes: int = 32 << 20, reps: int = 2) ->
     route has no kernel for the shape."""
     route, strip = c
     window = max(64, min(s.length, cap_bytes // (s.k + s.m)))
+    if window < s.length:
+        window -= window % 64  # a column window keeps the shard's 16-byte vector accesses
     window -= window % (2 if s.field == 16 else 1)
     window = max(2, window)
     ms = ctypes.c_double(0.0)
