@@ -1632,15 +1632,18 @@ struct trs_batch8 {
   };
   // One device work list per kernel: table-kernel jobs and generated-program jobs.
   struct List {
-    DevBuf jobs, tiles;
+    const void *jobs = nullptr, *tiles = nullptr;  // inside `blob`
     uint32_t njobs = 0;
     uint64_t total_tiles = 0;
     uint32_t grid = 0;
   };
   std::vector<Req> reqs;
-  DevBuf ptrs;
+  // One device buffer for everything prepare uploads (pointer tables, each
+  // list's jobs and tile map, the interpreter job table), so a prepare makes
+  // a single host-to-device copy.
+  DevBuf blob;
   List table, mfma;
-  DevBuf fft_blob;  // job table of the interpreter jobs
+  const void *fft_table = nullptr;  // interpreter job table, inside `blob`
   Fft16JobsPlan fft_plan;
   // Runtime-coefficient jobs run on the bit-sliced tile (k_batch8_planes)
   // unless TRS_BATCH8_TILE=table selects the packed-byte table tile.
@@ -1760,7 +1763,6 @@ int trs_batch8_prepare(trs_batch8 *b) {
   b->drop_graph();
   size_t nptr = 0;
   for (auto &r : b->reqs) nptr += r.ptrs.size();
-  TRS_TRY(b->ptrs.ensure(nptr * sizeof(void *) + 8));
   std::vector<const uint8_t *> hp;
   hp.reserve(nptr);
   const size_t nprog = std::min<size_t>(bs_program_count() + 1, trs_batch8::kMaxPrograms);
@@ -1773,11 +1775,15 @@ int trs_batch8_prepare(trs_batch8 *b) {
   b->planes = true;
   if (const char *e = std::getenv("TRS_BATCH8_TILE")) b->planes = std::strcmp(e, "table") != 0;
   const uint64_t table_chunk = b->planes ? kPlanesChunkBytes : kChunkBytes;
-  const auto *dptr = static_cast<const uint8_t *const *>(b->ptrs.p);
+  // Pointer tables sit at the start of `blob`; until it is allocated, job
+  // pointers hold byte offsets into it (patched below).
+  auto at_offset = [](size_t off) {
+    return reinterpret_cast<const uint8_t *const *>(static_cast<uintptr_t>(off));
+  };
   std::vector<Fft16JobSpec> fjobs;
   for (const auto &r : b->reqs) {
     BatchJob J{};
-    J.ptrs = dptr + hp.size();
+    J.ptrs = at_offset(hp.size() * sizeof(void *));
     bool al = true;
     for (auto q : r.ptrs) {
       hp.push_back(q);
@@ -1814,55 +1820,84 @@ int trs_batch8_prepare(trs_batch8 *b) {
       jt.push_back(J);
     }
   }
-  if (!hp.empty())
-    TRS_TRY(hipMemcpy(b->ptrs.p, hp.data(), hp.size() * sizeof(void *), hipMemcpyHostToDevice));
+  // Layout of `blob`: pointers, then per list its jobs and tile -> job map,
+  // then the interpreter job table; 256-byte aligned pieces.
+  auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+  size_t size = align(nptr * sizeof(void *) + 8);
+  struct Piece {
+    trs_batch8::List *list;
+    std::vector<BatchJob> *jobs;
+    std::vector<uint32_t> map;
+    uint64_t total;
+    uint64_t per_cu;
+    size_t jobs_off = 0, map_off = 0;
+  };
+  uint64_t per_cu_table = 8, per_cu_bs = 4, per_cu_mfma = 2;
+  if (const char *e = std::getenv("TRS_BATCH_GRID_PER_CU")) per_cu_table = std::atoi(e);
+  if (const char *e = std::getenv("TRS_BATCH_BS_GRID_PER_CU")) per_cu_bs = std::atoi(e);
+  if (const char *e = std::getenv("TRS_BATCH_MFMA_GRID_PER_CU")) per_cu_mfma = std::atoi(e);
+  std::vector<Piece> pieces;
+  auto add_piece = [&](trs_batch8::List &L, std::vector<BatchJob> &jobs,
+                       std::vector<uint64_t> &tiles, uint64_t total, uint64_t per_cu) {
+    L.njobs = 0;
+    L.total_tiles = 0;
+    L.grid = 0;
+    L.jobs = L.tiles = nullptr;
+    if (total == 0) return;  // nothing for this kernel
+    Piece pc{&L, &jobs, std::vector<uint32_t>(total, 0), total, per_cu};
+    for (size_t j = 0; j < jobs.size(); ++j) {  // tile -> job map, one uint32 per tile
+      const uint64_t end = j + 1 < jobs.size() ? tiles[j + 1] : total;
+      for (uint64_t t = tiles[j]; t < end; ++t) pc.map[t] = uint32_t(j);
+    }
+    pc.jobs_off = size;
+    size = align(size + jobs.size() * sizeof(BatchJob) + 8);
+    pc.map_off = size;
+    size = align(size + pc.map.size() * sizeof(uint32_t));
+    pieces.push_back(std::move(pc));
+  };
+  add_piece(b->table, jt, tt, tile_t, per_cu_table);
+  for (size_t p = 1; p < nprog; ++p) add_piece(b->bs[p], jb[p], tb[p], tile_b[p], per_cu_bs);
+  add_piece(b->mfma, jm, tm, tile_m, per_cu_mfma);
+  // the interpreter table's size does not depend on the pointer values
   TRS_TRY(fft16_jobs_plan(fjobs, true, &b->fft_plan));
-  if (b->fft_plan.total) {
-    TRS_TRY(b->fft_blob.ensure(b->fft_plan.blob.size()));
-    TRS_TRY(hipMemcpy(b->fft_blob.p, b->fft_plan.blob.data(), b->fft_plan.blob.size(),
-                      hipMemcpyHostToDevice));
+  const size_t fft_off = size;
+  size = align(size + b->fft_plan.blob.size());
+  TRS_TRY(b->blob.ensure(size));
+  const uintptr_t base = reinterpret_cast<uintptr_t>(b->blob.p);
+  auto patch = [base](const uint8_t *const *q) {
+    return reinterpret_cast<const uint8_t *const *>(base + reinterpret_cast<uintptr_t>(q));
+  };
+  std::vector<uint8_t> host(size, 0);
+  if (!hp.empty()) std::memcpy(host.data(), hp.data(), hp.size() * sizeof(void *));
+  for (Piece &pc : pieces) {
+    for (BatchJob &J : *pc.jobs) J.ptrs = patch(J.ptrs);
+    if (!pc.jobs->empty())
+      std::memcpy(host.data() + pc.jobs_off, pc.jobs->data(), pc.jobs->size() * sizeof(BatchJob));
+    std::memcpy(host.data() + pc.map_off, pc.map.data(), pc.map.size() * sizeof(uint32_t));
   }
+  b->fft_table = nullptr;
+  if (!fjobs.empty()) {
+    for (Fft16JobSpec &f : fjobs) f.ptrs = patch(f.ptrs);
+    TRS_TRY(fft16_jobs_plan(fjobs, true, &b->fft_plan));
+    if (b->fft_plan.total) {
+      std::memcpy(host.data() + fft_off, b->fft_plan.blob.data(), b->fft_plan.blob.size());
+      b->fft_table = reinterpret_cast<const void *>(base + fft_off);
+    }
+  }
+  TRS_TRY(hipMemcpy(b->blob.p, host.data(), size, hipMemcpyHostToDevice));
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  auto upload = [&](trs_batch8::List &L, std::vector<BatchJob> &jobs, std::vector<uint64_t> &tiles,
-                    uint64_t total, uint64_t per_cu) -> int {
-    if (total == 0) {  // nothing for this kernel: no copies (each is a host round trip)
-      L.njobs = 0;
-      L.total_tiles = 0;
-      L.grid = 0;
-      return TRS_OK;
-    }
-    // tile -> job map, one uint32 per tile
-    std::vector<uint32_t> tj(total, 0);
-    for (size_t j = 0; j < jobs.size(); ++j) {
-      const uint64_t end = j + 1 < jobs.size() ? tiles[j + 1] : total;
-      for (uint64_t t = tiles[j]; t < end; ++t) tj[t] = uint32_t(j);
-    }
-    TRS_TRY(L.jobs.ensure(jobs.size() * sizeof(BatchJob) + 8));
-    TRS_TRY(L.tiles.ensure(tj.size() * sizeof(uint32_t)));
-    if (!jobs.empty())
-      TRS_TRY(hipMemcpy(L.jobs.p, jobs.data(), jobs.size() * sizeof(BatchJob),
-                        hipMemcpyHostToDevice));
-    TRS_TRY(hipMemcpy(L.tiles.p, tj.data(), tj.size() * sizeof(uint32_t),
-                      hipMemcpyHostToDevice));
-    L.njobs = uint32_t(jobs.size());
-    L.total_tiles = total;
-    uint64_t grid = uint64_t(cus) * per_cu;
-    if (grid > total) grid = total;
+  for (Piece &pc : pieces) {
+    trs_batch8::List &L = *pc.list;
+    L.jobs = reinterpret_cast<const void *>(base + pc.jobs_off);
+    L.tiles = reinterpret_cast<const void *>(base + pc.map_off);
+    L.njobs = uint32_t(pc.jobs->size());
+    L.total_tiles = pc.total;
+    uint64_t grid = uint64_t(cus) * pc.per_cu;
+    if (grid > pc.total) grid = pc.total;
     L.grid = uint32_t(grid ? grid : 1);
-    return TRS_OK;
-  };
-  uint64_t per_cu_table = 8, per_cu_bs = 4;
-  if (const char *e = std::getenv("TRS_BATCH_GRID_PER_CU")) per_cu_table = std::atoi(e);
-  if (const char *e = std::getenv("TRS_BATCH_BS_GRID_PER_CU")) per_cu_bs = std::atoi(e);
-  int st = upload(b->table, jt, tt, tile_t, per_cu_table);
-  if (st) return st;
-  for (size_t p = 1; p < nprog; ++p)
-    if ((st = upload(b->bs[p], jb[p], tb[p], tile_b[p], per_cu_bs))) return st;
-  uint64_t per_cu_mfma = 2;
-  if (const char *e = std::getenv("TRS_BATCH_MFMA_GRID_PER_CU")) per_cu_mfma = std::atoi(e);
-  if ((st = upload(b->mfma, jm, tm, tile_m, per_cu_mfma))) return st;
+  }
   b->prepared = true;
   return TRS_OK;
 }
@@ -1872,25 +1907,25 @@ int batch8_launch(trs_batch8 *b, hipStream_t s) {
   for (size_t p = 1; p < trs_batch8::kMaxPrograms; ++p) {
     const trs_batch8::List &L = b->bs[p];
     if (!L.total_tiles) continue;
-    const int st = hip_status(launch_batch_bs(uint32_t(p), static_cast<const BatchJob *>(L.jobs.p),
-                                              static_cast<const uint32_t *>(L.tiles.p),
+    const int st = hip_status(launch_batch_bs(uint32_t(p), static_cast<const BatchJob *>(L.jobs),
+                                              static_cast<const uint32_t *>(L.tiles),
                                               L.total_tiles, L.grid, s));
     if (st) return st;
   }
   if (b->mfma.total_tiles) {
-    const int st = hip_status(launch_batch_mfma(static_cast<const BatchJob *>(b->mfma.jobs.p),
-                                                static_cast<const uint32_t *>(b->mfma.tiles.p),
+    const int st = hip_status(launch_batch_mfma(static_cast<const BatchJob *>(b->mfma.jobs),
+                                                static_cast<const uint32_t *>(b->mfma.tiles),
                                                 b->mfma.total_tiles, b->mfma.grid, s));
     if (st) return st;
   }
   if (b->fft_plan.total) {
-    const int st = hip_status(fft16_jobs_launch(b->fft_plan, b->fft_blob.p, s));
+    const int st = hip_status(fft16_jobs_launch(b->fft_plan, b->fft_table, s));
     if (st) return st;
   }
   if (b->table.total_tiles)
     return hip_status((b->planes ? launch_batch8_planes : launch_batch8)(
-        static_cast<const BatchJob *>(b->table.jobs.p),
-        static_cast<const uint32_t *>(b->table.tiles.p), b->table.total_tiles, b->table.grid, s));
+        static_cast<const BatchJob *>(b->table.jobs),
+        static_cast<const uint32_t *>(b->table.tiles), b->table.total_tiles, b->table.grid, s));
   return TRS_OK;
 }
 }  // namespace
