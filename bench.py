@@ -168,10 +168,10 @@ def host_leg(trs, torch, O, L: int = 32 << 20, reps: int = 3):
             rows = host = [want[i].copy() for i in range(k + m)]
         for j in range(k, k + m):
             host[j][:] = 0
-        rs.encode_host(rows[:k], rows[k:])
+        rs.encode_sep(rows[:k], rows[k:])
         t0 = time.perf_counter()
         for _ in range(reps):
-            rs.encode_host(rows[:k], rows[k:])
+            rs.encode_sep(rows[:k], rows[k:])
         enc_s = (time.perf_counter() - t0) / reps
         exact = all((host[j] == want[j]).all() for j in range(k, k + m))
         erased = [0, 3, 11, 13]
@@ -180,7 +180,7 @@ def host_leg(trs, torch, O, L: int = 32 << 20, reps: int = 3):
         for _ in range(reps):
             for i in erased:
                 host[i][:] = 0
-            rs.reconstruct_host(rows, present)
+            rs.reconstruct([(r, p) for r, p in zip(rows, present)])
         rec_s = (time.perf_counter() - t0) / reps
         exact = exact and all((host[i] == want[i]).all() for i in range(k + m))
         out[kind] = {"encode_gbs": round((k + m) * L / enc_s / 1e9, 2),
@@ -781,7 +781,20 @@ def main() -> int:
     value = (enc_bytes + rec_bytes) / (ms_step * 1e-3) / 1e9
     # dominant kernel: the batched reconstruct launch (k_batch8_planes)
     achieved = rec_bytes / (rec_ms * 1e-3) / 1e9
+    # The timed replays rewrite bytes that are already correct, so check that
+    # the same graphs really write: outside the timed region, wipe all parity
+    # and replay the encode graph, then erase and replay the reconstruct graph.
+    torch.cuda.synchronize()
+    for s, (k, m, L, _) in enumerate(meta):
+        o = int(offsets[s])
+        buf[o + k * L:o + (k + m) * L].zero_()
+    batch_enc.run()
+    torch.cuda.synchronize()
     exact_after = bool(torch.equal(buf, pristine))
+    erase()
+    batch_rec.run()
+    torch.cuda.synchronize()
+    exact_after = exact_after and bool(torch.equal(buf, pristine))
 
     # ---- plain per-call GPU path: one library call per stripe
     def percall(kind: str):

@@ -5,9 +5,11 @@
  * point names the reference method it replaces (file:line in the reference
  * crate).  Conventions shared by every call:
  *
- *  - Shard buffers are DEVICE pointers (HBM).  A shard set is a pointer array
- *    plus a parallel array of buffer lengths in bytes, so the reference's
- *    length checks (and their error order) are kept.
+ *  - Shard buffers are DEVICE pointers (HBM), except in the *_host entry
+ *    points, which take the same arguments over HOST memory (the reference's
+ *    slices).  A shard set is a pointer array plus a parallel array of buffer
+ *    lengths in bytes, so the reference's length checks (and their error
+ *    order) are kept.
  *  - `stream` is a hipStream_t passed as void*; NULL means the default stream.
  *    Work is enqueued on that stream and the call returns; results are ready
  *    once the stream reaches that point.  trs_*_verify waits for its answer.
@@ -294,7 +296,8 @@ void trs_dec16_free(trs_dec16 *dec);
 /* The reference is one-call-one-stripe and leaves batching to its callers
  * (BENCH-RESULTS.md:98-100).  A batch collects many encode / reconstruct
  * requests on GF(2^8) codecs, of mixed shapes, lengths and erasure patterns,
- * and runs them all in one persistent kernel launch over a device work list.
+ * and runs them with one kernel launch per device work list (generated
+ * program, MFMA, runtime-coefficient tile, interpreter).
  * Each request is validated exactly as the single-stripe call would be, and
  * each result is byte-identical to that call's. */
 typedef struct trs_batch8 trs_batch8;
@@ -315,10 +318,13 @@ int trs_batch8_add_reconstruct(trs_batch8 *b, const trs_rs8 *rs, uint8_t *const 
  * Requests whose shape has a generated program run it (one kernel per
  * program); the others run the bit-sliced runtime-coefficient kernel
  * (8 KiB tiles), or the packed-byte table kernel when the environment sets
- * TRS_BATCH8_TILE=table.  TRS_BATCH_GRID_PER_CU / TRS_BATCH_BS_GRID_PER_CU
- * set the persistent grids' workgroups per CU (default 8 / 4). */
+ * TRS_BATCH8_TILE=table.  TRS_BATCH_GRID_PER_CU / TRS_BATCH_BS_GRID_PER_CU /
+ * TRS_BATCH_MFMA_GRID_PER_CU cap each list's grid at that many workgroups per
+ * CU (default 128 / 128 / 2; the first two amount to one workgroup per tile).
+ * Prepare first waits for earlier runs of this batch to finish, since they
+ * read the device tables it rewrites. */
 int trs_batch8_prepare(trs_batch8 *b);
-/* Run every queued request in one launch on `stream`. */
+/* Run every queued request on `stream`: one kernel per non-empty work list. */
 int trs_batch8_run(trs_batch8 *b, void *stream);
 /* Graph mode: the first run after prepare captures the batch's launches into
  * a hipGraph (kernel nodes only, on a private stream) and every run replays
@@ -361,24 +367,84 @@ int trs_server_drain(trs_server *s);
 int trs_server_take(trs_server *s, uint64_t ticket, double *latency_us, uint64_t *bytes,
                     int *status);
 
-/* ============ host-side data path and wire formats (no reference ============
- * counterpart: the reference's shards are host slices) */
-/* ReedSolomon::encode_sep (src/reedsolomon.rs:284-309) for host-resident
- * shards: data (k pointers) and parity (m pointers) in host memory, `len`
- * bytes each.  The shards stream through device memory in chunks of `chunk`
- * bytes per shard (0 = 1 MiB): the host-to-device copy of one chunk, the
- * encode of the previous one and the device-to-host copy of the one before
- * run on three streams at once.  Pinned caller memory (hipHostMalloc) is
- * copied directly; pageable memory goes through library-owned pinned staging.
- * Blocks until the parity is in host memory. */
-int trs_rs8_encode_host(const trs_rs8 *rs, const uint8_t *const *data, uint8_t *const *parity,
-                        size_t len, size_t chunk);
-/* ReedSolomon::reconstruct(_data) (src/reedsolomon.rs:470-477) for
- * host-resident shards with presence flags, pipelined as above: only the first
- * k present shards travel to the device and only the rebuilt shards come
- * back.  Error order as trs_rs8_reconstruct. */
-int trs_rs8_reconstruct_host(const trs_rs8 *rs, uint8_t *const *shards, const uint8_t *present,
-                             size_t n, size_t len, int data_only, size_t chunk);
+/* ============ host-resident shards: the reference's slice API ============== */
+/* The reference's methods take host slices (src/reedsolomon.rs:271-535,
+ * src/reedsolomon16.rs:412-565).  Each _host entry point below takes the same
+ * arguments as its device counterpart above (minus the stream), keeps the same
+ * checks, error order and codes, and takes HOST pointers: pageable memory, or
+ * pinned memory from hipHostMalloc.  The shards stream through device memory
+ * in column chunks (trs_set_host_chunk): the host-to-device copy of one chunk,
+ * the kernels of the previous one and the device-to-host copy of the one
+ * before run on three library streams at once.  Only the shards a call reads
+ * go to the device (for a reconstruct: the first k present) and only the
+ * shards it writes come back.  Every call blocks until its outputs are in host
+ * memory.  A device pointer among the shards returns TRS_ERR_INVALID_ARGUMENT.
+ * RS16 shards keep the split-plane wire layout: each chunk carries the same
+ * column range of the low and the high plane. */
+/* ReedSolomon::encode, src/reedsolomon.rs:271-279. */
+int trs_rs8_encode_host(const trs_rs8 *rs, uint8_t *const *shards, const size_t *lens,
+                        size_t n_shards);
+/* ReedSolomon::encode_sep, src/reedsolomon.rs:284-309. */
+int trs_rs8_encode_sep_host(const trs_rs8 *rs, const uint8_t *const *data, const size_t *data_lens,
+                            size_t n_data, uint8_t *const *parity, const size_t *parity_lens,
+                            size_t n_parity);
+/* ReedSolomon::encode_rows_into, src/reedsolomon.rs:667-682. */
+int trs_rs8_encode_rows_into_host(const trs_rs8 *rs, const uint8_t *data, size_t data_len,
+                                  size_t row_len, uint8_t *parity, size_t parity_len);
+/* ReedSolomon::verify, src/reedsolomon.rs:430-463. */
+int trs_rs8_verify_host(const trs_rs8 *rs, const uint8_t *const *shards, const size_t *lens,
+                        size_t n_shards, int *ok);
+/* ReedSolomon::reconstruct / reconstruct_data, src/reedsolomon.rs:470-477. */
+int trs_rs8_reconstruct_host(const trs_rs8 *rs, uint8_t *const *shards, const size_t *lens,
+                             const uint8_t *present, size_t n_shards, int data_only);
+/* ReedSolomon::reconstruct_rows, src/reedsolomon.rs:514-535. */
+int trs_rs8_reconstruct_rows_host(const trs_rs8 *rs, uint8_t *rows, size_t rows_len,
+                                  size_t row_len, const uint8_t *present, size_t n_present);
+/* PreparedDecoder::{reconstruct, reconstruct_data}, src/reedsolomon.rs:762-797. */
+int trs_dec8_reconstruct_host(const trs_dec8 *dec, uint8_t *const *shards, const size_t *lens,
+                              const uint8_t *present, size_t n_shards, int data_only);
+/* ReedSolomon16::encode, src/reedsolomon16.rs:412-418. */
+int trs_rs16_encode_host(const trs_rs16 *rs, uint8_t *const *shards, const size_t *lens,
+                         size_t n_shards);
+/* ReedSolomon16::encode_sep, src/reedsolomon16.rs:422-447. */
+int trs_rs16_encode_sep_host(const trs_rs16 *rs, const uint8_t *const *data,
+                             const size_t *data_lens, size_t n_data, uint8_t *const *parity,
+                             const size_t *parity_lens, size_t n_parity);
+/* ReedSolomon16::verify, src/reedsolomon16.rs:453-505. */
+int trs_rs16_verify_host(const trs_rs16 *rs, const uint8_t *const *shards, const size_t *lens,
+                         size_t n_shards, int *ok);
+/* ReedSolomon16::reconstruct / reconstruct_data, src/reedsolomon16.rs:508-515. */
+int trs_rs16_reconstruct_host(const trs_rs16 *rs, uint8_t *const *shards, const size_t *lens,
+                              const uint8_t *present, size_t n_shards, int data_only);
+/* ReedSolomon16::reconstruct_rows, src/reedsolomon16.rs:546-569. */
+int trs_rs16_reconstruct_rows_host(const trs_rs16 *rs, uint8_t *rows, size_t rows_len,
+                                   size_t row_len, const uint8_t *present, size_t n_present);
+/* PreparedDecoder16::{reconstruct, reconstruct_data}, src/reedsolomon16.rs:722-753. */
+int trs_dec16_reconstruct_host(const trs_dec16 *dec, uint8_t *const *shards, const size_t *lens,
+                               const uint8_t *present, size_t n_shards, int data_only);
+/* Bytes of each shard per pipeline chunk of the _host calls (0 = 1 MiB, the
+ * default); a chunk of all of a call's shards is further capped at 64 MiB.
+ * Process-wide.  No reference counterpart (the reference tunes strips by hand,
+ * src/reedsolomon16.rs:796-804). */
+int trs_set_host_chunk(size_t bytes);
+
+/* Clone, src/reedsolomon.rs:725-739 and src/reedsolomon16.rs:685: a new codec
+ * of the same shape and routes whose plan cache starts as a copy of this
+ * one's (the cached plans are shared, they never change).  Free with
+ * trs_rs8_free / trs_rs16_free. */
+int trs_rs8_clone(const trs_rs8 *rs, trs_rs8 **out);
+int trs_rs16_clone(const trs_rs16 *rs, trs_rs16 **out);
+
+/* ============ device memory for callers without a HIP runtime of their own == */
+/* No reference counterpart (the reference's shards are host memory). */
+int trs_device_alloc(size_t bytes, void **out);
+int trs_device_free(void *p);
+/* Stream-ordered copy between any host and device buffers (direction taken
+ * from the pointers); pageable host memory makes it synchronous. */
+int trs_memcpy(void *dst, const void *src, size_t bytes, void *stream);
+int trs_stream_synchronize(void *stream);
+
+/* ============ wire formats and Option<T> shards (no reference counterpart) === */
 /* ReedSolomon::reconstruct with Option<T> shards (src/reedsolomon.rs:88-101,
  * :842-847): a NULL entry of `shards` is a missing shard that the codec
  * allocates in device memory (the present shards' length) and rebuilds; on
@@ -387,7 +453,6 @@ int trs_rs8_reconstruct_host(const trs_rs8 *rs, uint8_t *const *shards, const ui
  * the buffers with trs_device_free.  On an error nothing stays allocated. */
 int trs_rs8_reconstruct_alloc(const trs_rs8 *rs, uint8_t **shards, size_t *lens, size_t n,
                               int data_only, void *stream);
-int trs_device_free(void *p);
 /* Wire formats of a GF(2^16) shard (src/reedsolomon16.rs:4-7): n_symbols
  * interleaved little-endian u16 symbols (value (high << 8) | low) <-> the
  * split-plane layout [low n_symbols | high n_symbols] that the RS16 calls

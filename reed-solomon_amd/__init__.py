@@ -1,10 +1,13 @@
 """MI355X (gfx950) backend for tape-reed-solomon's bulk coding path.
 
 Python mirror of the reference crate's codec API over the C ABI of
-``libtrs_gpu.so`` (include/trs_gpu.h).  Shards are ``torch.uint8`` CUDA
-tensors (device memory); every byte of coding work runs in the library's HIP
-kernels.  Work is enqueued on torch's current stream; ``verify`` returns once
-its answer is known.
+``libtrs_gpu.so`` (include/trs_gpu.h).  Shards are either ``torch.uint8`` CUDA
+tensors (device memory: work is enqueued on torch's current stream) or host
+buffers -- uint8 numpy arrays or CPU tensors, like the reference's slices --
+which the library streams through the device (the ``*_host`` entry points);
+a host call returns with its outputs in host memory.  Every byte of coding
+work runs in the library's HIP kernels; ``verify`` returns once its answer is
+known.
 
 Reference surface kept (tape-reed-solomon 0.2.2):
 
@@ -45,6 +48,7 @@ __all__ = [
     "mul_slice",
     "mul_slice_xor",
     "fill_random",
+    "set_host_chunk",
     "version",
 ]
 
@@ -129,53 +133,128 @@ def _flags(flags: Sequence[bool]):
     return a
 
 
-def _normalise_reconstruct(shards: list, data_only: bool, k: int):
-    """Turns the reference's containers into (tensors, present flags).
+def _is_host(buf) -> bool:
+    """numpy arrays and CPU tensors are host shards; CUDA tensors are device
+    shards."""
+    import torch
 
-    ``(tensor, bool)`` entries pass through.  ``None`` entries are missing
-    Option shards: they get a zeroed tensor of the shared length, written back
-    into the list (get_or_initialize, src/reedsolomon.rs:88-101), except parity
-    entries of a data-only reconstruct, which stay ``None``.
+    if isinstance(buf, torch.Tensor):
+        return not buf.is_cuda
+    if isinstance(buf, np.ndarray):
+        return True
+    raise TypeError("shards must be uint8 numpy arrays or torch tensors")
+
+
+def _on_host(bufs) -> bool:
+    """Whether a shard list lives in host memory.  Host and device buffers
+    cannot be mixed in one call."""
+    kinds = {_is_host(b) for b in bufs if b is not None}
+    if len(kinds) > 1:
+        raise TypeError("a call's shards must all be host buffers or all device tensors")
+    return kinds == {True}
+
+
+def _host_ptr(buf) -> tuple:
+    """(address, length) of a contiguous host uint8 buffer."""
+    import torch
+
+    if isinstance(buf, torch.Tensor):
+        if buf.dtype != torch.uint8 or not buf.is_contiguous():
+            raise TypeError("host shards must be contiguous uint8 tensors or arrays")
+        return buf.data_ptr(), buf.numel()
+    if buf.dtype != np.uint8 or not buf.flags["C_CONTIGUOUS"]:
+        raise TypeError("host shards must be contiguous uint8 arrays")
+    if not buf.flags["WRITEABLE"]:
+        raise TypeError("host shards must be writeable")
+    return buf.ctypes.data, buf.size
+
+
+def _hptrs(bufs: Sequence) -> tuple:
+    """Pointer and length arrays of host shards (the _host entry points)."""
+    arr = (ctypes.c_void_p * max(1, len(bufs)))()
+    lens = (ctypes.c_size_t * max(1, len(bufs)))()
+    for i, b in enumerate(bufs):
+        arr[i], lens[i] = _host_ptr(b)
+    return arr, lens
+
+
+def _buf_ptr(buf, host: bool):
+    return ctypes.c_void_p(_host_ptr(buf)[0] if host else _dev_ptr(buf))
+
+
+def _numel(buf) -> int:
+    return buf.size if isinstance(buf, np.ndarray) else buf.numel()
+
+
+def set_host_chunk(nbytes: int = 0) -> None:
+    """Bytes of each shard per pipeline chunk of host-memory calls (0 = 1 MiB)."""
+    _check(lib.trs_set_host_chunk(nbytes))
+
+
+def _normalise_reconstruct(shards: list, data_only: bool, k: int):
+    """Turns the reference's containers into (buffers, present flags, filled
+    indices, host).
+
+    ``(buffer, bool)`` entries pass through.  ``None`` entries are missing
+    Option shards: they get a zeroed buffer of the shared length, in the same
+    memory as the present shards, written back into the list
+    (get_or_initialize, src/reedsolomon.rs:88-101), except parity entries of a
+    data-only reconstruct, which stay ``None``.
     """
     import torch
 
-    tensors, present, fill = [], [], []
+    bufs, present, fill = [], [], []
     length, device = None, None
     for i, s in enumerate(shards):
         if isinstance(s, tuple):
             t, p = s
-            tensors.append(t)
+            bufs.append(t)
             present.append(bool(p))
             if p and length is None:
-                length, device = t.numel(), t.device
+                length, device = _numel(t), t
         elif s is None:
-            tensors.append(None)
+            bufs.append(None)
             present.append(False)
             fill.append(i)
         else:
-            tensors.append(s)
+            bufs.append(s)
             present.append(True)
             if length is None:
-                length, device = s.numel(), s.device
+                length, device = _numel(s), s
+    host = _on_host(bufs)
     if fill:
         if length is None:
-            return None, present, fill
+            return None, present, fill, host
+
+        def zeros():
+            if host:
+                return np.zeros(length, np.uint8)
+            return torch.zeros(length, dtype=torch.uint8, device=device.device)
+
         dummy = None
         for i in fill:
             if data_only and i >= k:
                 # never read or written: a data-only reconstruct skips missing parity
                 if dummy is None:
-                    dummy = torch.zeros(length, dtype=torch.uint8, device=device)
-                tensors[i] = dummy
+                    dummy = zeros()
+                bufs[i] = dummy
             else:
-                tensors[i] = torch.zeros(length, dtype=torch.uint8, device=device)
-    return tensors, present, fill
+                bufs[i] = zeros()
+    return bufs, present, fill, host
 
 
 class _Codec:
+    """Shards may be CUDA tensors (device memory: the work is enqueued on
+    torch's current stream) or host buffers -- uint8 numpy arrays or CPU
+    tensors, the reference's slices -- which stream through the device and are
+    complete when the call returns.  One call takes one kind."""
+
     _fam = ""
 
-    def __init__(self, data_shards: int, parity_shards: int):
+    def __init__(self, data_shards: int, parity_shards: int, _handle=None):
+        if _handle is not None:
+            self._h = _handle
+            return
         h = ctypes.c_void_p()
         _check(getattr(lib, f"trs_{self._fam}_new")(data_shards, parity_shards, ctypes.byref(h)))
         self._h = h
@@ -188,6 +267,18 @@ class _Codec:
             except Exception:  # interpreter shutdown
                 pass
             self._h = None
+
+    def _fn(self, name: str, host: bool):
+        return getattr(lib, f"trs_{self._fam}_{name}{'_host' if host else ''}")
+
+    def clone(self):
+        """Clone (src/reedsolomon.rs:725-739): same shape and routes, a copy of
+        the plan cache."""
+        h = ctypes.c_void_p()
+        _check(getattr(lib, f"trs_{self._fam}_clone")(self._h, ctypes.byref(h)))
+        return type(self)(0, 0, _handle=h)
+
+    __copy__ = clone
 
     def data_shard_count(self) -> int:
         return getattr(lib, f"trs_{self._fam}_data_shard_count")(self._h)
@@ -205,37 +296,52 @@ class _Codec:
 
     def encode(self, shards: Sequence) -> None:
         """Parity shards (the last m of ``shards``) are overwritten."""
+        if _on_host(shards):
+            p, l = _hptrs(shards)
+            _check(self._fn("encode", True)(self._h, p, l, len(shards)))
+            return
         p, l = _ptrs(shards)
-        _check(getattr(lib, f"trs_{self._fam}_encode")(self._h, p, l, len(shards), _stream()))
+        _check(self._fn("encode", False)(self._h, p, l, len(shards), _stream()))
 
     def encode_sep(self, data: Sequence, parity: Sequence) -> None:
-        dp, dl = _ptrs(data)
-        pp, pl = _ptrs(parity)
-        _check(getattr(lib, f"trs_{self._fam}_encode_sep")(self._h, dp, dl, len(data), pp, pl,
-                                                            len(parity), _stream()))
+        host = _on_host(list(data) + list(parity))
+        conv = _hptrs if host else _ptrs
+        dp, dl = conv(data)
+        pp, pl = conv(parity)
+        args = (self._h, dp, dl, len(data), pp, pl, len(parity))
+        _check(self._fn("encode_sep", host)(*(args if host else args + (_stream(),))))
 
     def verify(self, shards: Sequence) -> bool:
-        p, l = _ptrs(shards)
         ok = ctypes.c_int(0)
-        _check(getattr(lib, f"trs_{self._fam}_verify")(self._h, p, l, len(shards),
-                                                        ctypes.byref(ok), _stream()))
+        if _on_host(shards):
+            p, l = _hptrs(shards)
+            _check(self._fn("verify", True)(self._h, p, l, len(shards), ctypes.byref(ok)))
+        else:
+            p, l = _ptrs(shards)
+            _check(self._fn("verify", False)(self._h, p, l, len(shards), ctypes.byref(ok),
+                                             _stream()))
         return bool(ok.value)
 
-    def _reconstruct(self, shards: list, data_only: bool) -> None:
-        tensors, present, filled = _normalise_reconstruct(shards, data_only, self.data_shard_count())
+    def _call_reconstruct(self, handle, fam: str, shards: list, data_only: bool,
+                          all_missing_error: int) -> None:
+        k = self.data_shard_count()
+        bufs, present, filled, host = _normalise_reconstruct(shards, data_only, k)
         n = len(shards)
-        if tensors is None:  # nothing present at all
+        if bufs is None:  # nothing present at all
             total = self.total_shard_count()
             if n != total:
                 raise Error(1 if n < total else 2)
-            raise Error(10)
-        p, l = _ptrs(tensors)
-        _check(getattr(lib, f"trs_{self._fam}_reconstruct")(
-            self._h, p, l, _flags(present), n, int(data_only), _stream()))
-        k = self.data_shard_count()
+            raise Error(all_missing_error)
+        p, l = (_hptrs if host else _ptrs)(bufs)
+        fn = getattr(lib, f"trs_{fam}_reconstruct{'_host' if host else ''}")
+        args = (handle, p, l, _flags(present), n, int(data_only))
+        _check(fn(*(args if host else args + (_stream(),))))
         for i in filled:
             if not (data_only and i >= k):
-                shards[i] = tensors[i]
+                shards[i] = bufs[i]
+
+    def _reconstruct(self, shards: list, data_only: bool) -> None:
+        self._call_reconstruct(self._h, self._fam, shards, data_only, 10)
 
     def reconstruct(self, shards: list) -> None:
         self._reconstruct(shards, False)
@@ -244,9 +350,10 @@ class _Codec:
         self._reconstruct(shards, True)
 
     def reconstruct_rows(self, rows, row_len: int, present: Sequence[bool]) -> None:
-        _check(getattr(lib, f"trs_{self._fam}_reconstruct_rows")(
-            self._h, ctypes.c_void_p(_dev_ptr(rows)), rows.numel(), row_len, _flags(present),
-            len(present), _stream()))
+        host = _is_host(rows)
+        args = (self._h, _buf_ptr(rows, host), _numel(rows), row_len, _flags(present),
+                len(present))
+        _check(self._fn("reconstruct_rows", host)(*(args if host else args + (_stream(),))))
 
     def prepare_decode(self, present: Sequence[bool]):
         h = ctypes.c_void_p()
@@ -273,18 +380,7 @@ class _Decoder:
             self._h = None
 
     def _reconstruct(self, shards: list, data_only: bool) -> None:
-        tensors, present, filled = _normalise_reconstruct(shards, data_only, self._k)
-        if tensors is None:
-            total = self._codec.total_shard_count()
-            if len(shards) != total:
-                raise Error(1 if len(shards) < total else 2)
-            raise Error(12)
-        p, l = _ptrs(tensors)
-        _check(getattr(lib, f"trs_{self._fam}_reconstruct")(
-            self._h, p, l, _flags(present), len(shards), int(data_only), _stream()))
-        for i in filled:
-            if not (data_only and i >= self._k):
-                shards[i] = tensors[i]
+        self._codec._call_reconstruct(self._h, self._fam, shards, data_only, 12)
 
     def reconstruct(self, shards: list) -> None:
         self._reconstruct(shards, False)
@@ -352,24 +448,6 @@ class ReedSolomon(_Codec):
         """Alias of encode (src/reedsolomon.rs:354-359)."""
         self.encode(shards)
 
-    def encode_host(self, data: Sequence, parity: Sequence, chunk: int = 0) -> None:
-        """encode_sep of host-resident shards (numpy uint8 arrays or CPU uint8
-        tensors, pinned or not), pipelined through the device in `chunk`-byte
-        pieces (0 = 1 MiB); returns when the parity is in host memory."""
-        d, length = _host_ptrs(data)
-        p, plen = _host_ptrs(parity)
-        if plen != length:
-            raise Error(ERROR_NAMES.index("IncorrectShardSize"))
-        _check(lib.trs_rs8_encode_host(self._h, d, p, length, chunk))
-
-    def reconstruct_host(self, shards: Sequence, present: Sequence[bool], data_only: bool = False,
-                         chunk: int = 0) -> None:
-        """reconstruct(_data) of host-resident shards with presence flags,
-        pipelined through the device; missing entries are overwritten."""
-        ptrs, length = _host_ptrs(shards)
-        _check(lib.trs_rs8_reconstruct_host(self._h, ptrs, _flags(present), len(shards), length,
-                                            int(data_only), chunk))
-
     def reconstruct_alloc(self, shards: Sequence, data_only: bool = False) -> list:
         """reconstruct over Option<T> shards (src/reedsolomon.rs:88-101): None
         entries are missing shards that the codec allocates on the device and
@@ -392,18 +470,23 @@ class ReedSolomon(_Codec):
         return out
 
     def encode_rows_into(self, data, row_len: int, parity) -> None:
-        _check(lib.trs_rs8_encode_rows_into(self._h, ctypes.c_void_p(_dev_ptr(data)), data.numel(),
-                                            row_len, ctypes.c_void_p(_dev_ptr(parity)),
-                                            parity.numel(), _stream()))
+        host = _on_host([data, parity])
+        args = (self._h, _buf_ptr(data, host), _numel(data), row_len, _buf_ptr(parity, host),
+                _numel(parity))
+        _check(self._fn("encode_rows_into", host)(*(args if host else args + (_stream(),))))
 
     def encode_rows(self, data, row_len: int):
-        """Panics (AssertionError) on a bad length, like src/reedsolomon.rs:652-657."""
+        """Panics (AssertionError) on a bad length, like src/reedsolomon.rs:652-657.
+        Returns the parity rows in the same memory as ``data``."""
         import torch
 
-        assert data.numel() == self.data_shard_count() * row_len, \
+        assert _numel(data) == self.data_shard_count() * row_len, \
             "data must be data_shard_count * row_len bytes"
-        parity = torch.zeros(self.parity_shard_count() * row_len, dtype=torch.uint8,
-                             device=data.device)
+        n = self.parity_shard_count() * row_len
+        if _is_host(data):
+            parity = np.zeros(n, np.uint8)
+        else:
+            parity = torch.zeros(n, dtype=torch.uint8, device=data.device)
         self.encode_rows_into(data, row_len, parity)
         return parity
 
@@ -539,7 +622,7 @@ def fft8_program_info(k: int, m: int) -> Optional[dict]:
 
 
 class Batch:
-    """Many GF(2^8) encode / reconstruct requests run in one persistent launch.
+    """Many GF(2^8) encode / reconstruct requests run as one kernel per work list.
 
     No reference counterpart: the reference codes one stripe per call and leaves
     batching to callers (BENCH-RESULTS.md:98-100).  Each queued request is
@@ -686,28 +769,6 @@ def mul_slice_xor(out, inp, c: int) -> None:
     if out.numel() != inp.numel():
         raise AssertionError("assertion `left == right` failed: slice lengths differ")
     _slice(out, inp, c, True)
-
-
-def _host_ptrs(arrays: Sequence) -> tuple:
-    """Host pointers and the common length of numpy uint8 arrays or CPU uint8
-    tensors."""
-    import torch
-
-    arr = (ctypes.c_void_p * max(1, len(arrays)))()
-    length = None
-    for i, a in enumerate(arrays):
-        if isinstance(a, torch.Tensor):
-            if a.is_cuda or a.dtype != torch.uint8 or not a.is_contiguous():
-                raise TypeError("host shards must be contiguous CPU uint8 tensors or arrays")
-            arr[i], n = a.data_ptr(), a.numel()
-        else:
-            if a.dtype != np.uint8 or not a.flags["C_CONTIGUOUS"]:
-                raise TypeError("host shards must be contiguous uint8 arrays")
-            arr[i], n = a.ctypes.data, a.size
-        if length is not None and n != length:
-            raise Error(ERROR_NAMES.index("IncorrectShardSize"))
-        length = n
-    return arr, (length or 0)
 
 
 class DeviceBuffer:

@@ -48,10 +48,12 @@ _sig("trs_status_message", ctypes.c_char_p, c_int)
 _sig("trs_version", ctypes.c_char_p)
 _sig("trs_gf8_mul_slice", c_int, c_vp, c_size, c_vp, c_size, ctypes.c_uint8, c_int, c_vp)
 _sig("trs_fill_random", c_int, c_vp, c_size, ctypes.c_uint64, c_vp)
-_sig("trs_rs8_encode_host", c_int, c_vp, c_ptrs, c_ptrs, c_size, c_size)
-_sig("trs_rs8_reconstruct_host", c_int, c_vp, c_ptrs, c_u8p, c_size, c_size, c_int, c_size)
 _sig("trs_rs8_reconstruct_alloc", c_int, c_vp, c_ptrs, c_sizep, c_size, c_int, c_vp)
+_sig("trs_device_alloc", c_int, c_size, ctypes.POINTER(c_vp))
 _sig("trs_device_free", c_int, c_vp)
+_sig("trs_memcpy", c_int, c_vp, c_vp, c_size, c_vp)
+_sig("trs_stream_synchronize", c_int, c_vp)
+_sig("trs_set_host_chunk", c_int, c_size)
 _sig("trs_u16_to_split", c_int, c_vp, c_size, c_vp, c_vp)
 _sig("trs_split_to_u16", c_int, c_vp, c_size, c_vp, c_vp)
 
@@ -68,6 +70,14 @@ for fam, elem in (("rs8", c_u8p), ("rs16", c_u16p)):
     _sig(f"trs_{fam}_reconstruct", c_int, c_vp, c_ptrs, c_sizep, c_u8p, c_size, c_int, c_vp)
     _sig(f"trs_{fam}_reconstruct_rows", c_int, c_vp, c_vp, c_size, c_size, c_u8p, c_size, c_vp)
     _sig(f"trs_{fam}_prepare_decode", c_int, c_vp, c_u8p, c_size, ctypes.POINTER(c_vp))
+    _sig(f"trs_{fam}_clone", c_int, c_vp, ctypes.POINTER(c_vp))
+    # host-memory forms: the same arguments without the stream
+    _sig(f"trs_{fam}_encode_host", c_int, c_vp, c_ptrs, c_sizep, c_size)
+    _sig(f"trs_{fam}_encode_sep_host", c_int, c_vp, c_ptrs, c_sizep, c_size, c_ptrs, c_sizep,
+         c_size)
+    _sig(f"trs_{fam}_verify_host", c_int, c_vp, c_ptrs, c_sizep, c_size, ctypes.POINTER(c_int))
+    _sig(f"trs_{fam}_reconstruct_host", c_int, c_vp, c_ptrs, c_sizep, c_u8p, c_size, c_int)
+    _sig(f"trs_{fam}_reconstruct_rows_host", c_int, c_vp, c_vp, c_size, c_size, c_u8p, c_size)
 
 _sig("trs_rs8_encode_route", ctypes.c_char_p, c_vp, c_size)
 _sig("trs_rs8_reference_route", ctypes.c_char_p, c_vp, c_size)
@@ -77,6 +87,7 @@ _sig("trs_rs8_encode_sep_route", c_int, c_vp, c_int, c_ptrs, c_sizep, c_size, c_
      c_size, c_vp)
 _sig("trs_rs8_set_apply_route", c_int, c_vp, c_int)
 _sig("trs_rs8_encode_rows_into", c_int, c_vp, c_vp, c_size, c_size, c_vp, c_size, c_vp)
+_sig("trs_rs8_encode_rows_into_host", c_int, c_vp, c_vp, c_size, c_size, c_vp, c_size)
 _sig("trs_rs16_encode_sep_matrix", c_int, c_vp, c_ptrs, c_sizep, c_size, c_ptrs, c_sizep, c_size,
      c_vp)
 _sig("trs_rs16_encode_sep_fft", c_int, c_vp, c_ptrs, c_sizep, c_size, c_ptrs, c_sizep, c_size,
@@ -99,6 +110,7 @@ _sig("trs_fft16_decode_program", c_int, c_size, c_size, c_u8p, c_u32p, c_size, c
      c_u32p, c_size, c_u32p, c_size, c_u32p, c_size, ctypes.POINTER(c_size))
 for fam in ("dec8", "dec16"):
     _sig(f"trs_{fam}_reconstruct", c_int, c_vp, c_ptrs, c_sizep, c_u8p, c_size, c_int, c_vp)
+    _sig(f"trs_{fam}_reconstruct_host", c_int, c_vp, c_ptrs, c_sizep, c_u8p, c_size, c_int)
     _sig(f"trs_{fam}_free", None, c_vp)
 
 _sig("trs_batch8_new", c_int, ctypes.POINTER(c_vp))

@@ -185,6 +185,23 @@ __device__ __forceinline__ void load_split(const uint8_t *base, uint64_t off, ui
   bytes_to_planes(w);
 }
 
+// Verify epilogue of a program: instead of storing `planes`, compare them
+// with the lane's 32 bytes of shard `base` and OR the difference into `bad`.
+// Bytes past the end read as zero, and a linear program's output there is
+// zero too (its inputs read as zero), so they never differ.
+template <bool NT = false>
+__device__ __forceinline__ void verify_split(const uint8_t *base, uint64_t off, uint64_t len,
+                                             int mode, const uint32_t (&planes)[8],
+                                             uint32_t &bad) {
+  uint32_t w[8], x[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) w[i] = planes[i];
+  planes_to_bytes(w);
+  load_split_raw<NT>(base, off, len, mode, x);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) bad |= w[i] ^ x[i];
+}
+
 template <bool NT = false>
 __device__ __forceinline__ void store_split(uint8_t *base, uint64_t off, uint64_t len, int mode,
                                             const uint32_t (&planes)[8]) {

@@ -428,8 +428,10 @@ struct PtrTable {
 // One stripe of a device program: ptrs = its loaded then stored shards,
 // `plane` columns.  Up to kFftInlinePtrs pointers travel in the kernel
 // arguments; more go through a device table.
+// flag non-null: verify (compare the stored registers with the shards).
 int launch_fft16_one(const Fft16Device &d, const std::vector<const uint8_t *> &ptrs,
-                     size_t nstore, uint64_t plane, hipStream_t stream, bool byte_planes = false) {
+                     size_t nstore, uint64_t plane, hipStream_t stream, bool byte_planes = false,
+                     uint32_t *flag = nullptr) {
   bool al = true;
   for (auto q : ptrs) al = al && aligned16(q);
   Fft16Call c{};
@@ -437,6 +439,7 @@ int launch_fft16_one(const Fft16Device &d, const std::vector<const uint8_t *> &p
   c.stripes = 1;
   c.nstore = uint32_t(nstore);
   c.vec_ok = al ? 1u : 0u;
+  c.flag = flag;
   if (ptrs.size() <= kFftInlinePtrs) {
     Fft16Ptrs P{};
     std::copy(ptrs.begin(), ptrs.end(), P.p);
@@ -447,6 +450,225 @@ int launch_fft16_one(const Fft16Device &d, const std::vector<const uint8_t *> &p
   if (st) return st;
   c.ptrs = static_cast<const uint8_t *const *>(table.d);
   return hip_status(launch_fft16(d, c, stream, nullptr, byte_planes));
+}
+
+// ---- host-resident shards ----------------------------------------------------
+// The reference's buffers are host slices.  Calls on host memory stream their
+// shards through device memory in column chunks: the host-to-device copy of
+// chunk c+1, the kernels of chunk c and the device-to-host copy of chunk c-1
+// run on three streams at once.
+
+// Library-owned staging of host-resident calls: S slots, each holding one
+// chunk of every shard row of a call in device memory and, for rows packed
+// through staging, in pinned host memory; one stream per pipeline stage.
+// Kept per host thread and grown on demand.
+struct HostPipe {
+  static constexpr int S = 3;
+  void *dev[S] = {};
+  void *pin[S] = {};
+  size_t dev_bytes = 0, pin_bytes = 0;
+  hipStream_t h2d = nullptr, comp = nullptr, d2h = nullptr;
+  hipEvent_t e_h2d[S] = {}, e_comp[S] = {}, e_d2h[S] = {};
+  bool ready = false;
+  ~HostPipe() {
+    for (int s = 0; s < S; ++s) {
+      if (dev[s]) (void)hipFree(dev[s]);
+      if (pin[s]) (void)hipHostFree(pin[s]);
+      if (e_h2d[s]) (void)hipEventDestroy(e_h2d[s]);
+      if (e_comp[s]) (void)hipEventDestroy(e_comp[s]);
+      if (e_d2h[s]) (void)hipEventDestroy(e_d2h[s]);
+    }
+    if (h2d) (void)hipStreamDestroy(h2d);
+    if (comp) (void)hipStreamDestroy(comp);
+    if (d2h) (void)hipStreamDestroy(d2h);
+  }
+  hipError_t setup(size_t bytes, bool staged) {
+    hipError_t e;
+    if (!ready) {
+      for (hipStream_t *st : {&h2d, &comp, &d2h})
+        if ((e = hipStreamCreateWithFlags(st, hipStreamNonBlocking)) != hipSuccess) return e;
+      for (int s = 0; s < S; ++s)
+        for (hipEvent_t *ev : {&e_h2d[s], &e_comp[s], &e_d2h[s]})
+          if ((e = hipEventCreateWithFlags(ev, hipEventDisableTiming)) != hipSuccess) return e;
+      ready = true;
+    }
+    if (bytes > dev_bytes) {
+      for (int s = 0; s < S; ++s) {
+        if (dev[s]) (void)hipFree(dev[s]);
+        dev[s] = nullptr;
+      }
+      dev_bytes = 0;
+      for (int s = 0; s < S; ++s)
+        if ((e = hipMalloc(&dev[s], bytes)) != hipSuccess) return e;
+      dev_bytes = bytes;
+    }
+    if (staged && bytes > pin_bytes) {
+      for (int s = 0; s < S; ++s) {
+        if (pin[s]) (void)hipHostFree(pin[s]);
+        pin[s] = nullptr;
+      }
+      pin_bytes = 0;
+      for (int s = 0; s < S; ++s)
+        if ((e = hipHostMalloc(&pin[s], bytes, hipHostMallocDefault)) != hipSuccess) return e;
+      pin_bytes = bytes;
+    }
+    return hipSuccess;
+  }
+};
+
+HostPipe &host_pipe() {
+  static thread_local HostPipe pipe;
+  return pipe;
+}
+
+enum class HostMem { kPageable, kPinned, kDevice };
+
+HostMem host_mem(const void *p) {
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // pageable memory is not known to HIP: clear the query's error
+    return HostMem::kPageable;
+  }
+  if (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeArray) return HostMem::kDevice;
+  return a.type == hipMemoryTypeHost ? HostMem::kPinned : HostMem::kPageable;
+}
+
+constexpr size_t kDefaultHostChunk = size_t(1) << 20;  // bytes of each shard per chunk
+constexpr size_t kHostSlotMax = size_t(64) << 20;      // bytes of all rows per chunk
+constexpr size_t kDirectMinRow = size_t(256) << 10;
+std::atomic<size_t> g_host_chunk{0};
+
+// One host-resident call pipelined over column chunks.  Host shard i of
+// `len` bytes holds `planes` planes of len / planes bytes (2 for the tower's
+// split-plane shards); chunk c of every shard is the same column range of
+// each plane, laid out on the device as a shard of its own ([plane 0 piece |
+// plane 1 piece]).  in_host rows are read, out_host rows written; run(rows,
+// chunk_bytes, stream) gets nin + nout device rows.  Pinned callers with long
+// chunks are copied row by row; other rows are packed into pinned staging and
+// cross the bus in one copy per direction and chunk.
+template <typename Run>
+int host_pipeline(const uint8_t *const *in_host, size_t nin, uint8_t *const *out_host,
+                  size_t nout, size_t len, size_t planes, Run &&run) {
+  const size_t rows = nin + nout;
+  if (rows == 0 || len == 0) return TRS_OK;
+  const size_t plane = len / planes;
+  size_t chunk = g_host_chunk.load();
+  if (chunk == 0) chunk = kDefaultHostChunk;
+  size_t cols = std::max<size_t>(chunk / planes, 64);
+  cols = std::min(cols, std::max<size_t>(kHostSlotMax / (rows * planes), 64));
+  cols &= ~size_t(63);  // chunk pieces keep the caller's 64-byte alignment
+  cols = std::min(cols, plane);
+  const size_t rowb = cols * planes;
+  bool pinned = true;
+  for (size_t r = 0; r < rows; ++r) {
+    const void *q = r < nin ? static_cast<const void *>(in_host[r]) : out_host[r - nin];
+    const HostMem k = host_mem(q);
+    if (k == HostMem::kDevice) return TRS_ERR_INVALID_ARGUMENT;  // a device shard in a host call
+    pinned = pinned && k == HostMem::kPinned;
+  }
+  const bool direct = pinned && rowb >= kDirectMinRow;
+  HostPipe &P = host_pipe();
+  TRS_TRY(P.setup(rows * rowb, !direct));
+  const size_t nchunks = (plane + cols - 1) / cols;
+  auto drain = [&](size_t c) -> int {  // chunk c's outputs are home once this returns
+    const int s = int(c % HostPipe::S);
+    TRS_TRY(hipEventSynchronize(P.e_d2h[s]));
+    if (!direct) {
+      const size_t off = c * cols, cl = std::min(cols, plane - off);
+      const uint8_t *pin = static_cast<const uint8_t *>(P.pin[s]) + nin * rowb;
+      for (size_t j = 0; j < nout; ++j)
+        for (size_t p = 0; p < planes; ++p)
+          std::memcpy(out_host[j] + p * plane + off, pin + j * rowb + p * cl, cl);
+    }
+    return TRS_OK;
+  };
+  auto body = [&]() -> int {
+    std::vector<uint8_t *> dev_rows(rows);
+    for (size_t c = 0; c < nchunks; ++c) {
+      const int s = int(c % HostPipe::S);
+      const size_t off = c * cols, cl = std::min(cols, plane - off);
+      if (c >= size_t(HostPipe::S)) {
+        const int st = drain(c - HostPipe::S);  // frees slot s
+        if (st) return st;
+      }
+      uint8_t *dev = static_cast<uint8_t *>(P.dev[s]);
+      uint8_t *pin = static_cast<uint8_t *>(P.pin[s]);
+      for (size_t r = 0; r < rows; ++r) dev_rows[r] = dev + r * rowb;
+      if (direct) {
+        for (size_t i = 0; i < nin; ++i)
+          for (size_t p = 0; p < planes; ++p)
+            TRS_TRY(hipMemcpyAsync(dev + i * rowb + p * cl, in_host[i] + p * plane + off, cl,
+                                   hipMemcpyHostToDevice, P.h2d));
+      } else if (nin) {
+        for (size_t i = 0; i < nin; ++i)
+          for (size_t p = 0; p < planes; ++p)
+            std::memcpy(pin + i * rowb + p * cl, in_host[i] + p * plane + off, cl);
+        TRS_TRY(hipMemcpyAsync(dev, pin, nin * rowb, hipMemcpyHostToDevice, P.h2d));
+      }
+      TRS_TRY(hipEventRecord(P.e_h2d[s], P.h2d));
+      TRS_TRY(hipStreamWaitEvent(P.comp, P.e_h2d[s], 0));
+      const int st = run(dev_rows.data(), cl * planes, P.comp);
+      if (st) return st;
+      TRS_TRY(hipEventRecord(P.e_comp[s], P.comp));
+      TRS_TRY(hipStreamWaitEvent(P.d2h, P.e_comp[s], 0));
+      if (direct) {
+        for (size_t j = 0; j < nout; ++j)
+          for (size_t p = 0; p < planes; ++p)
+            TRS_TRY(hipMemcpyAsync(out_host[j] + p * plane + off, dev + (nin + j) * rowb + p * cl,
+                                   cl, hipMemcpyDeviceToHost, P.d2h));
+      } else if (nout) {
+        TRS_TRY(hipMemcpyAsync(pin + nin * rowb, dev + nin * rowb, nout * rowb,
+                               hipMemcpyDeviceToHost, P.d2h));
+      }
+      TRS_TRY(hipEventRecord(P.e_d2h[s], P.d2h));
+    }
+    const size_t first = nchunks > size_t(HostPipe::S) ? nchunks - HostPipe::S : 0;
+    for (size_t c = first; c < nchunks; ++c) {
+      const int st = drain(c);
+      if (st) return st;
+    }
+    return TRS_OK;
+  };
+  const int st = body();
+  if (st) {  // nothing of this call may still be in flight on return
+    (void)hipStreamSynchronize(P.h2d);
+    (void)hipStreamSynchronize(P.comp);
+    (void)hipStreamSynchronize(P.d2h);
+  }
+  return st;
+}
+
+// Where a call's shards live.  Device pointers: the work is enqueued on
+// `stream`.  Host pointers (`host`): the shards stream through device memory
+// by host_pipeline, `planes` planes per shard.
+struct Exec {
+  hipStream_t stream = nullptr;
+  bool host = false;
+  size_t planes = 1;
+  Exec(hipStream_t s) : stream(s) {}  // NOLINT: a stream means device shards
+  static Exec on_host(size_t planes) {
+    Exec e(nullptr);
+    e.host = true;
+    e.planes = planes;
+    return e;
+  }
+};
+
+// Runs launch(ptrs, len, stream) over a validated shard list: ptrs[0, nin)
+// are read, the rest written.  Device calls launch once; host calls launch
+// once per chunk on device copies of the rows, with the chunk's length.
+template <typename Launch>
+int dispatch(const Exec &ex, const std::vector<const uint8_t *> &ptrs, size_t nin, size_t len,
+             Launch &&launch) {
+  if (!ex.host) return launch(ptrs, len, ex.stream);
+  std::vector<uint8_t *> out;
+  for (size_t j = nin; j < ptrs.size(); ++j) out.push_back(const_cast<uint8_t *>(ptrs[j]));
+  std::vector<const uint8_t *> dev(ptrs.size());
+  return host_pipeline(ptrs.data(), nin, out.data(), out.size(), len, ex.planes,
+                       [&](uint8_t **rows, size_t cl, hipStream_t s) {
+                         dev.assign(rows, rows + ptrs.size());
+                         return launch(dev, cl, s);
+                       });
 }
 
 }  // namespace
@@ -708,6 +930,27 @@ int read_flag(uint32_t *flag, hipStream_t stream, int *ok) {
   return TRS_OK;
 }
 
+// verify over a validated shard list (every shard read): launch(ptrs, len,
+// flag, stream) sets *flag on a mismatch; *ok = 1 when nothing differed.
+template <typename Launch>
+int verify_dispatch(const Exec &ex, const std::vector<const uint8_t *> &ptrs, size_t len, int *ok,
+                    Launch &&launch) {
+  uint32_t *flag = nullptr;
+  int st = verify_flag(&flag);
+  if (st) return st;
+  bool first = true;
+  hipStream_t last = ex.stream;
+  st = dispatch(ex, ptrs, ptrs.size(), len,
+                [&](const std::vector<const uint8_t *> &p, size_t l, hipStream_t s) -> int {
+                  if (first) TRS_TRY(hipMemsetAsync(flag, 0, sizeof(uint32_t), s));
+                  first = false;
+                  last = s;
+                  return launch(p, l, flag, s);
+                });
+  if (st) return st;
+  return read_flag(flag, last, ok);
+}
+
 }  // namespace
 
 // ============================================================================
@@ -899,7 +1142,7 @@ int check_all8(size_t total, const size_t *lens, size_t n) {
   return TRS_OK;
 }
 
-int encode8_ptrs(const trs_rs8 *rs, std::vector<const uint8_t *> &ptrs, size_t len,
+int encode8_ptrs(const trs_rs8 *rs, const std::vector<const uint8_t *> &ptrs, size_t len,
                  hipStream_t s, int route) {
   for (auto q : ptrs)
     if (!q) return TRS_ERR_INVALID_ARGUMENT;
@@ -922,56 +1165,93 @@ int encode8_ptrs(const trs_rs8 *rs, std::vector<const uint8_t *> &ptrs, size_t l
   c.vec_ok = al ? 1u : 0u;
   return hip_status(launch_bs_encode(route, c, s));
 }
-}  // namespace
 
-int trs_rs8_encode(const trs_rs8 *rs, uint8_t *const *shards, const size_t *lens, size_t n,
-                   void *stream) {
+// The automatic encode of a validated shard list (k data then m parity).
+int encode8_auto(const trs_rs8 *rs, const std::vector<const uint8_t *> &ptrs, size_t len,
+                 const Exec &ex) {
+  return dispatch(ex, ptrs, rs->k, len,
+                  [rs](const std::vector<const uint8_t *> &p, size_t l, hipStream_t s) {
+                    return encode8_ptrs(rs, p, l, s, auto_route8(rs, l));
+                  });
+}
+
+// verify of k data + m parity shards through the codec's encode route: the
+// generated bit-sliced program or the staged FFT program compare their
+// outputs with the parity shards instead of writing them; the matrix route
+// runs the apply kernel's verify mode.  Reads every shard once.
+int verify8_launch(const trs_rs8 *rs, const std::vector<const uint8_t *> &ptrs, size_t len,
+                   uint32_t *flag, hipStream_t s) {
+  const int route = auto_route8(rs, len);
+  if (route == kRouteStaged8) {
+    const Fft16Device *d = nullptr;
+    const int st = staged8_device(rs, &d);
+    if (st) return st;
+    if (fft16_device_loads(*d) + fft16_device_stores(*d) != ptrs.size())
+      return TRS_ERR_UNSUPPORTED;
+    return launch_fft16_one(*d, ptrs, fft16_device_stores(*d), len, s, true, flag);
+  }
+  if (route == kRouteBsFft || route == kRouteBsMatrix) {
+    Gf8Call c{};
+    bool al = true;
+    for (size_t i = 0; i < ptrs.size(); ++i) {
+      c.ptr[i] = ptrs[i];
+      al = al && aligned16(ptrs[i]);
+    }
+    c.len = len;
+    c.k = uint32_t(rs->k);
+    c.nout = uint32_t(rs->m);
+    c.vec_ok = al ? 1u : 0u;
+    c.flag = flag;
+    return hip_status(launch_bs_verify(route, c, s));
+  }
+  return run_gf8(ptrs, rs->k, rs->m, rs->parity, len, kVerify, flag, s);
+}
+
+// ReedSolomon::encode (src/reedsolomon.rs:271-279) over device or host shards.
+int rs8_encode(const trs_rs8 *rs, uint8_t *const *shards, const size_t *lens, size_t n,
+               const Exec &ex) {
   if (!rs || (n && (!shards || !lens))) return TRS_ERR_INVALID_ARGUMENT;
   if (n == 0) return TRS_TOO_FEW_SHARDS;
   int st = check_all8(rs->n, lens, n);
   if (st) return st;
   std::vector<const uint8_t *> ptrs(shards, shards + n);
-  return encode8_ptrs(rs, ptrs, lens[0], as_stream(stream), auto_route8(rs, lens[0]));
+  for (auto q : ptrs)
+    if (!q) return TRS_ERR_INVALID_ARGUMENT;
+  return encode8_auto(rs, ptrs, lens[0], ex);
 }
 
-int trs_rs8_encode_sep(const trs_rs8 *rs, const uint8_t *const *data, const size_t *dl,
-                       size_t nd, uint8_t *const *parity, const size_t *pl, size_t np,
-                       void *stream) {
+// The checks of encode_sep (src/reedsolomon.rs:289-308); fills ptrs and len.
+int check_sep8(const trs_rs8 *rs, const uint8_t *const *data, const size_t *dl, size_t nd,
+               uint8_t *const *parity, const size_t *pl, size_t np,
+               std::vector<const uint8_t *> &ptrs, size_t &len) {
   if (!rs || (nd && (!data || !dl)) || (np && (!parity || !pl))) return TRS_ERR_INVALID_ARGUMENT;
-  // src/reedsolomon.rs:289-308
   if (nd != rs->k) return nd < rs->k ? TRS_TOO_FEW_DATA_SHARDS : TRS_TOO_MANY_DATA_SHARDS;
   if (np != rs->m) return np < rs->m ? TRS_TOO_FEW_PARITY_SHARDS : TRS_TOO_MANY_PARITY_SHARDS;
-  const size_t len = dl[0];
+  len = dl[0];
   for (size_t i = 0; i < nd; ++i)
     if (dl[i] != len) return TRS_INCORRECT_SHARD_SIZE;
   for (size_t i = 0; i < np; ++i)
     if (pl[i] != len) return TRS_INCORRECT_SHARD_SIZE;
-  if (len == 0) return TRS_OK;
-  std::vector<const uint8_t *> ptrs(data, data + nd);
+  ptrs.assign(data, data + nd);
   ptrs.insert(ptrs.end(), parity, parity + np);
-  return encode8_ptrs(rs, ptrs, len, as_stream(stream), auto_route8(rs, len));
+  if (len)
+    for (auto q : ptrs)
+      if (!q) return TRS_ERR_INVALID_ARGUMENT;
+  return TRS_OK;
 }
 
-int trs_rs8_encode_sep_route(const trs_rs8 *rs, int route, const uint8_t *const *data,
-                             const size_t *dl, size_t nd, uint8_t *const *parity, const size_t *pl,
-                             size_t np, void *stream) {
-  if (!rs || (nd && (!data || !dl)) || (np && (!parity || !pl))) return TRS_ERR_INVALID_ARGUMENT;
-  if (route < kRouteTable || route > kRouteStaged8) return TRS_ERR_INVALID_ARGUMENT;
-  if (nd != rs->k) return nd < rs->k ? TRS_TOO_FEW_DATA_SHARDS : TRS_TOO_MANY_DATA_SHARDS;
-  if (np != rs->m) return np < rs->m ? TRS_TOO_FEW_PARITY_SHARDS : TRS_TOO_MANY_PARITY_SHARDS;
-  const size_t len = dl[0];
-  for (size_t i = 0; i < nd; ++i)
-    if (dl[i] != len) return TRS_INCORRECT_SHARD_SIZE;
-  for (size_t i = 0; i < np; ++i)
-    if (pl[i] != len) return TRS_INCORRECT_SHARD_SIZE;
-  if (len == 0) return TRS_OK;
-  std::vector<const uint8_t *> ptrs(data, data + nd);
-  ptrs.insert(ptrs.end(), parity, parity + np);
-  return encode8_ptrs(rs, ptrs, len, as_stream(stream), route);
+int rs8_encode_sep(const trs_rs8 *rs, const uint8_t *const *data, const size_t *dl, size_t nd,
+                   uint8_t *const *parity, const size_t *pl, size_t np, const Exec &ex) {
+  std::vector<const uint8_t *> ptrs;
+  size_t len = 0;
+  int st = check_sep8(rs, data, dl, nd, parity, pl, np, ptrs, len);
+  if (st || len == 0) return st;
+  return encode8_auto(rs, ptrs, len, ex);
 }
 
-int trs_rs8_encode_rows_into(const trs_rs8 *rs, const uint8_t *data, size_t data_len,
-                             size_t row_len, uint8_t *parity, size_t parity_len, void *stream) {
+// ReedSolomon::encode_rows_into, src/reedsolomon.rs:667-682.
+int rs8_encode_rows_into(const trs_rs8 *rs, const uint8_t *data, size_t data_len, size_t row_len,
+                         uint8_t *parity, size_t parity_len, const Exec &ex) {
   if (!rs) return TRS_ERR_INVALID_ARGUMENT;
   // src/reedsolomon.rs:673-680
   if (data_len != rs->k * row_len) return TRS_INCORRECT_SHARD_SIZE;
@@ -984,11 +1264,12 @@ int trs_rs8_encode_rows_into(const trs_rs8 *rs, const uint8_t *data, size_t data
   // The reference's rows API always takes its matrix route
   // (src/reedsolomon.rs:684-697); every route here is byte-identical, so the
   // rows take the codec's fastest one.
-  return encode8_ptrs(rs, ptrs, row_len, as_stream(stream), auto_route8(rs, row_len));
+  return encode8_auto(rs, ptrs, row_len, ex);
 }
 
-int trs_rs8_verify(const trs_rs8 *rs, const uint8_t *const *shards, const size_t *lens, size_t n,
-                   int *ok, void *stream) {
+// ReedSolomon::verify, src/reedsolomon.rs:430-463.
+int rs8_verify(const trs_rs8 *rs, const uint8_t *const *shards, const size_t *lens, size_t n,
+               int *ok, const Exec &ex) {
   if (!rs || !ok || (n && (!shards || !lens))) return TRS_ERR_INVALID_ARGUMENT;
   if (n == 0) return TRS_TOO_FEW_SHARDS;
   int st = check_all8(rs->n, lens, n);
@@ -996,20 +1277,32 @@ int trs_rs8_verify(const trs_rs8 *rs, const uint8_t *const *shards, const size_t
   std::vector<const uint8_t *> ptrs(shards, shards + n);
   for (auto q : ptrs)
     if (!q) return TRS_ERR_INVALID_ARGUMENT;
-  uint32_t *flag = nullptr;
-  if ((st = verify_flag(&flag))) return st;
-  hipStream_t s = as_stream(stream);
-  TRS_TRY(hipMemsetAsync(flag, 0, sizeof(uint32_t), s));
-  if ((st = run_gf8(ptrs, rs->k, rs->m, rs->parity, lens[0], kVerify, flag, s))) return st;
-  return read_flag(flag, s, ok);
+  return verify_dispatch(ex, ptrs, lens[0], ok,
+                         [rs](const std::vector<const uint8_t *> &p, size_t l, uint32_t *flag,
+                              hipStream_t s) { return verify8_launch(rs, p, l, flag, s); });
 }
 
-namespace {
+// The apply of a reconstruct: ptrs = the plan's k survivors, then its nout
+// outputs.
+int rec8_launch(const Plan &plan, size_t k, size_t total, size_t nout,
+                const std::vector<const uint8_t *> &ptrs, size_t len, hipStream_t s,
+                int apply_route, int decode_route) {
+  if (nout && use_fft_decode8(decode_route, total, k, nout, len)) {
+    const Fft16Device *d = nullptr;
+    const int st = plan_fft16(const_cast<Plan &>(plan), total, &d, true);
+    if (st) return st;
+    return launch_fft16_one(*d, ptrs, nout, len, s, true);
+  }
+  if (use_mfma(apply_route, k, len, false) && plan.frag.p)
+    return run_mfma(ptrs, k, nout, plan.frag.p, len, s);
+  return run_gf8(ptrs, k, nout, plan.coef, len, kWrite, nullptr, s);
+}
+
 // reconstruct_internal / PreparedDecoder::reconstruct_internal
 // (src/reedsolomon.rs:615-641, :771-796).  `plan` non-null = prepared.
 int reconstruct8(const trs_rs8 *rs, size_t k, size_t total, std::shared_ptr<Plan> plan,
                  uint8_t *const *shards, const size_t *lens, const uint8_t *present, size_t n,
-                 int data_only, hipStream_t s, int apply_route, int dec_decode_route = 0) {
+                 int data_only, const Exec &ex, int apply_route, int dec_decode_route = 0) {
   if (n && (!shards || !lens || !present)) return TRS_ERR_INVALID_ARGUMENT;
   if (n != total) return n < total ? TRS_TOO_FEW_SHARDS : TRS_TOO_MANY_SHARDS;
   Mask mask;
@@ -1028,17 +1321,83 @@ int reconstruct8(const trs_rs8 *rs, size_t k, size_t total, std::shared_ptr<Plan
   std::vector<const uint8_t *> ptrs;
   size_t nout = 0;
   if ((st = gather(*plan, shards, lens, k, len, data_only != 0, ptrs, nout))) return st;
-  if (nout && use_fft_decode8(rs ? rs->decode_route.load() : dec_decode_route, total, k, nout,
-                              len)) {
-    const Fft16Device *d = nullptr;
-    if ((st = plan_fft16(*plan, total, &d, true))) return st;
-    return launch_fft16_one(*d, ptrs, nout, len, s, true);
-  }
-  if (use_mfma(apply_route, k, len, false) && plan->frag.p)
-    return run_mfma(ptrs, k, nout, plan->frag.p, len, s);
-  return run_gf8(ptrs, k, nout, plan->coef, len, kWrite, nullptr, s);
+  if (nout == 0) return TRS_OK;
+  const int decode_route = rs ? rs->decode_route.load() : dec_decode_route;
+  const Plan &p = *plan;
+  return dispatch(ex, ptrs, k, len,
+                  [&](const std::vector<const uint8_t *> &q, size_t l, hipStream_t s) {
+                    return rec8_launch(p, k, total, nout, q, l, s, apply_route, decode_route);
+                  });
+}
+
+// ReedSolomon::reconstruct_rows, src/reedsolomon.rs:514-535.
+int rs8_reconstruct_rows(const trs_rs8 *rs, uint8_t *rows, size_t rows_len, size_t row_len,
+                         const uint8_t *present, size_t n_present, const Exec &ex) {
+  if (!rs || (n_present && !present)) return TRS_ERR_INVALID_ARGUMENT;
+  // src/reedsolomon.rs:520-528
+  if (n_present != rs->n) return TRS_INVALID_SHARD_FLAGS;
+  if (row_len == 0) return TRS_EMPTY_SHARD;
+  if (rows_len != rs->n * row_len) return TRS_INCORRECT_SHARD_SIZE;
+  if (!rows) return TRS_ERR_INVALID_ARGUMENT;
+  std::vector<uint8_t *> shards(rs->n);
+  std::vector<size_t> lens(rs->n, row_len);
+  for (size_t i = 0; i < rs->n; ++i) shards[i] = rows + i * row_len;
+  return reconstruct8(rs, rs->k, rs->n, nullptr, shards.data(), lens.data(), present, rs->n, 0,
+                      ex, rs->apply_route.load());
 }
 }  // namespace
+
+int trs_rs8_encode(const trs_rs8 *rs, uint8_t *const *shards, const size_t *lens, size_t n,
+                   void *stream) {
+  return rs8_encode(rs, shards, lens, n, as_stream(stream));
+}
+
+int trs_rs8_encode_host(const trs_rs8 *rs, uint8_t *const *shards, const size_t *lens, size_t n) {
+  return rs8_encode(rs, shards, lens, n, Exec::on_host(1));
+}
+
+int trs_rs8_encode_sep(const trs_rs8 *rs, const uint8_t *const *data, const size_t *dl,
+                       size_t nd, uint8_t *const *parity, const size_t *pl, size_t np,
+                       void *stream) {
+  return rs8_encode_sep(rs, data, dl, nd, parity, pl, np, as_stream(stream));
+}
+
+int trs_rs8_encode_sep_host(const trs_rs8 *rs, const uint8_t *const *data, const size_t *dl,
+                            size_t nd, uint8_t *const *parity, const size_t *pl, size_t np) {
+  return rs8_encode_sep(rs, data, dl, nd, parity, pl, np, Exec::on_host(1));
+}
+
+int trs_rs8_encode_sep_route(const trs_rs8 *rs, int route, const uint8_t *const *data,
+                             const size_t *dl, size_t nd, uint8_t *const *parity, const size_t *pl,
+                             size_t np, void *stream) {
+  if (!rs || (nd && (!data || !dl)) || (np && (!parity || !pl))) return TRS_ERR_INVALID_ARGUMENT;
+  if (route < kRouteTable || route > kRouteStaged8) return TRS_ERR_INVALID_ARGUMENT;
+  std::vector<const uint8_t *> ptrs;
+  size_t len = 0;
+  int st = check_sep8(rs, data, dl, nd, parity, pl, np, ptrs, len);
+  if (st || len == 0) return st;
+  return encode8_ptrs(rs, ptrs, len, as_stream(stream), route);
+}
+
+int trs_rs8_encode_rows_into(const trs_rs8 *rs, const uint8_t *data, size_t data_len,
+                             size_t row_len, uint8_t *parity, size_t parity_len, void *stream) {
+  return rs8_encode_rows_into(rs, data, data_len, row_len, parity, parity_len, as_stream(stream));
+}
+
+int trs_rs8_encode_rows_into_host(const trs_rs8 *rs, const uint8_t *data, size_t data_len,
+                                  size_t row_len, uint8_t *parity, size_t parity_len) {
+  return rs8_encode_rows_into(rs, data, data_len, row_len, parity, parity_len, Exec::on_host(1));
+}
+
+int trs_rs8_verify(const trs_rs8 *rs, const uint8_t *const *shards, const size_t *lens, size_t n,
+                   int *ok, void *stream) {
+  return rs8_verify(rs, shards, lens, n, ok, as_stream(stream));
+}
+
+int trs_rs8_verify_host(const trs_rs8 *rs, const uint8_t *const *shards, const size_t *lens,
+                        size_t n, int *ok) {
+  return rs8_verify(rs, shards, lens, n, ok, Exec::on_host(1));
+}
 
 int trs_rs8_set_decode_route(trs_rs8 *rs, int route) {
   if (!rs || route < 0 || route > 2) return TRS_ERR_INVALID_ARGUMENT;
@@ -1060,19 +1419,21 @@ int trs_rs8_reconstruct(const trs_rs8 *rs, uint8_t *const *shards, const size_t 
                       as_stream(stream), rs->apply_route.load());
 }
 
+int trs_rs8_reconstruct_host(const trs_rs8 *rs, uint8_t *const *shards, const size_t *lens,
+                             const uint8_t *present, size_t n, int data_only) {
+  if (!rs) return TRS_ERR_INVALID_ARGUMENT;
+  return reconstruct8(rs, rs->k, rs->n, nullptr, shards, lens, present, n, data_only,
+                      Exec::on_host(1), rs->apply_route.load());
+}
+
 int trs_rs8_reconstruct_rows(const trs_rs8 *rs, uint8_t *rows, size_t rows_len, size_t row_len,
                              const uint8_t *present, size_t n_present, void *stream) {
-  if (!rs || (n_present && !present)) return TRS_ERR_INVALID_ARGUMENT;
-  // src/reedsolomon.rs:520-528
-  if (n_present != rs->n) return TRS_INVALID_SHARD_FLAGS;
-  if (row_len == 0) return TRS_EMPTY_SHARD;
-  if (rows_len != rs->n * row_len) return TRS_INCORRECT_SHARD_SIZE;
-  if (!rows) return TRS_ERR_INVALID_ARGUMENT;
-  std::vector<uint8_t *> shards(rs->n);
-  std::vector<size_t> lens(rs->n, row_len);
-  for (size_t i = 0; i < rs->n; ++i) shards[i] = rows + i * row_len;
-  return reconstruct8(rs, rs->k, rs->n, nullptr, shards.data(), lens.data(), present, rs->n, 0,
-                      as_stream(stream), rs->apply_route.load());
+  return rs8_reconstruct_rows(rs, rows, rows_len, row_len, present, n_present, as_stream(stream));
+}
+
+int trs_rs8_reconstruct_rows_host(const trs_rs8 *rs, uint8_t *rows, size_t rows_len,
+                                  size_t row_len, const uint8_t *present, size_t n_present) {
+  return rs8_reconstruct_rows(rs, rows, rows_len, row_len, present, n_present, Exec::on_host(1));
 }
 
 int trs_rs8_prepare_decode(const trs_rs8 *rs, const uint8_t *present, size_t n, trs_dec8 **out) {
@@ -1102,6 +1463,13 @@ int trs_dec8_reconstruct(const trs_dec8 *dec, uint8_t *const *shards, const size
   if (!dec) return TRS_ERR_INVALID_ARGUMENT;
   return reconstruct8(nullptr, dec->k, dec->n, dec->plan, shards, lens, present, n, data_only,
                       as_stream(stream), dec->apply_route, dec->decode_route);
+}
+
+int trs_dec8_reconstruct_host(const trs_dec8 *dec, uint8_t *const *shards, const size_t *lens,
+                              const uint8_t *present, size_t n, int data_only) {
+  if (!dec) return TRS_ERR_INVALID_ARGUMENT;
+  return reconstruct8(nullptr, dec->k, dec->n, dec->plan, shards, lens, present, n, data_only,
+                      Exec::on_host(1), dec->apply_route, dec->decode_route);
 }
 
 void trs_dec8_free(trs_dec8 *dec) { delete dec; }
@@ -1206,10 +1574,34 @@ int check_shards16(const size_t *lens, size_t n, size_t *len_out) {
   *len_out = len;
   return TRS_OK;
 }
-}  // namespace
 
-int trs_rs16_encode(const trs_rs16 *rs, uint8_t *const *shards, const size_t *lens, size_t n,
-                    void *stream) {
+// The automatic encode of a validated shard list, device or host.  Host
+// shards keep their split-plane layout per column chunk (two planes).
+int encode16_auto(const trs_rs16 *rs, const std::vector<const uint8_t *> &ptrs, size_t len,
+                  const Exec &ex) {
+  return dispatch(ex, ptrs, rs->k, len,
+                  [rs](const std::vector<const uint8_t *> &p, size_t l, hipStream_t s) {
+                    return encode16_ptrs(rs, p, l, s);
+                  });
+}
+
+// verify through the encode route: the tower FFT program compares its parity
+// registers with the parity shards; the matrix route runs the apply kernel's
+// verify mode.
+int verify16_launch(const trs_rs16 *rs, const std::vector<const uint8_t *> &ptrs, size_t len,
+                    uint32_t *flag, hipStream_t s) {
+  if (rs->staged) {
+    const Fft16Device *d = nullptr;
+    const int st = fft16_device_for(rs, &d);
+    if (st) return st;
+    return launch_fft16_one(*d, ptrs, fft16_device_stores(*d), len / 2, s, false, flag);
+  }
+  return run_gf16(ptrs, rs->k, rs->m, rs->parity, len, kVerify, flag, s);
+}
+
+// ReedSolomon16::encode, src/reedsolomon16.rs:412-418.
+int rs16_encode(const trs_rs16 *rs, uint8_t *const *shards, const size_t *lens, size_t n,
+                const Exec &ex) {
   if (!rs || (n && (!shards || !lens))) return TRS_ERR_INVALID_ARGUMENT;
   if (n < rs->n) return TRS_TOO_FEW_SHARDS;
   if (n > rs->n) return TRS_TOO_MANY_SHARDS;
@@ -1219,12 +1611,12 @@ int trs_rs16_encode(const trs_rs16 *rs, uint8_t *const *shards, const size_t *le
   std::vector<const uint8_t *> ptrs(shards, shards + n);
   for (auto q : ptrs)
     if (!q) return TRS_ERR_INVALID_ARGUMENT;
-  return encode16_ptrs(rs, ptrs, len, as_stream(stream));
+  return encode16_auto(rs, ptrs, len, ex);
 }
 
-int trs_rs16_encode_sep(const trs_rs16 *rs, const uint8_t *const *data, const size_t *dl,
-                        size_t nd, uint8_t *const *parity, const size_t *pl, size_t np,
-                        void *stream) {
+// ReedSolomon16::encode_sep, src/reedsolomon16.rs:422-447.
+int rs16_encode_sep(const trs_rs16 *rs, const uint8_t *const *data, const size_t *dl, size_t nd,
+                    uint8_t *const *parity, const size_t *pl, size_t np, const Exec &ex) {
   if (!rs || (nd && (!data || !dl)) || (np && (!parity || !pl))) return TRS_ERR_INVALID_ARGUMENT;
   // src/reedsolomon16.rs:427-446
   if (nd != rs->k) return nd < rs->k ? TRS_TOO_FEW_DATA_SHARDS : TRS_TOO_MANY_DATA_SHARDS;
@@ -1238,7 +1630,29 @@ int trs_rs16_encode_sep(const trs_rs16 *rs, const uint8_t *const *data, const si
   ptrs.insert(ptrs.end(), parity, parity + np);
   for (auto q : ptrs)
     if (!q) return TRS_ERR_INVALID_ARGUMENT;
-  return encode16_ptrs(rs, ptrs, len, as_stream(stream));
+  return encode16_auto(rs, ptrs, len, ex);
+}
+}  // namespace
+
+int trs_rs16_encode(const trs_rs16 *rs, uint8_t *const *shards, const size_t *lens, size_t n,
+                    void *stream) {
+  return rs16_encode(rs, shards, lens, n, as_stream(stream));
+}
+
+int trs_rs16_encode_host(const trs_rs16 *rs, uint8_t *const *shards, const size_t *lens,
+                         size_t n) {
+  return rs16_encode(rs, shards, lens, n, Exec::on_host(2));
+}
+
+int trs_rs16_encode_sep(const trs_rs16 *rs, const uint8_t *const *data, const size_t *dl,
+                        size_t nd, uint8_t *const *parity, const size_t *pl, size_t np,
+                        void *stream) {
+  return rs16_encode_sep(rs, data, dl, nd, parity, pl, np, as_stream(stream));
+}
+
+int trs_rs16_encode_sep_host(const trs_rs16 *rs, const uint8_t *const *data, const size_t *dl,
+                             size_t nd, uint8_t *const *parity, const size_t *pl, size_t np) {
+  return rs16_encode_sep(rs, data, dl, nd, parity, pl, np, Exec::on_host(2));
 }
 
 int trs_rs16_encode_sep_matrix(const trs_rs16 *rs, const uint8_t *const *data,
@@ -1254,6 +1668,8 @@ int trs_rs16_encode_sep_matrix(const trs_rs16 *rs, const uint8_t *const *data,
   if (nd != rs->k || np != rs->m) return TRS_ERR_INVALID_ARGUMENT;
   std::vector<const uint8_t *> ptrs(data, data + nd);
   ptrs.insert(ptrs.end(), parity, parity + np);
+  for (auto q : ptrs)
+    if (!q) return TRS_ERR_INVALID_ARGUMENT;
   return run_gf16(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, as_stream(stream));
 }
 
@@ -1501,8 +1917,10 @@ int trs_rs16_encode_stripes(const trs_rs16 *rs, uint8_t *const *shards, size_t s
   return TRS_OK;
 }
 
-int trs_rs16_verify(const trs_rs16 *rs, const uint8_t *const *shards, const size_t *lens,
-                    size_t n, int *ok, void *stream) {
+namespace {
+// ReedSolomon16::verify, src/reedsolomon16.rs:453-505.
+int rs16_verify(const trs_rs16 *rs, const uint8_t *const *shards, const size_t *lens, size_t n,
+                int *ok, const Exec &ex) {
   if (!rs || !ok || (n && (!shards || !lens))) return TRS_ERR_INVALID_ARGUMENT;
   if (n < rs->n) return TRS_TOO_FEW_SHARDS;
   if (n > rs->n) return TRS_TOO_MANY_SHARDS;
@@ -1512,18 +1930,28 @@ int trs_rs16_verify(const trs_rs16 *rs, const uint8_t *const *shards, const size
   std::vector<const uint8_t *> ptrs(shards, shards + n);
   for (auto q : ptrs)
     if (!q) return TRS_ERR_INVALID_ARGUMENT;
-  uint32_t *flag = nullptr;
-  if ((st = verify_flag(&flag))) return st;
-  hipStream_t s = as_stream(stream);
-  TRS_TRY(hipMemsetAsync(flag, 0, sizeof(uint32_t), s));
-  if ((st = run_gf16(ptrs, rs->k, rs->m, rs->parity, len, kVerify, flag, s))) return st;
-  return read_flag(flag, s, ok);
+  return verify_dispatch(ex, ptrs, len, ok,
+                         [rs](const std::vector<const uint8_t *> &p, size_t l, uint32_t *flag,
+                              hipStream_t s) { return verify16_launch(rs, p, l, flag, s); });
 }
 
-namespace {
+// The apply of a tower reconstruct over the plan's survivors and outputs.
+int rec16_launch(Plan &plan, size_t k, size_t total, size_t nout,
+                 const std::vector<const uint8_t *> &ptrs, size_t len, hipStream_t s, int route) {
+  if (use_fft_decode(route, total, k, nout)) {
+    const Fft16Device *d = nullptr;
+    const int st = plan_fft16(plan, total, &d);
+    if (st) return st;
+    return run_fft16_program(*d, ptrs, nout, len, s);
+  }
+  const int st = plan16_matrix(plan, k);
+  if (st) return st;
+  return run_gf16(ptrs, k, nout, plan.coef, len, kWrite, nullptr, s);
+}
+
 int reconstruct16(const trs_rs16 *rs, size_t k, size_t total, std::shared_ptr<Plan> plan,
                   uint8_t *const *shards, const size_t *lens, const uint8_t *present, size_t n,
-                  int data_only, hipStream_t s, int route) {
+                  int data_only, const Exec &ex, int route) {
   if (n && (!shards || !lens || !present)) return TRS_ERR_INVALID_ARGUMENT;
   if (n != total) return n < total ? TRS_TOO_FEW_SHARDS : TRS_TOO_MANY_SHARDS;
   Mask mask;
@@ -1542,21 +1970,53 @@ int reconstruct16(const trs_rs16 *rs, size_t k, size_t total, std::shared_ptr<Pl
   std::vector<const uint8_t *> ptrs;
   size_t nout = 0;
   if ((st = gather(*plan, shards, lens, k, len, data_only != 0, ptrs, nout))) return st;
-  if (use_fft_decode(route, total, k, nout)) {
-    const Fft16Device *d = nullptr;
-    if ((st = plan_fft16(*plan, total, &d))) return st;
-    return run_fft16_program(*d, ptrs, nout, len, s);
-  }
-  if ((st = plan16_matrix(*plan, k))) return st;
-  return run_gf16(ptrs, k, nout, plan->coef, len, kWrite, nullptr, s);
+  if (nout == 0) return TRS_OK;
+  Plan &p = *plan;
+  return dispatch(ex, ptrs, k, len,
+                  [&](const std::vector<const uint8_t *> &q, size_t l, hipStream_t s) {
+                    return rec16_launch(p, k, total, nout, q, l, s, route);
+                  });
+}
+
+// ReedSolomon16::reconstruct_rows, src/reedsolomon16.rs:546-569.
+int rs16_reconstruct_rows(const trs_rs16 *rs, uint8_t *rows, size_t rows_len, size_t row_len,
+                          const uint8_t *present, size_t n_present, const Exec &ex) {
+  if (!rs || (n_present && !present)) return TRS_ERR_INVALID_ARGUMENT;
+  // src/reedsolomon16.rs:552-560
+  if (n_present != rs->n) return TRS_INVALID_SHARD_FLAGS;
+  if (row_len == 0 || (row_len & 1)) return TRS_INCORRECT_SHARD_SIZE;
+  if (rows_len != rs->n * row_len) return TRS_INCORRECT_SHARD_SIZE;
+  if (!rows) return TRS_ERR_INVALID_ARGUMENT;
+  std::vector<uint8_t *> shards(rs->n);
+  std::vector<size_t> lens(rs->n, row_len);
+  for (size_t i = 0; i < rs->n; ++i) shards[i] = rows + i * row_len;
+  return reconstruct16(rs, rs->k, rs->n, nullptr, shards.data(), lens.data(), present, rs->n, 0,
+                       ex, rs->decode_route.load());
 }
 }  // namespace
+
+int trs_rs16_verify(const trs_rs16 *rs, const uint8_t *const *shards, const size_t *lens,
+                    size_t n, int *ok, void *stream) {
+  return rs16_verify(rs, shards, lens, n, ok, as_stream(stream));
+}
+
+int trs_rs16_verify_host(const trs_rs16 *rs, const uint8_t *const *shards, const size_t *lens,
+                         size_t n, int *ok) {
+  return rs16_verify(rs, shards, lens, n, ok, Exec::on_host(2));
+}
 
 int trs_rs16_reconstruct(const trs_rs16 *rs, uint8_t *const *shards, const size_t *lens,
                          const uint8_t *present, size_t n, int data_only, void *stream) {
   if (!rs) return TRS_ERR_INVALID_ARGUMENT;
   return reconstruct16(rs, rs->k, rs->n, nullptr, shards, lens, present, n, data_only,
                        as_stream(stream), rs->decode_route.load());
+}
+
+int trs_rs16_reconstruct_host(const trs_rs16 *rs, uint8_t *const *shards, const size_t *lens,
+                              const uint8_t *present, size_t n, int data_only) {
+  if (!rs) return TRS_ERR_INVALID_ARGUMENT;
+  return reconstruct16(rs, rs->k, rs->n, nullptr, shards, lens, present, n, data_only,
+                       Exec::on_host(2), rs->decode_route.load());
 }
 
 int trs_rs16_set_decode_route(trs_rs16 *rs, int route) {
@@ -1568,17 +2028,14 @@ int trs_rs16_set_decode_route(trs_rs16 *rs, int route) {
 
 int trs_rs16_reconstruct_rows(const trs_rs16 *rs, uint8_t *rows, size_t rows_len, size_t row_len,
                               const uint8_t *present, size_t n_present, void *stream) {
-  if (!rs || (n_present && !present)) return TRS_ERR_INVALID_ARGUMENT;
-  // src/reedsolomon16.rs:552-560
-  if (n_present != rs->n) return TRS_INVALID_SHARD_FLAGS;
-  if (row_len == 0 || (row_len & 1)) return TRS_INCORRECT_SHARD_SIZE;
-  if (rows_len != rs->n * row_len) return TRS_INCORRECT_SHARD_SIZE;
-  if (!rows) return TRS_ERR_INVALID_ARGUMENT;
-  std::vector<uint8_t *> shards(rs->n);
-  std::vector<size_t> lens(rs->n, row_len);
-  for (size_t i = 0; i < rs->n; ++i) shards[i] = rows + i * row_len;
-  return reconstruct16(rs, rs->k, rs->n, nullptr, shards.data(), lens.data(), present, rs->n, 0,
-                       as_stream(stream), rs->decode_route.load());
+  return rs16_reconstruct_rows(rs, rows, rows_len, row_len, present, n_present,
+                               as_stream(stream));
+}
+
+int trs_rs16_reconstruct_rows_host(const trs_rs16 *rs, uint8_t *rows, size_t rows_len,
+                                   size_t row_len, const uint8_t *present, size_t n_present) {
+  return rs16_reconstruct_rows(rs, rows, rows_len, row_len, present, n_present,
+                               Exec::on_host(2));
 }
 
 int trs_rs16_prepare_decode(const trs_rs16 *rs, const uint8_t *present, size_t n,
@@ -1607,6 +2064,13 @@ int trs_dec16_reconstruct(const trs_dec16 *dec, uint8_t *const *shards, const si
   if (!dec) return TRS_ERR_INVALID_ARGUMENT;
   return reconstruct16(nullptr, dec->k, dec->n, dec->plan, shards, lens, present, n, data_only,
                        as_stream(stream), dec->decode_route);
+}
+
+int trs_dec16_reconstruct_host(const trs_dec16 *dec, uint8_t *const *shards, const size_t *lens,
+                               const uint8_t *present, size_t n, int data_only) {
+  if (!dec) return TRS_ERR_INVALID_ARGUMENT;
+  return reconstruct16(nullptr, dec->k, dec->n, dec->plan, shards, lens, present, n, data_only,
+                       Exec::on_host(2), dec->decode_route);
 }
 
 void trs_dec16_free(trs_dec16 *dec) { delete dec; }
@@ -1656,6 +2120,17 @@ struct trs_batch8 {
   hipStream_t capture = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // Recorded after every run on the caller's stream: prepare rewrites `blob`
+  // (pointer tables, job lists, tile maps) with a host copy, so it first waits
+  // for the kernels of earlier runs, which read those tables, to finish.
+  hipEvent_t done = nullptr;
+  bool done_pending = false;
+  hipError_t wait_done() {
+    if (!done_pending) return hipSuccess;
+    const hipError_t e = hipEventSynchronize(done);
+    if (e == hipSuccess) done_pending = false;
+    return e;
+  }
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -1663,8 +2138,10 @@ struct trs_batch8 {
     graph = nullptr;
   }
   ~trs_batch8() {
+    (void)wait_done();
     drop_graph();
     if (capture) (void)hipStreamDestroy(capture);
+    if (done) (void)hipEventDestroy(done);
   }
 };
 extern "C" {
@@ -1679,6 +2156,7 @@ void trs_batch8_free(trs_batch8 *b) { delete b; }
 
 void trs_batch8_clear(trs_batch8 *b) {
   if (!b) return;
+  (void)b->wait_done();
   b->reqs.clear();
   b->prepared = false;
   b->drop_graph();
@@ -1760,6 +2238,7 @@ int trs_batch8_add_reconstruct(trs_batch8 *b, const trs_rs8 *rs, uint8_t *const 
 
 int trs_batch8_prepare(trs_batch8 *b) {
   if (!b) return TRS_ERR_INVALID_ARGUMENT;
+  TRS_TRY(b->wait_done());  // earlier runs still read the tables in `blob`
   b->drop_graph();
   size_t nptr = 0;
   for (auto &r : b->reqs) nptr += r.ptrs.size();
@@ -1939,7 +2418,10 @@ int batch8_launch(trs_batch8 *b, hipStream_t s) {
 int trs_batch8_set_graph(trs_batch8 *b, int enable) {
   if (!b) return TRS_ERR_INVALID_ARGUMENT;
   b->use_graph = enable != 0;
-  if (!b->use_graph) b->drop_graph();
+  if (!b->use_graph) {
+    TRS_TRY(b->wait_done());
+    b->drop_graph();
+  }
   return TRS_OK;
 }
 
@@ -1950,7 +2432,14 @@ int trs_batch8_run(trs_batch8 *b, void *stream) {
     if (st) return st;
   }
   hipStream_t s = as_stream(stream);
-  if (!b->use_graph) return batch8_launch(b, s);
+  if (!b->done) TRS_TRY(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+  if (!b->use_graph) {
+    const int st = batch8_launch(b, s);
+    if (st) return st;
+    TRS_TRY(hipEventRecord(b->done, s));
+    b->done_pending = true;
+    return TRS_OK;
+  }
   if (!b->exec) {
     // The launches read only buffers uploaded by prepare, so the captured
     // graph replays them unchanged; it holds kernel nodes only.
@@ -1967,7 +2456,10 @@ int trs_batch8_run(trs_batch8 *b, void *stream) {
     b->graph = g;
     TRS_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   }
-  return hip_status(hipGraphLaunch(b->exec, s));
+  TRS_TRY(hipGraphLaunch(b->exec, s));
+  TRS_TRY(hipEventRecord(b->done, s));
+  b->done_pending = true;
+  return TRS_OK;
 }
 
 }  // extern "C"
@@ -2543,167 +3035,6 @@ int trs_server_take(trs_server *S, uint64_t ticket, double *latency_us, uint64_t
 // reference's buffers are host slices)
 // ============================================================================
 
-namespace {
-
-// Library-owned staging of one host-resident call: S slots, each holding the
-// call's shard rows for one chunk in device memory and, for pageable
-// callers, in pinned host memory; one stream per pipeline stage.  Kept per
-// host thread and grown on demand.
-struct HostPipe {
-  static constexpr int S = 3;
-  void *dev[S] = {};
-  void *pin[S] = {};
-  size_t dev_bytes = 0, pin_bytes = 0;
-  hipStream_t h2d = nullptr, comp = nullptr, d2h = nullptr;
-  hipEvent_t e_h2d[S] = {}, e_comp[S] = {}, e_d2h[S] = {};
-  bool ready = false;
-  ~HostPipe() {
-    for (int s = 0; s < S; ++s) {
-      if (dev[s]) (void)hipFree(dev[s]);
-      if (pin[s]) (void)hipHostFree(pin[s]);
-      if (e_h2d[s]) (void)hipEventDestroy(e_h2d[s]);
-      if (e_comp[s]) (void)hipEventDestroy(e_comp[s]);
-      if (e_d2h[s]) (void)hipEventDestroy(e_d2h[s]);
-    }
-    if (h2d) (void)hipStreamDestroy(h2d);
-    if (comp) (void)hipStreamDestroy(comp);
-    if (d2h) (void)hipStreamDestroy(d2h);
-  }
-  hipError_t setup(size_t bytes, bool staged) {
-    hipError_t e;
-    if (!ready) {
-      for (hipStream_t *st : {&h2d, &comp, &d2h})
-        if ((e = hipStreamCreateWithFlags(st, hipStreamNonBlocking)) != hipSuccess) return e;
-      for (int s = 0; s < S; ++s)
-        for (hipEvent_t *ev : {&e_h2d[s], &e_comp[s], &e_d2h[s]})
-          if ((e = hipEventCreateWithFlags(ev, hipEventDisableTiming)) != hipSuccess) return e;
-      ready = true;
-    }
-    if (bytes > dev_bytes) {
-      for (int s = 0; s < S; ++s) {
-        if (dev[s]) (void)hipFree(dev[s]);
-        dev[s] = nullptr;
-      }
-      dev_bytes = 0;
-      for (int s = 0; s < S; ++s)
-        if ((e = hipMalloc(&dev[s], bytes)) != hipSuccess) return e;
-      dev_bytes = bytes;
-    }
-    if (staged && bytes > pin_bytes) {
-      for (int s = 0; s < S; ++s) {
-        if (pin[s]) (void)hipHostFree(pin[s]);
-        pin[s] = nullptr;
-      }
-      pin_bytes = 0;
-      for (int s = 0; s < S; ++s)
-        if ((e = hipHostMalloc(&pin[s], bytes, hipHostMallocDefault)) != hipSuccess) return e;
-      pin_bytes = bytes;
-    }
-    return hipSuccess;
-  }
-};
-
-HostPipe &host_pipe() {
-  static thread_local HostPipe pipe;
-  return pipe;
-}
-
-// Pinned (page-locked, device-visible) host memory takes the direct copy path.
-bool pinned_host(const void *p) {
-  hipPointerAttribute_t a{};
-  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
-    (void)hipGetLastError();  // pageable memory is not known to HIP: clear the query's error
-    return false;
-  }
-  return a.type == hipMemoryTypeHost;
-}
-
-constexpr size_t kDefaultHostChunk = size_t(1) << 20;
-
-// One host-resident apply, pipelined over chunks of the shard length: chunk c
-// of the `nin` input rows goes host -> device on one stream, `run` computes on
-// the device rows of its slot on another, and the `nout` output rows come back
-// on a third, so the copies of neighbouring chunks overlap the compute.
-// in_host / out_host are the caller's rows; run(dev_rows, chunk_len, stream)
-// receives nin + nout device row pointers.
-template <typename Run>
-int host_pipeline_body(HostPipe &P, bool direct, const uint8_t *const *in_host, size_t nin,
-                       uint8_t *const *out_host, size_t nout, size_t len, size_t chunk, Run &run);
-
-template <typename Run>
-int host_pipeline(const uint8_t *const *in_host, size_t nin, uint8_t *const *out_host,
-                  size_t nout, size_t len, size_t chunk, Run run) {
-  if (chunk == 0) chunk = kDefaultHostChunk;
-  chunk = std::min(chunk, len);
-  bool direct = true;
-  for (size_t i = 0; i < nin && direct; ++i) direct = pinned_host(in_host[i]);
-  for (size_t j = 0; j < nout && direct; ++j) direct = pinned_host(out_host[j]);
-  HostPipe &P = host_pipe();
-  TRS_TRY(P.setup((nin + nout) * chunk, !direct));
-  const int st = host_pipeline_body(P, direct, in_host, nin, out_host, nout, len, chunk, run);
-  if (st) {  // nothing of this call may still be in flight on return
-    (void)hipStreamSynchronize(P.h2d);
-    (void)hipStreamSynchronize(P.comp);
-    (void)hipStreamSynchronize(P.d2h);
-  }
-  return st;
-}
-
-template <typename Run>
-int host_pipeline_body(HostPipe &P, bool direct, const uint8_t *const *in_host, size_t nin,
-                       uint8_t *const *out_host, size_t nout, size_t len, size_t chunk, Run &run) {
-  const size_t rows = nin + nout;
-  const size_t nchunks = (len + chunk - 1) / chunk;
-  auto drain = [&](size_t c) -> int {  // chunk c's outputs are home once this returns
-    const int s = int(c % HostPipe::S);
-    TRS_TRY(hipEventSynchronize(P.e_d2h[s]));
-    if (!direct) {
-      const size_t off = c * chunk, cl = std::min(chunk, len - off);
-      for (size_t j = 0; j < nout; ++j)
-        std::memcpy(out_host[j] + off, static_cast<uint8_t *>(P.pin[s]) + (nin + j) * chunk, cl);
-    }
-    return TRS_OK;
-  };
-  std::vector<uint8_t *> dev_rows(rows);
-  for (size_t c = 0; c < nchunks; ++c) {
-    const int s = int(c % HostPipe::S);
-    const size_t off = c * chunk, cl = std::min(chunk, len - off);
-    if (c >= size_t(HostPipe::S)) {
-      const int st = drain(c - HostPipe::S);  // frees slot s
-      if (st) return st;
-    }
-    uint8_t *dev = static_cast<uint8_t *>(P.dev[s]);
-    uint8_t *pin = static_cast<uint8_t *>(P.pin[s]);
-    for (size_t r = 0; r < rows; ++r) dev_rows[r] = dev + r * chunk;
-    for (size_t i = 0; i < nin; ++i) {
-      const uint8_t *src = in_host[i] + off;
-      if (!direct) {
-        std::memcpy(pin + i * chunk, src, cl);
-        src = pin + i * chunk;
-      }
-      TRS_TRY(hipMemcpyAsync(dev_rows[i], src, cl, hipMemcpyHostToDevice, P.h2d));
-    }
-    TRS_TRY(hipEventRecord(P.e_h2d[s], P.h2d));
-    TRS_TRY(hipStreamWaitEvent(P.comp, P.e_h2d[s], 0));
-    const int st = run(dev_rows.data(), cl, P.comp);
-    if (st) return st;
-    TRS_TRY(hipEventRecord(P.e_comp[s], P.comp));
-    TRS_TRY(hipStreamWaitEvent(P.d2h, P.e_comp[s], 0));
-    for (size_t j = 0; j < nout; ++j) {
-      uint8_t *dst = direct ? out_host[j] + off : pin + (nin + j) * chunk;
-      TRS_TRY(hipMemcpyAsync(dst, dev_rows[nin + j], cl, hipMemcpyDeviceToHost, P.d2h));
-    }
-    TRS_TRY(hipEventRecord(P.e_d2h[s], P.d2h));
-  }
-  const size_t first = nchunks > size_t(HostPipe::S) ? nchunks - HostPipe::S : 0;
-  for (size_t c = first; c < nchunks; ++c) {
-    const int st = drain(c);
-    if (st) return st;
-  }
-  return TRS_OK;
-}
-
-}  // namespace
 
 extern "C" {
 
@@ -2773,65 +3104,58 @@ int trs_rs8_reconstruct_alloc(const trs_rs8 *rs, uint8_t **shards, size_t *lens,
 
 int trs_device_free(void *p) { return p ? hip_status(hipFree(p)) : TRS_OK; }
 
-int trs_rs8_encode_host(const trs_rs8 *rs, const uint8_t *const *data, uint8_t *const *parity,
-                        size_t len, size_t chunk) {
-  if (!rs || !data || !parity) return TRS_ERR_INVALID_ARGUMENT;
-  if (len == 0) return TRS_EMPTY_SHARD;
-  for (size_t i = 0; i < rs->k; ++i)
-    if (!data[i]) return TRS_ERR_INVALID_ARGUMENT;
-  for (size_t j = 0; j < rs->m; ++j)
-    if (!parity[j]) return TRS_ERR_INVALID_ARGUMENT;
-  const int st = host_pipeline(
-      data, rs->k, parity, rs->m, len, chunk, [&](uint8_t **rows, size_t cl, hipStream_t s) {
-        std::vector<const uint8_t *> ptrs(rows, rows + rs->k + rs->m);
-        return encode8_ptrs(rs, ptrs, cl, s, auto_route8(rs, cl));
-      });
-  return st;
+int trs_device_alloc(size_t bytes, void **out) {
+  if (!out) return TRS_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  return hip_status(hipMalloc(out, bytes ? bytes : 1));
 }
 
-int trs_rs8_reconstruct_host(const trs_rs8 *rs, uint8_t *const *shards, const uint8_t *present,
-                             size_t n, size_t len, int data_only, size_t chunk) {
-  if (!rs || (n && (!shards || !present))) return TRS_ERR_INVALID_ARGUMENT;
-  if (n != rs->n) return n < rs->n ? TRS_TOO_FEW_SHARDS : TRS_TOO_MANY_SHARDS;
-  size_t count = 0;
-  for (size_t i = 0; i < n; ++i) count += present[i] != 0;
-  if (count == n) return TRS_OK;
-  if (len == 0) return TRS_EMPTY_SHARD;
-  if (count < rs->k) return TRS_TOO_FEW_SHARDS_PRESENT;
-  // inputs: the first k present shards (the only ones a plan reads); outputs:
-  // the missing data, then (unless data_only) the missing parity
-  std::vector<const uint8_t *> in_host;
-  std::vector<uint8_t *> out_host;
-  std::vector<size_t> in_idx, out_idx;
-  for (size_t i = 0; i < n; ++i) {
-    if (present[i]) {
-      if (in_idx.size() < rs->k) {
-        in_idx.push_back(i);
-        in_host.push_back(shards[i]);
-      }
-    } else if (i < rs->k || !data_only) {
-      out_idx.push_back(i);
-      out_host.push_back(shards[i]);
-    }
+int trs_memcpy(void *dst, const void *src, size_t bytes, void *stream) {
+  if (bytes && (!dst || !src)) return TRS_ERR_INVALID_ARGUMENT;
+  if (!bytes) return TRS_OK;
+  return hip_status(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, as_stream(stream)));
+}
+
+int trs_stream_synchronize(void *stream) {
+  return hip_status(hipStreamSynchronize(as_stream(stream)));
+}
+
+int trs_set_host_chunk(size_t bytes) {
+  g_host_chunk.store(bytes);
+  return TRS_OK;
+}
+
+// Clone (src/reedsolomon.rs:725-739): the same codec with a copy of the plan
+// cache; the cached plans themselves are immutable and shared.
+int trs_rs8_clone(const trs_rs8 *rs, trs_rs8 **out) {
+  if (!rs || !out) return TRS_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  trs_rs8 *c = nullptr;
+  const int st = trs_rs8_new(rs->k, rs->m, &c);
+  if (st) return st;
+  c->apply_route.store(rs->apply_route.load());
+  c->decode_route.store(rs->decode_route.load());
+  {
+    std::shared_lock<std::shared_mutex> lk(rs->plans.mu);
+    c->plans.entries = rs->plans.entries;
   }
-  for (auto q : in_host)
-    if (!q) return TRS_ERR_INVALID_ARGUMENT;
-  for (auto q : out_host)
-    if (!q) return TRS_ERR_INVALID_ARGUMENT;
-  if (out_idx.empty()) return TRS_OK;  // data_only with every data shard present
-  std::vector<uint8_t> pres(present, present + n);
-  return host_pipeline(
-      in_host.data(), in_host.size(), out_host.data(), out_host.size(), len, chunk,
-      [&](uint8_t **rows, size_t cl, hipStream_t s) {
-        // device view of the stripe: present shards past the first k are never
-        // read by the plan, so they share the first input's row
-        std::vector<uint8_t *> view(n, rows[0]);
-        std::vector<size_t> lens(n, cl);
-        for (size_t r = 0; r < in_idx.size(); ++r) view[in_idx[r]] = rows[r];
-        for (size_t r = 0; r < out_idx.size(); ++r) view[out_idx[r]] = rows[in_idx.size() + r];
-        return reconstruct8(rs, rs->k, rs->n, nullptr, view.data(), lens.data(), pres.data(), n,
-                            data_only, s, rs->apply_route.load());
-      });
+  *out = c;
+  return TRS_OK;
+}
+
+int trs_rs16_clone(const trs_rs16 *rs, trs_rs16 **out) {
+  if (!rs || !out) return TRS_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  trs_rs16 *c = nullptr;
+  const int st = trs_rs16_new(rs->k, rs->m, &c);
+  if (st) return st;
+  c->decode_route.store(rs->decode_route.load());
+  {
+    std::shared_lock<std::shared_mutex> lk(rs->plans.mu);
+    c->plans.entries = rs->plans.entries;
+  }
+  *out = c;
+  return TRS_OK;
 }
 
 }  // extern "C"

@@ -599,6 +599,7 @@ struct Fft16Args {
   uint32_t cblocks;  // workgroups per stripe
   uint32_t total;    // workgroups (single-program launch); first workgroup (job table)
   uint32_t vec_ok;
+  uint32_t *flag;    // verify: compare the stored registers with the shards, set on a mismatch
 };
 
 // XCD-aware order: consecutive workgroups of one XCD take consecutive
@@ -756,6 +757,7 @@ __device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
     __syncthreads();
   }
 
+  uint32_t bad = 0;
   for (uint32_t q = threadIdx.x; q < (a.nstore << a.gshift); q += T) {
     const uint32_t j = q >> a.gshift, g = q & (G - 1);
     const uint32_t cg = g0 + g;
@@ -766,6 +768,21 @@ __device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
     const uint64_t rem = a.cols - col;
     const bool full = a.vec_ok && rem >= kGroupCols;
     uint8_t *d = const_cast<uint8_t *>(ptr[a.nload + j]);
+    if (a.flag) {
+      // Compare in plane form: both sides are the same transpose of their
+      // bytes, and columns past the end are zero on both sides (the program
+      // is linear and its inputs read as zero there).
+      uint32_t x[8];
+      load_planes(d + col, full, rem, x);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) bad |= x[i] ^ v[i];
+      if constexpr (P == 16) {
+        load_planes(d + a.plane + col, full, rem, x);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) bad |= x[i] ^ v[8 + i];
+      }
+      continue;
+    }
     uint32_t lo[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) lo[i] = v[i];
@@ -777,6 +794,7 @@ __device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
       store_planes(d + a.plane + col, full, rem, hi);
     }
   }
+  if (bad) *a.flag = 1u;
 }
 
 }  // namespace
@@ -938,6 +956,7 @@ hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t
   a.total = uint32_t(total);
   // pointers carry any column offset; the high plane sits `plane` bytes on
   a.vec_ok = (call.vec_ok && (byte_planes || call.plane % 16 == 0)) ? 1u : 0u;
+  a.flag = call.flag;
   const size_t lds = (d.regs * reg_bytes) << gs;
   const uint32_t threads = choose_threads(lds, uint64_t(d.max_count) << gs);
   hipError_t e = raise_lds_limit(lds);

@@ -104,6 +104,8 @@ struct Fft16Call {
   uint32_t vec_ok;             // every shard pointer 16-byte aligned
   uint64_t cols;               // columns from each pointer (0: plane); pointers may sit at a
                                // column offset within their low plane
+  uint32_t *flag;              // non-null: verify -- compare the stored registers with the
+                               // shards instead of writing them, *flag = 1 on a mismatch
 };
 
 // Lowers p for the device (glue scheduled into independent sub-steps, whole

@@ -112,6 +112,9 @@ hipError_t launch_batch_mfma(const BatchJob *jobs, const uint32_t *tile_job,
 // (codec.cpp), not a bit-sliced kernel.
 enum EncodeRoute : int { kRouteTable = 0, kRouteBsMatrix = 1, kRouteBsFft = 2, kRouteStaged8 = 3 };
 hipError_t launch_bs_encode(int route, const Gf8Call &call, hipStream_t stream);
+// The same programs comparing their outputs with call.ptr[k..k+m) instead of
+// storing them: *call.flag = 1 on a mismatch (verify).
+hipError_t launch_bs_verify(int route, const Gf8Call &call, hipStream_t stream);
 bool bs_encode_available(int route, uint32_t k, uint32_t m);
 uint32_t bs_program_id(int route, uint32_t k, uint32_t m);
 // Persistent work-list kernels over jobs with a generated program (8 KiB

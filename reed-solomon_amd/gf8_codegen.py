@@ -103,14 +103,17 @@ def fft_program(k: int, m: int):
     return ops, p.register_count
 
 
-def _dispatch(name: str, ins: str, len_: str, vec: str, ind: str) -> str:
+def _dispatch(name: str, ins: str, len_: str, vec: str, ind: str, verify: bool = False) -> str:
     """One branch per tile on its access mode (bitslice.hpp, split_mode): the
-    full and masked paths get straight-line copies of the program."""
+    full and masked paths get straight-line copies of the program.  `verify`
+    compares the outputs with the shards instead of storing them."""
+    fn = f"{name}<{'true' if verify else 'false'}>"
     return "\n".join([
         f"{ind}const int mode = split_mode({vec}, tile, {len_});",
-        f"{ind}if (mode == kSplitFull) {name}({ins}, out, off, {len_}, kSplitFull);",
-        f"{ind}else if (mode == kSplitMasked) {name}({ins}, out, off, {len_}, kSplitMasked);",
-        f"{ind}else {name}({ins}, out, off, {len_}, mode);",
+        f"{ind}uint32_t bad = 0;",
+        f"{ind}if (mode == kSplitFull) {fn}({ins}, out, off, {len_}, kSplitFull, bad);",
+        f"{ind}else if (mode == kSplitMasked) {fn}({ins}, out, off, {len_}, kSplitMasked, bad);",
+        f"{ind}else {fn}({ins}, out, off, {len_}, mode, bad);",
     ])
 
 
@@ -123,7 +126,10 @@ def emit_body(k: int, ops, nreg: int) -> str:
         kind = op[0]
         if kind == "st":
             _, j, r = op
-            out.append(f"  store_split<true>(ld_const_ptr(out + {j}), off, len, mode, R[{r}]);")
+            out.append(f"  if constexpr (V) verify_split<true>(ld_const_ptr(out + {j}), off, len, "
+                       f"mode, R[{r}], bad);")
+            out.append(f"  else store_split<true>(ld_const_ptr(out + {j}), off, len, mode, "
+                       f"R[{r}]);")
         elif kind == "zero":
             out.append(f"  for (int r = 0; r < 8; ++r) R[{op[1]}][r] = 0;")
         elif kind in ("xo", "cp"):
@@ -195,8 +201,9 @@ def generate() -> str:
             name = f"bs_{route}_{k}_{m}"
             parts.append(f"// ({k},{m}) {route}: {nreg} registers, ~{xor_cost(ops)} plane XORs "
                          "per 32-byte lane chunk")
-            parts.append(f"__device__ __forceinline__ void {name}(const uint8_t *const *in, "
-                         "uint8_t *const *out, uint64_t off, uint64_t len, int mode) {")
+            parts.append(f"template <bool V> __device__ __forceinline__ void {name}("
+                         "const uint8_t *const *in, uint8_t *const *out, uint64_t off, "
+                         "uint64_t len, int mode, uint32_t &bad) {")
             parts.append(emit_body(k, ops, nreg))
             parts.append("}")
             parts.append(f"__global__ __launch_bounds__(kBlockThreads) {_waves_attr(route, k, m)}"
@@ -206,6 +213,17 @@ def generate() -> str:
             parts.append("  if (off >= a.len) return;")
             parts.append(f"  uint8_t *const *out = const_cast<uint8_t *const *>(a.ptr + {k});")
             parts.append(_dispatch(name, "a.ptr", "a.len", "a.vec_ok != 0", "  "))
+            parts.append("}")
+            # verify: the same program comparing its outputs with the parity
+            # shards (ReedSolomon::verify); a mismatching lane sets the flag
+            parts.append(f"__global__ __launch_bounds__(kBlockThreads) {_waves_attr(route, k, m)}"
+                         f"void k_{name}_verify(const Gf8Call a) {{")
+            parts.append("  const uint64_t tile = uint64_t(blockIdx.x) * 8192;")
+            parts.append("  const uint64_t off = tile + split_lane_offset(threadIdx.x);")
+            parts.append("  if (off >= a.len) return;")
+            parts.append(f"  uint8_t *const *out = const_cast<uint8_t *const *>(a.ptr + {k});")
+            parts.append(_dispatch(name, "a.ptr", "a.len", "a.vec_ok != 0", "  ", verify=True))
+            parts.append("  if (bad) *a.flag = 1u;")
             parts.append("}")
             parts.append("")
             table.append((route, k, m, name))
@@ -266,6 +284,19 @@ def generate() -> str:
         rid = 1 if route == "mat" else 2
         parts.append(f"  if (route == {rid} && a.k == {k} && a.nout == {m}) {{")
         parts.append(f"    hipLaunchKernelGGL(k_{name}, grid, dim3(kBlockThreads), 0, stream, a);")
+        parts.append("    return hipGetLastError();")
+        parts.append("  }")
+    parts.append("  return hipErrorNotSupported;")
+    parts.append("}")
+    parts.append("")
+    parts.append("// The verify form of launch_bs_encode: a.flag is set to 1 on any mismatch.")
+    parts.append("hipError_t launch_bs_verify(int route, const Gf8Call &a, hipStream_t stream) {")
+    parts.append("  if (a.len == 0) return hipSuccess;")
+    parts.append("  const dim3 grid(uint32_t((a.len + 8191) / 8192));")
+    for route, k, m, name in table:
+        rid = 1 if route == "mat" else 2
+        parts.append(f"  if (route == {rid} && a.k == {k} && a.nout == {m}) {{")
+        parts.append(f"    hipLaunchKernelGGL(k_{name}_verify, grid, dim3(kBlockThreads), 0, stream, a);")
         parts.append("    return hipGetLastError();")
         parts.append("  }")
     parts.append("  return hipErrorNotSupported;")

@@ -169,3 +169,37 @@ def test_batch_fft_jobs_match_oracle(trs):
         for shards, full, _, check in cases:
             got = to_host(shards)
             assert (got[check] == full[check]).all(), graph
+
+
+def test_batch_reprepare_waits_for_previous_run(trs):
+    """clear -> add -> prepare right after run on a non-blocking side stream:
+    prepare rewrites the device work list that the earlier run's kernels read,
+    so it must wait for them; both batches' outputs stay exact."""
+    import torch
+
+    k, m, L = 10, 4, 1 << 20
+    rs = trs.ReedSolomon(k, m)
+    rs.set_apply_route("table")  # the runtime-coefficient list (k_batch8_planes)
+    G = O.build_matrix(8, k, k + m)
+    side = torch.cuda.Stream()
+    sets = []
+    for t in range(2):
+        stripes = []
+        for s in range(12):
+            data = seeded(k, L, 7000 + 100 * t + s)
+            stripes.append((rows_to_dev(np.concatenate([data, np.zeros((m, L), np.uint8)])),
+                            data))
+        sets.append(stripes)
+    torch.cuda.synchronize()
+    b = trs.Batch()
+    for t, stripes in enumerate(sets):
+        b.clear()
+        for shards, _ in stripes:
+            # reconstruct of every parity shard: the runtime-coefficient tile
+            b.add_reconstruct(rs, shards, [True] * k + [False] * m)
+        b.prepare()
+        b.run(side.cuda_stream)
+    side.synchronize()
+    for stripes in sets:
+        for shards, data in stripes:
+            assert (to_host(shards[k:]) == O.rs8_encode(G, k, data)).all()

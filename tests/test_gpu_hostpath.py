@@ -20,43 +20,52 @@ def _stripe(k, m, length, seed):
 @pytest.mark.parametrize("k,m", [(10, 4), (7, 13), (20, 10), (48, 16)])
 @pytest.mark.parametrize("pinned", [False, True])
 def test_encode_host(trs, k, m, pinned):
-    for length, chunk in ((1, 0), (1000, 0), (50_000, 4096), (50_001, 4096),
-                          (300_000, 65536 + 64)):
-        G, data, want = _stripe(k, m, length, 11 + length)
-        if pinned:
-            d = [torch.from_numpy(data[i].copy()).pin_memory() for i in range(k)]
-            p = [torch.zeros(length, dtype=torch.uint8).pin_memory() for _ in range(m)]
-            got = lambda: np.stack([t.numpy() for t in p])
-        else:
-            d = [data[i].copy() for i in range(k)]
-            p = [np.zeros(length, np.uint8) for _ in range(m)]
-            got = lambda: np.stack(p)
-        trs.ReedSolomon(k, m).encode_host(d, p, chunk)
-        assert (got() == want).all(), (k, m, length, chunk, pinned)
+    try:
+        for length, chunk in ((1, 0), (1000, 0), (50_000, 4096), (50_001, 4096),
+                              (300_000, 65536 + 64)):
+            trs.set_host_chunk(chunk)
+            G, data, want = _stripe(k, m, length, 11 + length)
+            if pinned:
+                d = [torch.from_numpy(data[i].copy()).pin_memory() for i in range(k)]
+                p = [torch.zeros(length, dtype=torch.uint8).pin_memory() for _ in range(m)]
+                got = lambda: np.stack([t.numpy() for t in p])
+            else:
+                d = [data[i].copy() for i in range(k)]
+                p = [np.zeros(length, np.uint8) for _ in range(m)]
+                got = lambda: np.stack(p)
+            trs.ReedSolomon(k, m).encode_sep(d, p)
+            assert (got() == want).all(), (k, m, length, chunk, pinned)
+    finally:
+        trs.set_host_chunk(0)
 
 
 @pytest.mark.parametrize("k,m", [(10, 4), (20, 10)])
 def test_reconstruct_host(trs, k, m):
     rng = np.random.default_rng(k)
     rs = trs.ReedSolomon(k, m)
-    for length, chunk in ((777, 0), (100_000, 8192)):
-        G, data, parity = _stripe(k, m, length, 5 + length)
-        full = np.concatenate([data, parity])
-        for data_only in (False, True):
-            erased = set(rng.choice(k + m, m, replace=False).tolist())
-            rows = [full[i].copy() for i in range(k + m)]
-            for i in erased:
-                rows[i][:] = 0
-            present = [i not in erased for i in range(k + m)]
-            rs.reconstruct_host(rows, present, data_only, chunk)
-            for i in range(k + m):
-                if data_only and i >= k and i in erased:
-                    assert not rows[i].any()
-                else:
-                    assert (rows[i] == full[i]).all(), (length, sorted(erased), i)
+    try:
+        for length, chunk in ((777, 0), (100_000, 8192)):
+            trs.set_host_chunk(chunk)
+            G, data, parity = _stripe(k, m, length, 5 + length)
+            full = np.concatenate([data, parity])
+            for data_only in (False, True):
+                erased = set(rng.choice(k + m, m, replace=False).tolist())
+                rows = [full[i].copy() for i in range(k + m)]
+                for i in erased:
+                    rows[i][:] = 0
+                present = [i not in erased for i in range(k + m)]
+                shards = [(r, p) for r, p in zip(rows, present)]
+                (rs.reconstruct_data if data_only else rs.reconstruct)(shards)
+                for i in range(k + m):
+                    if data_only and i >= k and i in erased:
+                        assert not rows[i].any()
+                    else:
+                        assert (rows[i] == full[i]).all(), (length, sorted(erased), i)
+    finally:
+        trs.set_host_chunk(0)
     rows = [np.zeros(64, np.uint8) for _ in range(k + m)]
     with pytest.raises(trs.Error) as e:
-        rs.reconstruct_host(rows, [i < k - 1 for i in range(k + m)])
+        rs.reconstruct([(r, i < k - 1) for i, r in enumerate(rows)])
     assert e.value.name == "TooFewShardsPresent"
 
 

@@ -193,3 +193,21 @@ def test_c3_shape_1024_1024(trs):
         rs.reconstruct(view)
         torch.cuda.synchronize()
         assert torch.equal(buf, original), stripe
+
+
+def test_encode_sep_matrix_rejects_null_shard(trs):
+    """A NULL data or parity pointer is an argument error, never a launch."""
+    import ctypes
+
+    k, m, L = 4, 2, 64
+    rs = trs.ReedSolomon16(k, m)
+    bufs = rows_to_dev(np.zeros((k + m, L), np.uint8))
+    for hole in (1, k + 1):
+        ptrs = [t.data_ptr() for t in bufs]
+        ptrs[hole] = 0
+        data = (ctypes.c_void_p * k)(*ptrs[:k])
+        parity = (ctypes.c_void_p * m)(*ptrs[k:])
+        dl = (ctypes.c_size_t * k)(*([L] * k))
+        pl = (ctypes.c_size_t * m)(*([L] * m))
+        rc = trs._lib.lib.trs_rs16_encode_sep_matrix(rs._h, data, dl, k, parity, pl, m, None)
+        assert rc == -3, hole
