@@ -51,7 +51,8 @@ enum trs_status {
     TRS_ERR_DEVICE = -1,      /* a HIP runtime call failed */
     TRS_ERR_OUT_OF_MEMORY = -2,
     TRS_ERR_INVALID_ARGUMENT = -3, /* where the reference panics (length asserts) */
-    TRS_ERR_UNSUPPORTED = -4
+    TRS_ERR_UNSUPPORTED = -4,
+    TRS_ERR_SCRATCH_LIMIT = -5 /* beyond the allocation-free entry point's scratch */
 };
 
 /* Human-readable text for a status code (src/errors.rs:24-41 for 1..14). */
@@ -232,6 +233,20 @@ int trs_fft8_lowered_program(size_t k, size_t m, uint32_t *steps, size_t steps_c
  * merges the launches. */
 int trs_rs16_encode_stripes(const trs_rs16 *rs, uint8_t *const *shards, size_t stripes,
                             size_t len, int route, void *stream);
+/* trs_rs16_encode_stripes on the tower-FFT route without any device
+ * allocation.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The per-call shard-pointer table goes into
+ * a fixed 1 MiB device array that each GPU holds for the life of the process
+ * (the codec's FFT program is uploaded by trs_rs16_new), so stripes * (k + m)
+ * may not exceed 131072 pointers (e.g. 64 stripes of (1024,1024)):
+ * TRS_ERR_SCRATCH_LIMIT beyond.  TRS_ERR_UNSUPPORTED for a codec without a
+ * staged program that fits the device (trs_rs16_encode_route "matrix").  All
+ * calls on one GPU share that array and its last call's contents: calls that
+ * overlap on one GPU (other host threads, other streams) race; calls ordered
+ * on one stream do not. */
+int trs_rs16_encode_stripes_noalloc(const trs_rs16 *rs, uint8_t *const *shards, size_t stripes,
+                                    size_t len, void *stream);
 /* Host-only: the fused decode rows a reconstruct with flags `present` (n
  * entries) applies -- rows for the missing data shards, then for the missing
  * parity shards, k coefficients each (uint8 for field 8, uint16 for field 16),

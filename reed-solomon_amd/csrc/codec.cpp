@@ -980,6 +980,8 @@ const char *trs_status_message(int s) {
     case TRS_ERR_OUT_OF_MEMORY: return "device out of memory";
     case TRS_ERR_INVALID_ARGUMENT: return "invalid argument";
     case TRS_ERR_UNSUPPORTED: return "unsupported";
+    case TRS_ERR_SCRATCH_LIMIT:
+      return "request exceeds the allocation-free entry point's scratch (1 MiB of shard pointers)";
     default: return "unknown status";
   }
 }
@@ -1476,6 +1478,18 @@ void trs_dec8_free(trs_dec8 *dec) { delete dec; }
 
 // ---------------------------------------------------------------- RS16
 
+namespace {
+// The device form of the codec's program, uploaded once.
+int fft16_device_for(const trs_rs16 *rs, const Fft16Device **out) {
+  if (!rs->has_program) return TRS_ERR_UNSUPPORTED;
+  std::call_once(rs->fft_once, [rs] { rs->fft_status = fft16_device_new(rs->program, &rs->fft); });
+  if (rs->fft_status == hipErrorInvalidValue) return TRS_ERR_UNSUPPORTED;
+  if (rs->fft_status != hipSuccess) return hip_status(rs->fft_status);
+  *out = rs->fft.get();
+  return TRS_OK;
+}
+}  // namespace
+
 int trs_rs16_new(size_t k, size_t m, trs_rs16 **out) {
   if (!out) return TRS_ERR_INVALID_ARGUMENT;
   *out = nullptr;
@@ -1494,6 +1508,11 @@ int trs_rs16_new(size_t k, size_t m, trs_rs16 **out) {
     rs->program = build_staged_program16(k, m);
     rs->has_program = true;
     rs->staged = fft16_fits_device(rs->program);
+  }
+  if (rs->staged) {  // the encode route's device program, uploaded with the codec
+    const Fft16Device *d = nullptr;
+    const int ust = fft16_device_for(rs.get(), &d);
+    if (ust) return ust;
   }
   *out = rs.release();
   return TRS_OK;
@@ -1514,15 +1533,6 @@ int trs_rs16_generator(const trs_rs16 *rs, uint16_t *out, size_t out_len) {
 }
 
 namespace {
-// The device form of the codec's program, uploaded once.
-int fft16_device_for(const trs_rs16 *rs, const Fft16Device **out) {
-  if (!rs->has_program) return TRS_ERR_UNSUPPORTED;
-  std::call_once(rs->fft_once, [rs] { rs->fft_status = fft16_device_new(rs->program, &rs->fft); });
-  if (rs->fft_status == hipErrorInvalidValue) return TRS_ERR_UNSUPPORTED;
-  if (rs->fft_status != hipSuccess) return hip_status(rs->fft_status);
-  *out = rs->fft.get();
-  return TRS_OK;
-}
 
 // Tower-FFT encode of `stripes` stripes; ptrs holds stripes * (k + m) shards.
 int run_fft16(const trs_rs16 *rs, const std::vector<const uint8_t *> &ptrs, size_t stripes,
@@ -1994,6 +2004,37 @@ int rs16_reconstruct_rows(const trs_rs16 *rs, uint8_t *rows, size_t rows_len, si
                        ex, rs->decode_route.load());
 }
 }  // namespace
+
+int trs_rs16_encode_stripes_noalloc(const trs_rs16 *rs, uint8_t *const *shards, size_t stripes,
+                                    size_t len, void *stream) {
+  if (!rs || (stripes && !shards)) return TRS_ERR_INVALID_ARGUMENT;
+  if (stripes == 0) return TRS_OK;
+  if (len == 0) return TRS_EMPTY_SHARD;
+  if (len & 1) return TRS_INCORRECT_SHARD_SIZE;
+  if (!rs->staged) return TRS_ERR_UNSUPPORTED;  // the FFT route only
+  const size_t nptr = stripes * rs->n;
+  if (nptr > kNoAllocScratchBytes / sizeof(void *)) return TRS_ERR_SCRATCH_LIMIT;
+  for (size_t i = 0; i < nptr; ++i)
+    if (!shards[i]) return TRS_ERR_INVALID_ARGUMENT;
+  // uploaded by trs_rs16_new, so this only reads the codec
+  const Fft16Device *d = nullptr;
+  int st = fft16_device_for(rs, &d);
+  if (st) return st;
+  void *table = nullptr;  // the device address of this GPU's scratch array
+  TRS_TRY(noalloc_scratch(&table));
+  hipStream_t s = as_stream(stream);
+  // pageable source: the runtime has read it when the call returns
+  TRS_TRY(hipMemcpyAsync(table, shards, nptr * sizeof(void *), hipMemcpyHostToDevice, s));
+  bool al = true;
+  for (size_t i = 0; i < nptr; ++i) al = al && aligned16(shards[i]);
+  Fft16Call c{};
+  c.ptrs = static_cast<const uint8_t *const *>(table);
+  c.plane = len / 2;
+  c.stripes = uint32_t(stripes);
+  c.nstore = uint32_t(fft16_device_stores(*d));
+  c.vec_ok = al ? 1u : 0u;
+  return hip_status(launch_fft16(*d, c, s));
+}
 
 int trs_rs16_verify(const trs_rs16 *rs, const uint8_t *const *shards, const size_t *lens,
                     size_t n, int *ok, void *stream) {

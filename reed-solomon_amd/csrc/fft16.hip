@@ -60,6 +60,12 @@ constexpr size_t kLdsTarget = 65536;    // two workgroups per CU
 constexpr size_t kLdsMax = 160 * 1024;  // gfx950 LDS per workgroup
 constexpr uint32_t kMaxGroupsPerWg = 8;
 
+// Device memory of the allocation-free entry point: every GPU the library runs
+// on holds one copy of this array until the process ends.  Host code reaches
+// it only through the per-device address noalloc_scratch returns (the array's
+// name is a host-side stand-in, not a device address).
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
 uint32_t kara_word(uint16_t c) {
   const uint8_t hi = uint8_t(c >> 8), lo = uint8_t(c & 0xff);
   const uint8_t a0 = lo, a1 = uint8_t(hi ^ lo), a2 = gf8().mul[hi][kTowerL];
@@ -1041,6 +1047,24 @@ hipError_t fft16_jobs_launch(const Fft16JobsPlan &plan, const void *dev_blob, hi
   }
 #undef TRS_JOBS
   return hipGetLastError();
+}
+
+hipError_t noalloc_scratch(void **out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void *addr[kMaxDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {  // once per device, with that device current
+    void *p = nullptr;
+    if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess) return e;
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return hipSuccess;
 }
 
 hipError_t launch_fft16_jobs(const std::vector<Fft16JobSpec> &jobs, hipStream_t stream,

@@ -194,4 +194,11 @@ hipError_t fft16_jobs_plan(const std::vector<Fft16JobSpec> &jobs, bool byte_plan
                            Fft16JobsPlan *plan);
 hipError_t fft16_jobs_launch(const Fft16JobsPlan &plan, const void *dev_blob, hipStream_t stream);
 
+// Scratch of the allocation-free entry point: a file-scope device array of
+// kNoAllocScratchBytes in fft16.hip.  noalloc_scratch returns its address on
+// the current device (looked up once per device); every call on one device
+// shares it.
+constexpr size_t kNoAllocScratchBytes = size_t(1) << 20;
+hipError_t noalloc_scratch(void **out);
+
 }  // namespace trs
