@@ -190,6 +190,45 @@ def host_leg(trs, torch, O, L: int = 32 << 20, reps: int = 3):
     return out
 
 
+def costmodel_leg(trs, n_train: int = 240, n_test: int = 60, seed: int = 0xC057):
+    """Learned kernel choice (the cost-model configuration, scaled down to run
+    in the default bench): shapes sampled as in the full study (field, k, m,
+    log-uniform lengths 1 KiB - 4 MiB), every kernel variant of each timed on
+    the GPU, the FP32 MLP trained on the GPU on the first n_train shapes; on the
+    n_test held-out shapes, the regret against the exhaustive optimum of the
+    model's measured top 3, of its top-1 pick, of the codec's own rules and of
+    the reference's routing (both without a model)."""
+    from reed_solomon_amd import autotune as A
+
+    t0 = time.perf_counter()
+    shapes = A.sample_shapes(n_train + n_test, seed)
+    timings = [A.exhaustive(s) for s in shapes]
+    t1 = time.perf_counter()
+    X, T = A.dataset(shapes[:n_train], timings[:n_train])
+    cm = A.CostModel(seed=1)
+    loss = cm.fit(X, T, steps=1500)
+    t2 = time.perf_counter()
+    held, meas = shapes[n_train:], timings[n_train:]
+
+    def stats(choose):
+        r = np.array([A.regret(t, choose(s, t)) for s, t in zip(held, meas)])
+        return {"mean": round(float(r.mean()), 4), "p90": round(float(np.percentile(r, 90)), 4),
+                "within_10pct": round(float((r <= 0.10).mean()), 4)}
+
+    return {
+        "workload": f"{n_train + n_test} sampled shapes (GF(2^8)/GF(2^16), k, m, 1 KiB - 4 MiB), "
+                    f"{n_test} held out; candidates = kernel variants",
+        "train_samples": int(len(T)), "train_loss": round(float(loss), 5),
+        "measure_s": round(t1 - t0, 1), "train_s": round(t2 - t1, 1),
+        "candidates_per_shape": round(float(np.mean([len(t) for t in meas])), 2),
+        "regret_model_top3": stats(lambda s, t: A.Tuner(cm, top_n=3).choose(s, t)),
+        "regret_model_top1": stats(lambda s, t: A.Tuner(cm, top_n=1).choose(s, t)),
+        "regret_codec_rules": stats(lambda s, t: A.rule(s)),
+        "regret_reference_routing": stats(lambda s, t: A.heuristic(s)),
+        "seed": seed,
+    }
+
+
 def wide_leg(trs, torch, reps: int = 20):
     """GF(2^8) shapes with k > 32 (SURVEY (f) 4 and (f) 1 over GF(2^8)): per
     call, the staged FFT program (src/fft.rs:393-592) against the matrix route
@@ -654,6 +693,7 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=800)
     ap.add_argument("--no-tower", action="store_true")
+    ap.add_argument("--no-costmodel", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-wide", action="store_true",
                     help="skip the k > 32 leg (staged FFT and FFT decoder vs matrix)")
@@ -853,6 +893,10 @@ def main() -> int:
     if not args.no_wide:
         wide = wide_leg(trs, torch)
 
+    costmodel = None
+    if not args.no_costmodel:
+        costmodel = costmodel_leg(trs)
+
     served = None
     if not args.no_serve:
         served = served_leg(trs, torch, args.serve_requests, [1000, 5000, 20000, 50000],
@@ -914,6 +958,7 @@ def main() -> int:
         "host_path": host,
         "wide": wide,
         "tower": tower,
+        "cost_model": costmodel,
         "served": served,
         "byte_exact": {"batched_reconstruct": rec_exact, "batched_encode": enc_exact,
                        "after_timed_steps": exact_after, "per_call": exact_percall,

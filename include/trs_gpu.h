@@ -476,22 +476,52 @@ int trs_u16_to_split(const uint16_t *symbols, size_t n_symbols, uint8_t *shard, 
 int trs_split_to_u16(const uint8_t *shard, size_t n_symbols, uint16_t *symbols, void *stream);
 
 /* ================= autotuning (no reference counterpart) ================== */
-/* Times one apply candidate on synthetic device data: `nout` outputs from `k`
- * inputs of `len` bytes over GF(2^8) (field 8) or the tower (field 16,
- * split-plane, even len), random nonzero coefficients, through `route`
- * (0 = matrix route with runtime coefficients, dispatched by strip length as
- * the codec's single calls are: the table kernel below 128 KiB, the
- * bit-sliced planes kernel above; 1 = FFT route: the generated bit-sliced
- * GF(2^8) program of the shape where it has one, else the staged LCH FFT
- * encode program of shape (k, nout) on the LDS interpreter -- GF(2^8)
- * (src/fft.rs:393-592; even len and strip) or the tower (2 <= k <= 1024,
- * k + nout <= 4096); 2 = int8 MFMA, GF(2^8), k <= 32) in column strips of
- * `strip` bytes (0 = whole shard; one launch per strip).  *ms_out = mean milliseconds of `reps`
- * timed applies after one warm-up.  TRS_ERR_UNSUPPORTED for a route the shape
- * has no kernel for.  The reference tunes by hand (STRIP_BYTES,
- * src/reedsolomon16.rs:796-804; MULT_MARGIN, src/reedsolomon.rs:226-232). */
-int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len, int route, uint64_t strip,
-                   int reps, double *ms_out);
+/* Kernel variants of one apply of k inputs to nout outputs (the autotuner's
+ * and the learned router's candidates; every variant is byte-identical):
+ *   0 table tile (4 KiB column chunks, rows by nout)
+ *   1 / 2 bit-sliced planes tile with 4 / 8 output rows per workgroup
+ *   3 int8 MFMA on the bit-expanded matrix (GF(2^8), k <= 32)
+ *   4 / 5 / 6 the shape's FFT program (encode): the generated bit-sliced
+ *     program where the GF(2^8) shape has one (variant 5 only), else the
+ *     staged program on the LDS interpreter -- GF(2^8) byte planes
+ *     (src/fft.rs:393-592) or the tower (src/fft16.rs:357-376) -- sized for
+ *     16 / 32 / 64 KiB of LDS per workgroup; targets that give the same launch
+ *     are one variant.
+ * The reference tunes by hand: MULT_MARGIN (src/reedsolomon.rs:226-232),
+ * STRIP_BYTES (src/reedsolomon16.rs:796-804), 9 mults < 2 k m
+ * (src/fft16.rs:105). */
+/* *mask: bit v set when variant v can encode shape (k, nout) over GF(field)
+ * at this shard length (the tower's len is the split-plane shard length). */
+int trs_tune_feasible(int field, uint32_t k, uint32_t nout, uint64_t len, uint32_t *mask);
+/* The variant the codec's rules pick for that encode (no model). */
+int trs_tune_rule(int field, uint32_t k, uint32_t nout, uint64_t len, int *variant);
+/* The cost model's 12 inputs for (shape, length, variant): field, log2 k,
+ * log2 nout, log2 len, log2 k*nout, log2 bytes moved, four kernel-kind flags,
+ * log2 of the tile parameter (rows or LDS KiB), k / (k + nout). */
+int trs_tune_features(int field, uint32_t k, uint32_t nout, uint64_t len, int variant,
+                      float *out, size_t n);
+/* Times variant `variant` on synthetic device data (random nonzero
+ * coefficients; kernel time does not depend on their values): *ms_out = mean
+ * milliseconds of `reps` applies after one warm-up, or the warm-up's time when
+ * that alone exceeds limit_ms (> 0; a candidate already far behind).
+ * TRS_ERR_UNSUPPORTED for an infeasible variant. */
+int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len, int variant, int reps,
+                   double limit_ms, double *ms_out);
+/* The learned router.  Installs a trained cost model (the trs_mlp parameters
+ * and the feature standardisation: 12 means then 12 standard deviations):
+ * from then on an automatic encode (both codecs) or matrix-plan reconstruct
+ * (GF(2^8)) whose (field, op, k, nout, half-octave length bucket) has no
+ * cached choice ranks its feasible variants with the model on the GPU, runs
+ * the top `top_n` on the call's own buffers (a warm-up and a timed run each;
+ * the call then synchronises its stream once) and caches the fastest; later
+ * calls of that key run the cached variant.  Calls with a forced route
+ * (trs_rs8_set_apply_route) and the FFT erasure decoders keep their rules.
+ * Process-wide; installing again or removing drops the cache. */
+int trs_tuner_install(const float *params, size_t n_params, const float *norm, size_t n_norm,
+                      int top_n);
+int trs_tuner_remove(void);
+/* The cached choice for a key (op 0 encode, 1 reconstruct), -1 when none. */
+int trs_tuner_lookup(int field, int op, size_t k, size_t nout, size_t len, int *variant);
 
 /* FP32 MLP cost model (12 -> 64 -> 64 -> 1, ReLU), trained and evaluated by
  * hand-written HIP kernels; it predicts log kernel time from shape / route /

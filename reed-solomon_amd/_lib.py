@@ -128,7 +128,16 @@ _sig("trs_batch8_size", c_size, c_vp)
 
 c_fp = ctypes.POINTER(ctypes.c_float)
 _sig("trs_tune_apply", c_int, c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, c_int,
-     ctypes.c_uint64, c_int, ctypes.POINTER(ctypes.c_double))
+     c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double))
+_sig("trs_tune_feasible", c_int, c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+     ctypes.POINTER(ctypes.c_uint32))
+_sig("trs_tune_rule", c_int, c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+     ctypes.POINTER(c_int))
+_sig("trs_tune_features", c_int, c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, c_int,
+     c_fp, c_size)
+_sig("trs_tuner_install", c_int, c_fp, c_size, c_fp, c_size, c_int)
+_sig("trs_tuner_remove", c_int)
+_sig("trs_tuner_lookup", c_int, c_int, c_int, c_size, c_size, c_size, ctypes.POINTER(c_int))
 _sig("trs_mlp_param_count", c_int)
 _sig("trs_mlp_new", c_int, ctypes.c_uint64, ctypes.POINTER(c_vp))
 _sig("trs_mlp_free", None, c_vp)

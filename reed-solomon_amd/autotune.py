@@ -1,42 +1,49 @@
-"""Autotuner and learned cost model for route / strip selection.
+"""Autotuner and learned cost model for kernel selection.
 
-Candidates for one apply shape (field, k inputs, m outputs, shard length L)
-are a device route -- "table" (the matrix route with runtime coefficients:
-the table kernel below 128 KiB strips, the bit-sliced planes kernel above),
-"bitsliced" (the FFT route: the generated bit-sliced LCH program of a GF(2^8)
-shape that has one, else the staged LCH FFT program on the LDS interpreter --
-GF(2^8) on byte planes, src/fft.rs:393-592, or the tower) or "mfma"
-(bit-sliced int8 MFMA, GF(2^8), k <= 32) -- times a
-column strip size (one launch per strip, 0 = whole shard), the GPU analogue of
-the reference's hand-tuned STRIP_BYTES (src/reedsolomon16.rs:796-804) and of
-its FFT-vs-matrix routing rule (src/fft16.rs:85-106, src/reedsolomon.rs:204-234).
+Candidates for one encode-shaped apply (field, k inputs, m outputs, shard
+length L) are the library's kernel variants (include/trs_gpu.h): the table
+tile, the bit-sliced planes tile with 4 or 8 rows per workgroup, int8 MFMA
+(GF(2^8), k <= 32), and the shape's FFT program -- the generated bit-sliced
+program of a GF(2^8) shape that has one, else the staged LCH program on the
+LDS interpreter (GF(2^8) byte planes, src/fft.rs:393-592, or the tower,
+src/fft16.rs:357-376) sized for 16 / 32 / 64 KiB of LDS per workgroup.
+Launch parameters that come out the same are one candidate.  These replace
+the reference's hand-tuned constants: MULT_MARGIN (src/reedsolomon.rs:226-232),
+STRIP_BYTES (src/reedsolomon16.rs:796-804) and its FFT-vs-matrix rule
+(src/fft16.rs:85-106).  The library computes the candidate set, the model's
+features and its own rules, so training here and routing in the library see
+the same numbers.
 
-* ``exhaustive(shape)`` measures every candidate with ``trs_tune_apply``
-  (kernel time on synthetic device data, HIP events).  Applies larger than
-  ``cap_bytes`` are timed on a prefix window of the columns and scaled by
-  L / window: kernel time is linear in the column count at those sizes.
+* ``exhaustive(shape)`` times every candidate with ``trs_tune_apply`` (HIP
+  events, synthetic device data).  Applies larger than ``cap_bytes`` are
+  timed on a prefix window of the columns and scaled by L / window: kernel
+  time is linear in the column count at those sizes.
 * ``CostModel`` wraps the library's GPU MLP (12 features -> 64 -> 64 -> 1,
-  log time), trained on the exhaustive timings.
-* ``Tuner.choose(shape)`` ranks all candidates with the model and measures
+  log time), trained on the exhaustive timings; ``install()`` hands it to the
+  library's router, which then ranks the candidates of every automatic call
+  it has not seen, times the top few on that call and caches the winner.
+* ``Tuner.choose(shape)`` ranks the candidates with the model and measures
   only the top ``top_n``; the exhaustive optimum judges its regret.
-* ``heuristic(shape)`` is the method without the model: the reference's
-  routing rules, whole shard -- GF(2^8): the generated program when the shape
-  has one, the staged program for the shapes src/reedsolomon.rs:204-234
-  takes; GF(2^16): the FFT when 9 * multiplies < 2 k m (src/fft16.rs:105).
+* ``rule(shape)`` is the codec's own GPU-calibrated rule (no model);
+  ``heuristic(shape)`` the reference's CPU routing -- GF(2^8): the generated
+  program when the shape has one, the staged program for the shapes
+  src/reedsolomon.rs:204-234 takes; GF(2^16): the FFT when
+  9 * multiplies < 2 k m (src/fft16.rs:105) -- with the codec's matrix kernel
+  otherwise.
 """
 from __future__ import annotations
 
 import ctypes
 import math
 from dataclasses import dataclass
-from typing import Dict, List, Optional, Sequence, Tuple
+from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
 from ._lib import lib
 
-ROUTES = ("table", "bitsliced", "mfma")
-STRIPS = (0, 8 << 10, 16 << 10, 32 << 10, 64 << 10, 256 << 10, 1 << 20, 4 << 20)
+VARIANTS = ("table", "planes4", "planes8", "mfma", "program16k", "program32k", "program64k")
+PROGRAM_DEFAULT = {8: 5, 16: 6}  # the FFT program at the codec's default LDS target
 GENERATED = {(7, 13), (10, 10), (14, 14), (16, 16), (18, 6), (32, 32), (10, 4)}
 FEATURES = 12
 
@@ -49,7 +56,7 @@ class Shape:
     length: int  # bytes per shard
 
 
-Candidate = Tuple[str, int]  # (route, strip bytes)
+Candidate = int  # variant id, an index into VARIANTS
 
 
 def sample_shapes(n: int, seed: int) -> List[Shape]:
@@ -75,88 +82,95 @@ def sample_shapes(n: int, seed: int) -> List[Shape]:
     return out
 
 
-def _tower_fft(s: Shape) -> bool:
-    return 2 <= s.k <= 1024 and s.k + s.m <= 4096
-
-
-def _byte_staged(s: Shape) -> bool:
-    return s.field == 8 and (s.k, s.m) not in GENERATED
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc})")
 
 
 def candidates(s: Shape) -> List[Candidate]:
-    routes = ["table"]
-    if (s.field == 8 and (s.k, s.m) in GENERATED) or (s.field == 16 and _tower_fft(s)) \
-            or _byte_staged(s):
-        routes.append("bitsliced")
-    if s.field == 8 and s.k <= 32:
-        routes.append("mfma")
-    return [(r, st) for r in routes for st in STRIPS]
+    mask = ctypes.c_uint32(0)
+    _check(lib.trs_tune_feasible(s.field, s.k, s.m, s.length, ctypes.byref(mask)),
+           "trs_tune_feasible")
+    return [v for v in range(len(VARIANTS)) if (mask.value >> v) & 1]
 
 
 def features(s: Shape, c: Candidate) -> np.ndarray:
-    route, strip = c
-    eff = s.length if strip == 0 or strip >= s.length else strip
-    nstrips = math.ceil(s.length / eff)
-    return np.array([
-        1.0 if s.field == 16 else 0.0,
-        math.log2(s.k),
-        math.log2(s.m),
-        math.log2(s.length),
-        math.log2(s.k * s.m),
-        math.log2((s.k + s.m) * s.length),
-        1.0 if route == "table" else 0.0,
-        1.0 if route == "bitsliced" else 0.0,
-        1.0 if route == "mfma" else 0.0,
-        math.log2(eff),
-        math.log2(nstrips),
-        s.k / (s.k + s.m),
-    ], np.float32)
+    out = np.zeros(FEATURES, np.float32)
+    _check(lib.trs_tune_features(s.field, s.k, s.m, s.length, c,
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), FEATURES),
+           "trs_tune_features")
+    return out
 
 
-def measure(s: Shape, c: Candidate, cap_bytes: int = 32 << 20, reps: int = 2) -> Optional[float]:
+def measure(s: Shape, c: Candidate, cap_bytes: int = 64 << 20, min_window: int = 1 << 20,
+            reps: int = 2, limit_ms: float = 0.0) -> Optional[float]:
     """Milliseconds of one apply of candidate c on shape s, or None if the
-    route has no kernel for the shape."""
-    route, strip = c
-    window = max(64, min(s.length, cap_bytes // (s.k + s.m)))
+    variant has no kernel for the shape.  A long apply is timed on a prefix
+    window of its columns -- at least min_window bytes per shard, so that every
+    kernel still gets a full-size grid, else cap_bytes over all shards -- and
+    scaled by L / window.  limit_ms: report the warm-up alone when it is
+    already slower."""
+    window = min(s.length, max(min_window, cap_bytes // (s.k + s.m)))
     if window < s.length:
         window -= window % 64  # a column window keeps the shard's 16-byte vector accesses
     window -= window % (2 if s.field == 16 else 1)
     window = max(2, window)
     ms = ctypes.c_double(0.0)
-    rc = lib.trs_tune_apply(s.field, s.k, s.m, window, ROUTES.index(route), strip, reps,
+    rc = lib.trs_tune_apply(s.field, s.k, s.m, window, c, reps, limit_ms * window / s.length,
                             ctypes.byref(ms))
     if rc == -4:
         return None
-    if rc != 0:
-        raise RuntimeError(f"trs_tune_apply failed ({rc}) for {s} {c}")
+    _check(rc, f"trs_tune_apply {s} {VARIANTS[c]}")
     return ms.value * (s.length / window)
 
 
 def exhaustive(s: Shape, **kw) -> Dict[Candidate, float]:
-    out = {}
-    for c in candidates(s):
-        t = measure(s, c, **kw)
+    """Every candidate's time; the FFT programs (usually fastest) first, and a
+    candidate whose warm-up is already 10x the best so far keeps that time."""
+    out: Dict[Candidate, float] = {}
+    for c in sorted(candidates(s), key=lambda v: -v):
+        best = min(out.values()) if out else 0.0
+        t = measure(s, c, limit_ms=10.0 * best, **kw)
         if t is not None:
             out[c] = t
     return out
 
 
+def rule(s: Shape) -> Candidate:
+    """The codec's own GPU-calibrated choice for an automatic encode."""
+    v = ctypes.c_int(-1)
+    _check(lib.trs_tune_rule(s.field, s.k, s.m, s.length, ctypes.byref(v)), "trs_tune_rule")
+    return v.value
+
+
 def heuristic(s: Shape) -> Candidate:
+    """The reference's CPU routing, with the codec's matrix kernel for its
+    matrix route."""
+    fft = False
     if s.field == 8 and (s.k, s.m) in GENERATED:
-        return ("bitsliced", 0)
-    if _byte_staged(s) and s.length >= 32:
+        fft = True
+    elif s.field == 8 and s.length >= 32:
         from . import fft8_program_info
 
         info = fft8_program_info(s.k, s.m)
-        if info is not None and info["reference_route"]:
-            return ("bitsliced", 0)
-    if s.field == 16 and _tower_fft(s):
+        fft = info is not None and bool(info["reference_route"])
+    elif s.field == 16 and 2 <= s.k <= 1024 and s.k + s.m <= 4096:
         from . import fft16_program_info
 
         info = fft16_program_info(s.k, s.m)
-        if info is not None and info["profitable"]:
-            return ("bitsliced", 0)
-    return ("table", 0)
+        fft = info is not None and bool(info["profitable"])
+    cands = candidates(s)
+    if fft and PROGRAM_DEFAULT[s.field] in cands:
+        return PROGRAM_DEFAULT[s.field]
+    # the codec's matrix kernel by length (its rule without the FFT program)
+    if s.field == 16:
+        return 0
+    if 3 in cands and s.length < (512 << 10):
+        return 3
+    if s.length < (128 << 10):
+        return 0
+    wg8 = math.ceil(s.length / 8192) * math.ceil(s.m / 8)
+    return 1 if s.m <= 4 or wg8 <= 640 else 2
 
 
 class CostModel:
@@ -232,6 +246,25 @@ class CostModel:
     def predict_ms(self, X: np.ndarray) -> np.ndarray:
         y = self.raw_predict((X - self.mu) / self.sd)
         return np.exp(y * self.tsd + self.tmu)
+
+    def install(self, top_n: int = 3) -> None:
+        """Hands the trained model to the library's router (trs_tuner_install)."""
+        p = self.params()
+        norm = np.ascontiguousarray(np.concatenate([self.mu, self.sd]), np.float32)
+        _check(lib.trs_tuner_install(self._fp(p), p.size, self._fp(norm), norm.size, top_n),
+               "trs_tuner_install")
+
+
+def uninstall() -> None:
+    """Back to the codec's rules (trs_tuner_remove)."""
+    _check(lib.trs_tuner_remove(), "trs_tuner_remove")
+
+
+def routed_choice(field: int, op: int, k: int, nout: int, length: int) -> int:
+    """The router's cached variant for a key (op 0 encode, 1 reconstruct), -1 if none."""
+    v = ctypes.c_int(-1)
+    _check(lib.trs_tuner_lookup(field, op, k, nout, length, ctypes.byref(v)), "trs_tuner_lookup")
+    return v.value
 
 
 class Tuner:

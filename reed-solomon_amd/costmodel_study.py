@@ -6,8 +6,9 @@ training on the GPU, regret on held-out shapes.
 
 Prints progress lines while measuring and one JSON summary at the end:
 mean / p90 / max regret of the model-guided choice (top-n measured), of the
-model's top-1 pick (no measurement) and of the no-model heuristic, all
-against the exhaustive optimum of each held-out shape.
+model's top-1 pick (no measurement), of the codec's own GPU-calibrated rules
+and of the reference's routing (both without a model), all against the
+exhaustive optimum of each held-out shape.
 """
 import argparse
 import json
@@ -55,17 +56,18 @@ def main() -> int:
     loss = model.fit(Xtr, Ttr, steps=args.steps, lr=args.lr)
     train_s = time.time() - t1
     tuner = A.Tuner(model, top_n=args.top_n)
-    reg_model, reg_top1, reg_heur, measured_counts = [], [], [], []
+    reg_model, reg_top1, reg_heur, reg_rule, measured_counts = [], [], [], [], []
     detail = open(args.detail, "w") if args.detail else None
     for idx, (s, meas) in enumerate(zip(shapes, timings)):
         if detail is None:
             break
         best = min(meas, key=meas.get)
         rec = {"field": s.field, "k": s.k, "m": s.m, "length": s.length, "held_out": idx >= ntrain,
-               "best": list(best), "best_ms": meas[best], "heuristic": list(A.heuristic(s)),
-               "times_ms": {f"{r}/{st}": t for (r, st), t in meas.items()}}
+               "best": A.VARIANTS[best], "best_ms": meas[best],
+               "heuristic": A.VARIANTS[A.heuristic(s)], "rule": A.VARIANTS[A.rule(s)],
+               "times_ms": {A.VARIANTS[v]: t for v, t in meas.items()}}
         if idx >= ntrain:
-            rec["model"] = list(tuner.choose(s, meas))
+            rec["model"] = A.VARIANTS[tuner.choose(s, meas)]
         detail.write(json.dumps(rec) + "\n")
     if detail is not None:
         detail.close()
@@ -73,6 +75,7 @@ def main() -> int:
         reg_model.append(A.regret(meas, tuner.choose(s, meas)))
         reg_top1.append(A.regret(meas, A.Tuner(model, top_n=1).choose(s, meas)))
         reg_heur.append(A.regret(meas, A.heuristic(s)))
+        reg_rule.append(A.regret(meas, A.rule(s)))
         measured_counts.append(len(meas))
 
     def stats(r):
@@ -89,6 +92,7 @@ def main() -> int:
         "regret_model_top_n": stats(reg_model),
         "regret_model_top1": stats(reg_top1),
         "regret_heuristic_no_model": stats(reg_heur),
+        "regret_codec_rules_no_model": stats(reg_rule),
     }
     print(json.dumps(summary), flush=True)
     if args.out:

@@ -22,8 +22,12 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <cmath>
+#include <map>
 #include <mutex>
+#include <numeric>
 #include <shared_mutex>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -431,7 +435,7 @@ struct PtrTable {
 // flag non-null: verify (compare the stored registers with the shards).
 int launch_fft16_one(const Fft16Device &d, const std::vector<const uint8_t *> &ptrs,
                      size_t nstore, uint64_t plane, hipStream_t stream, bool byte_planes = false,
-                     uint32_t *flag = nullptr) {
+                     uint32_t *flag = nullptr, uint64_t lds_target = 0) {
   bool al = true;
   for (auto q : ptrs) al = al && aligned16(q);
   Fft16Call c{};
@@ -440,6 +444,7 @@ int launch_fft16_one(const Fft16Device &d, const std::vector<const uint8_t *> &p
   c.nstore = uint32_t(nstore);
   c.vec_ok = al ? 1u : 0u;
   c.flag = flag;
+  c.lds_target = lds_target;
   if (ptrs.size() <= kFftInlinePtrs) {
     Fft16Ptrs P{};
     std::copy(ptrs.begin(), ptrs.end(), P.p);
@@ -842,8 +847,9 @@ size_t planes_min_len() {
   return v;
 }
 
+// variant: the kernel (kVarTable, kVarPlanes4, kVarPlanes8); -1 = by length.
 int run_gf8(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout, const DevCoef &coef,
-            size_t len, uint32_t mode, uint32_t *flag, hipStream_t stream) {
+            size_t len, uint32_t mode, uint32_t *flag, hipStream_t stream, int variant = -1) {
   if (nout == 0 || len == 0) return TRS_OK;
   if (k + nout > size_t(kInlinePtrs)) return TRS_ERR_INVALID_ARGUMENT;
   Gf8Call c{};
@@ -859,7 +865,9 @@ int run_gf8(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout, con
   c.ostride = coef.ostride;
   c.vec_ok = al ? 1u : 0u;
   c.flag = flag;
-  if (len >= planes_min_len()) return hip_status(launch_gf8_planes(c, mode, stream));
+  if (variant == 1) return hip_status(launch_gf8_planes(c, mode, stream, 4));
+  if (variant == 2) return hip_status(launch_gf8_planes(c, mode, stream, kPlanesOut));
+  if (variant < 0 && len >= planes_min_len()) return hip_status(launch_gf8_planes(c, mode, stream));
   return hip_status(launch_gf8_apply(c, mode, stream));
 }
 
@@ -927,6 +935,186 @@ int read_flag(uint32_t *flag, hipStream_t stream, int *ok) {
   TRS_TRY(hipMemcpyAsync(&h, flag, sizeof h, hipMemcpyDeviceToHost, stream));
   TRS_TRY(hipStreamSynchronize(stream));
   *ok = h ? 0 : 1;
+  return TRS_OK;
+}
+
+// ---- kernel variants and the learned router ---------------------------------
+//
+// One apply of k inputs to nout outputs over `len` bytes per shard can run
+// on several kernels, all byte-identical:
+//   kVarTable       packed-byte table tile (4 KiB chunks; 4/8/16 rows by nout)
+//   kVarPlanes4/8   bit-sliced planes tile, 4 or 8 output rows per workgroup
+//   kVarMfma        int8 MFMA on the bit-expanded matrix (GF(2^8), k <= 32)
+//   kVarProgram*    the shape's FFT program (encode only): the generated
+//                   bit-sliced program where it has one (one variant), else
+//                   the staged program on the LDS interpreter sized for 16,
+//                   32 or 64 KiB of LDS per workgroup (launches that come out
+//                   the same are one candidate).
+// The codec's rules (measured thresholds) pick one.  With a trained cost
+// model installed (trs_tuner_install), an automatic call whose (field, op,
+// k, nout, length bucket) has no cached choice ranks its candidates with the
+// GPU MLP, times the top few on the call's own buffers (each run writes the
+// same bytes) and caches the fastest.
+enum Variant : int {
+  kVarTable = 0,
+  kVarPlanes4 = 1,
+  kVarPlanes8 = 2,
+  kVarMfma = 3,
+  kVarProgram16K = 4,
+  kVarProgram32K = 5,
+  kVarProgram64K = 6,
+  kVariants = 7
+};
+constexpr size_t kTuneFeatures = 12;
+
+uint64_t variant_lds(int v) {
+  return v == kVarProgram16K ? 16384 : v == kVarProgram32K ? 32768 : v == kVarProgram64K ? 65536 : 0;
+}
+bool is_program(int v) { return v >= kVarProgram16K && v <= kVarProgram64K; }
+
+// The model's inputs: shape and length (log2), which kernel, its tile
+// parameter (log2 rows per workgroup, or LDS KiB) and the data fraction.
+void tune_features(int field, size_t k, size_t nout, size_t len, int v, float *f) {
+  const double L = double(std::max<size_t>(len, 1)), K = double(k), N = double(nout);
+  double tile = 0;
+  if (v == kVarTable) tile = field == 16 ? (nout <= 4 ? 2 : 3) : (nout <= 4 ? 2 : nout <= 8 ? 3 : 4);
+  if (v == kVarPlanes4) tile = 2;
+  if (v == kVarPlanes8 || v == kVarMfma) tile = 3;
+  if (is_program(v)) tile = std::log2(double(variant_lds(v) >> 10));
+  f[0] = field == 16 ? 1.f : 0.f;
+  f[1] = float(std::log2(K));
+  f[2] = float(std::log2(N));
+  f[3] = float(std::log2(L));
+  f[4] = float(std::log2(K * N));
+  f[5] = float(std::log2((K + N) * L));
+  f[6] = v == kVarTable ? 1.f : 0.f;
+  f[7] = (v == kVarPlanes4 || v == kVarPlanes8) ? 1.f : 0.f;
+  f[8] = v == kVarMfma ? 1.f : 0.f;
+  f[9] = is_program(v) ? 1.f : 0.f;
+  f[10] = float(tile);
+  f[11] = float(K / (K + N));
+}
+
+// Rows per planes tile the launch rule picks (launch_gf8_planes with nt = 0).
+int planes_rule_variant(size_t len, size_t nout) {
+  const uint64_t gx = (len + kPlanesChunkBytes - 1) / kPlanesChunkBytes;
+  const uint64_t wg8 = gx * ((nout + kPlanesOut - 1) / kPlanesOut);
+  return nout <= 4 || wg8 <= 640 ? kVarPlanes4 : kVarPlanes8;
+}
+
+// Feasible runtime-coefficient variants of a GF(2^8) apply.
+uint32_t matrix_variants8(size_t k, size_t nout) {
+  uint32_t mask = (1u << kVarTable) | (1u << kVarPlanes4);
+  if (nout > 4) mask |= 1u << kVarPlanes8;  // at most 4 rows: the same work as 4-row tiles
+  if (mfma_available(uint32_t(k))) mask |= 1u << kVarMfma;
+  return mask;
+}
+
+// The distinct LDS-target variants of an interpreter program over `cols`.
+uint32_t program_variants(const Fft16Device &d, uint64_t cols, bool byte_planes) {
+  uint32_t mask = 0;
+  uint32_t seen[3] = {~0u, ~0u, ~0u};
+  for (int v = kVarProgram16K, i = 0; v <= kVarProgram64K; ++v, ++i) {
+    const uint32_t gs = fft16_launch_gshift(d, cols, byte_planes, variant_lds(v));
+    bool dup = false;
+    for (int j = 0; j < i; ++j) dup = dup || seen[j] == gs;
+    seen[i] = gs;
+    if (!dup) mask |= 1u << v;
+  }
+  return mask;
+}
+
+struct RouteKey {
+  int field, op;  // op: 0 encode, 1 reconstruct apply
+  size_t k, nout;
+  int bucket;     // round(2 log2 len): half-octave length buckets
+  bool operator<(const RouteKey &o) const {
+    return std::tie(field, op, k, nout, bucket) < std::tie(o.field, o.op, o.k, o.nout, o.bucket);
+  }
+};
+
+int length_bucket(size_t len) {
+  return int(std::lround(2.0 * std::log2(double(std::max<size_t>(len, 1)))));
+}
+
+struct TunerModel {
+  trs_mlp *mlp = nullptr;
+  float mu[kTuneFeatures], sd[kTuneFeatures];
+  int top_n = 3;
+  ~TunerModel() {
+    if (mlp) trs_mlp_free(mlp);
+  }
+};
+
+struct Router {
+  std::shared_mutex mu;
+  std::shared_ptr<TunerModel> model;
+  std::map<RouteKey, int> cache;
+};
+
+Router &router() {
+  static Router *r = new Router();  // kept for the life of the process
+  return *r;
+}
+
+// Runs one automatic apply: run(v) launches variant v on the call's stream.
+// Without a model: the rule's variant.  With one: the cached choice for the
+// key, or -- on a miss -- the model's top candidates timed on this call (a
+// warm-up run, then a timed run each), the fastest cached.  Every run writes
+// the same bytes, so the call's outputs are complete whichever ran last.
+template <typename Feasible, typename Run>
+int router_run(const RouteKey &key, size_t len, int rule, hipStream_t s, Feasible &&feasible_fn,
+               Run &&run) {
+  Router &R = router();
+  std::shared_ptr<TunerModel> model;
+  {
+    std::shared_lock<std::shared_mutex> lk(R.mu);
+    model = R.model;
+    if (!model) return run(rule);
+    auto it = R.cache.find(key);
+    if (it != R.cache.end()) return run(it->second);
+  }
+  const uint32_t feasible = feasible_fn() | (1u << rule);
+  std::vector<int> cand;
+  for (int v = 0; v < kVariants; ++v)
+    if ((feasible >> v) & 1u) cand.push_back(v);
+  if (cand.size() > 1) {
+    std::vector<float> X(cand.size() * kTuneFeatures), Y(cand.size());
+    for (size_t c = 0; c < cand.size(); ++c) {
+      float *f = X.data() + c * kTuneFeatures;
+      tune_features(key.field, key.k, key.nout, len, cand[c], f);
+      for (size_t i = 0; i < kTuneFeatures; ++i) f[i] = (f[i] - model->mu[i]) / model->sd[i];
+    }
+    if (trs_mlp_predict(model->mlp, X.data(), cand.size(), Y.data()) != TRS_OK) return run(rule);
+    std::vector<size_t> order(cand.size());
+    std::iota(order.begin(), order.end(), size_t(0));
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return Y[a] < Y[b]; });
+    std::vector<int> ranked;
+    for (size_t i : order) ranked.push_back(cand[i]);
+    cand.swap(ranked);
+    if (cand.size() > size_t(model->top_n)) cand.resize(size_t(model->top_n));
+  }
+  thread_local hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!e0) TRS_TRY(hipEventCreate(&e0));
+  if (!e1) TRS_TRY(hipEventCreate(&e1));
+  int best = -1;
+  float best_ms = 0;
+  for (int v : cand) {
+    if (run(v) != TRS_OK) continue;  // warm-up (and first write of the outputs)
+    TRS_TRY(hipEventRecord(e0, s));
+    if (run(v) != TRS_OK) continue;
+    TRS_TRY(hipEventRecord(e1, s));
+    TRS_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    TRS_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if (best < 0 || ms < best_ms) {
+      best = v;
+      best_ms = ms;
+    }
+  }
+  if (best < 0) return run(rule);
+  std::unique_lock<std::shared_mutex> lk(R.mu);
+  if (R.model == model) R.cache[key] = best;
   return TRS_OK;
 }
 
@@ -1168,12 +1356,57 @@ int encode8_ptrs(const trs_rs8 *rs, const std::vector<const uint8_t *> &ptrs, si
   return hip_status(launch_bs_encode(route, c, s));
 }
 
+// The kernel variant the codec's rules take for an encode of this length:
+// the generated program, or the staged program at its default LDS target
+// (32 KiB of byte planes), or the matrix kernels by length.
+int enc8_rule_variant(const trs_rs8 *rs, size_t len) {
+  if (auto_route8(rs, len) != kRouteTable) return kVarProgram32K;
+  if (use_mfma(rs->apply_route.load(), rs->k, len, false) && rs->parity_frag.p) return kVarMfma;
+  return len >= planes_min_len() ? planes_rule_variant(len, rs->m) : kVarTable;
+}
+
+// Every kernel an encode of this codec can run at this length.
+uint32_t enc8_variants(const trs_rs8 *rs, size_t len) {
+  uint32_t mask = matrix_variants8(rs->k, rs->m);
+  if (!rs->parity_frag.p) mask &= ~(1u << kVarMfma);
+  if (rs->route != kRouteTable) return mask | (1u << kVarProgram32K);
+  const Fft16Device *d = nullptr;
+  if (staged8_device(rs, &d) == TRS_OK &&
+      fft16_device_loads(*d) + fft16_device_stores(*d) == rs->k + rs->m)
+    mask |= program_variants(*d, len, true);
+  return mask;
+}
+
+int run_enc8_variant(const trs_rs8 *rs, int v, const std::vector<const uint8_t *> &ptrs,
+                     size_t len, hipStream_t s) {
+  if (v == kVarMfma) {
+    if (!rs->parity_frag.p) return TRS_ERR_UNSUPPORTED;
+    return run_mfma(ptrs, rs->k, rs->m, rs->parity_frag.p, len, s);
+  }
+  if (!is_program(v)) return run_gf8(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, s, v);
+  if (rs->route != kRouteTable) return encode8_ptrs(rs, ptrs, len, s, rs->route);
+  const Fft16Device *d = nullptr;
+  const int st = staged8_device(rs, &d);
+  if (st) return st;
+  if (fft16_device_loads(*d) + fft16_device_stores(*d) != ptrs.size()) return TRS_ERR_UNSUPPORTED;
+  return launch_fft16_one(*d, ptrs, fft16_device_stores(*d), len, s, true, nullptr,
+                          variant_lds(v));
+}
+
 // The automatic encode of a validated shard list (k data then m parity).
 int encode8_auto(const trs_rs8 *rs, const std::vector<const uint8_t *> &ptrs, size_t len,
                  const Exec &ex) {
   return dispatch(ex, ptrs, rs->k, len,
                   [rs](const std::vector<const uint8_t *> &p, size_t l, hipStream_t s) {
-                    return encode8_ptrs(rs, p, l, s, auto_route8(rs, l));
+                    for (auto q : p)
+                      if (!q) return int(TRS_ERR_INVALID_ARGUMENT);
+                    // a forced apply route (trs_rs8_set_apply_route) is not tuned
+                    if (rs->apply_route.load() != 0)
+                      return run_enc8_variant(rs, enc8_rule_variant(rs, l), p, l, s);
+                    return router_run(
+                        RouteKey{8, 0, rs->k, rs->m, length_bucket(l)}, l, enc8_rule_variant(rs, l),
+                        s, [&] { return enc8_variants(rs, l); },
+                        [&](int v) { return run_enc8_variant(rs, v, p, l, s); });
                   });
 }
 
@@ -1295,9 +1528,26 @@ int rec8_launch(const Plan &plan, size_t k, size_t total, size_t nout,
     if (st) return st;
     return launch_fft16_one(*d, ptrs, nout, len, s, true);
   }
-  if (use_mfma(apply_route, k, len, false) && plan.frag.p)
-    return run_mfma(ptrs, k, nout, plan.frag.p, len, s);
-  return run_gf8(ptrs, k, nout, plan.coef, len, kWrite, nullptr, s);
+  // the decode plan's matrix on the runtime-coefficient kernels
+  const bool mfma = use_mfma(apply_route, k, len, false) && plan.frag.p;
+  const int rule = mfma ? kVarMfma : len >= planes_min_len() ? planes_rule_variant(len, nout)
+                                                             : kVarTable;
+  auto run = [&](int v) {
+    if (v == kVarMfma) {
+      if (!plan.frag.p) return int(TRS_ERR_UNSUPPORTED);
+      return run_mfma(ptrs, k, nout, plan.frag.p, len, s);
+    }
+    return run_gf8(ptrs, k, nout, plan.coef, len, kWrite, nullptr, s, v);
+  };
+  // a forced apply route (trs_rs8_set_apply_route) is not tuned
+  if (apply_route != 0) return run(rule);
+  return router_run(RouteKey{8, 1, k, nout, length_bucket(len)}, len, rule, s,
+                    [&] {
+                      uint32_t mask = matrix_variants8(k, nout);
+                      if (!plan.frag.p) mask &= ~(1u << kVarMfma);
+                      return mask;
+                    },
+                    run);
 }
 
 // reconstruct_internal / PreparedDecoder::reconstruct_internal
@@ -1564,10 +1814,26 @@ int run_fft16_program(const Fft16Device &d, const std::vector<const uint8_t *> &
   return launch_fft16_one(d, ptrs, nstore, len / 2, stream);
 }
 
+// RS16 encode through the rules (the FFT program at its default 64 KiB LDS
+// target whenever the shape has one) or, with a model installed, the router
+// over the matrix kernel and the program's LDS targets.
 int encode16_ptrs(const trs_rs16 *rs, const std::vector<const uint8_t *> &ptrs, size_t len,
                   hipStream_t stream) {
-  if (rs->staged) return run_fft16(rs, ptrs, 1, len, stream);
-  return run_gf16(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, stream);
+  const Fft16Device *d = nullptr;
+  if (rs->staged) {
+    const int st = fft16_device_for(rs, &d);
+    if (st) return st;
+  }
+  auto run = [&](int v) {
+    if (v == kVarTable) return run_gf16(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, stream);
+    if (!d) return int(TRS_ERR_UNSUPPORTED);
+    return launch_fft16_one(*d, ptrs, fft16_device_stores(*d), len / 2, stream, false, nullptr,
+                            variant_lds(v));
+  };
+  return router_run(RouteKey{16, 0, rs->k, rs->m, length_bucket(len)}, len,
+                    d ? kVarProgram64K : kVarTable, stream,
+                    [&] { return (1u << kVarTable) | (d ? program_variants(*d, len / 2, false) : 0u); },
+                    run);
 }
 
 // note_shard_len over a shard list (src/reedsolomon16.rs:758-772, :704-711).
@@ -2537,47 +2803,128 @@ uint64_t tune_rng(uint64_t &s) {
 
 }  // namespace
 
-extern "C" int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len, int route,
-                              uint64_t strip, int reps, double *ms_out) {
-  if (!ms_out || k == 0 || nout == 0 || len == 0 || reps <= 0) return TRS_ERR_INVALID_ARGUMENT;
-  if (field != 8 && field != 16) return TRS_ERR_INVALID_ARGUMENT;
-  if (field == 16 && ((len & 1) || (strip & 1))) return TRS_ERR_INVALID_ARGUMENT;
-  if (field == 8 && k + nout > 256) return TRS_ERR_INVALID_ARGUMENT;
-  int bs_route = 0;
-  // FFT candidate on the interpreter: the tower program, or a GF(2^8) shape's
-  // staged program (src/fft.rs:393-592) over byte planes when it has no
-  // bit-sliced program
-  bool interp = field == 16 && route == 1;
-  if (field == 8 && route == 1) {
-    bs_route = bs_encode_available(kRouteBsFft, k, nout)      ? kRouteBsFft
-               : bs_encode_available(kRouteBsMatrix, k, nout) ? kRouteBsMatrix
-                                                                : 0;
-    interp = bs_route == 0;
-  } else if (field == 8 && route == 2) {
-    if (!mfma_available(k)) return TRS_ERR_UNSUPPORTED;
-  } else if (field == 16 && route == 1) {
-    if (k < 2 || k > kMaxFftDataShards || k + nout > 4096) return TRS_ERR_UNSUPPORTED;
-  } else if (route != 0) {
-    return TRS_ERR_UNSUPPORTED;
-  }
-  TuneSpace &T = tune_space();
-  std::lock_guard<std::mutex> lk(T.mu);
-  const Fft16Device *prog16 = nullptr;
-  if (interp) {
-    const uint64_t key = (uint64_t(field) << 48) | (uint64_t(k) << 24) | nout;
-    for (auto &e : T.fft16)
-      if (e.first == key) prog16 = e.second.get();
-    if (!prog16) {
-      Fft16DevicePtr d;
-      const hipError_t e = fft16_device_new(
-          field == 16 ? build_staged_program16(k, nout) : build_staged_program8(k, nout), &d);
-      if (e == hipErrorInvalidValue) return TRS_ERR_UNSUPPORTED;
-      TRS_TRY(e);
-      if (T.fft16.size() >= 64) T.fft16.clear();
-      prog16 = d.get();
-      T.fft16.emplace_back(key, std::move(d));
+namespace {
+// A tuning shape's interpreter program (the tower's, or a GF(2^8) shape's
+// staged program when it has no generated one), built once per shape.
+int tune_program(TuneSpace &T, int field, uint32_t k, uint32_t nout, const Fft16Device **out) {
+  *out = nullptr;
+  if (field == 16 && (k < 2 || k > kMaxFftDataShards || k + nout > 4096)) return TRS_ERR_UNSUPPORTED;
+  if (field == 8 && k + nout > 256) return TRS_ERR_UNSUPPORTED;
+  const uint64_t key = (uint64_t(field) << 48) | (uint64_t(k) << 24) | nout;
+  for (auto &e : T.fft16)
+    if (e.first == key) {
+      *out = e.second.get();
+      return *out ? TRS_OK : TRS_ERR_UNSUPPORTED;
+    }
+  Fft16DevicePtr d;
+  const hipError_t e = fft16_device_new(
+      field == 16 ? build_staged_program16(k, nout) : build_staged_program8(k, nout), &d);
+  if (e != hipSuccess && e != hipErrorInvalidValue) return hip_status(e);
+  if (T.fft16.size() >= 64) T.fft16.clear();
+  *out = d.get();
+  T.fft16.emplace_back(key, std::move(d));  // an entry without a device program: does not fit
+  return *out ? TRS_OK : TRS_ERR_UNSUPPORTED;
+}
+
+// The encode variants of shape (k, nout) at this length (the codec's own set).
+int tune_feasible(TuneSpace &T, int field, uint32_t k, uint32_t nout, uint64_t len,
+                  uint32_t *mask) {
+  if (field == 16) {
+    *mask = 1u << kVarTable;
+  } else {
+    *mask = matrix_variants8(k, nout);
+    if (bs_encode_available(kRouteBsFft, k, nout) || bs_encode_available(kRouteBsMatrix, k, nout)) {
+      *mask |= 1u << kVarProgram32K;
+      return TRS_OK;
     }
   }
+  const Fft16Device *d = nullptr;
+  const int st = tune_program(T, field, k, nout, &d);
+  if (st == TRS_ERR_UNSUPPORTED) return TRS_OK;
+  if (st) return st;
+  *mask |= program_variants(*d, field == 16 ? len / 2 : len, field == 8);
+  return TRS_OK;
+}
+}  // namespace
+
+extern "C" int trs_tune_features(int field, uint32_t k, uint32_t nout, uint64_t len, int variant,
+                                 float *out, size_t n) {
+  if (!out || n != kTuneFeatures || (field != 8 && field != 16) || k == 0 || nout == 0 ||
+      variant < 0 || variant >= kVariants)
+    return TRS_ERR_INVALID_ARGUMENT;
+  tune_features(field, k, nout, len, variant, out);
+  return TRS_OK;
+}
+
+extern "C" int trs_tune_feasible(int field, uint32_t k, uint32_t nout, uint64_t len,
+                                 uint32_t *mask) {
+  if (!mask || (field != 8 && field != 16) || k == 0 || nout == 0 || len == 0)
+    return TRS_ERR_INVALID_ARGUMENT;
+  if (field == 16 && (len & 1)) return TRS_ERR_INVALID_ARGUMENT;
+  if (field == 8 && k + nout > 256) return TRS_ERR_INVALID_ARGUMENT;
+  TuneSpace &T = tune_space();
+  std::lock_guard<std::mutex> lk(T.mu);
+  return tune_feasible(T, field, k, nout, len, mask);
+}
+
+extern "C" int trs_tune_rule(int field, uint32_t k, uint32_t nout, uint64_t len, int *variant) {
+  if (!variant || (field != 8 && field != 16) || k == 0 || nout == 0 || len == 0)
+    return TRS_ERR_INVALID_ARGUMENT;
+  if (field == 16 && (len & 1)) return TRS_ERR_INVALID_ARGUMENT;
+  if (field == 8 && k + nout > 256) return TRS_ERR_INVALID_ARGUMENT;
+  TuneSpace &T = tune_space();
+  std::lock_guard<std::mutex> lk(T.mu);
+  uint32_t mask = 0;
+  int st = tune_feasible(T, field, k, nout, len, &mask);
+  if (st) return st;
+  int v;
+  if (field == 16) {  // the FFT program whenever it fits, at its 64 KiB default
+    v = (mask >> kVarTable) == 1u ? kVarTable : kVarProgram64K;
+  } else if (bs_encode_available(kRouteBsFft, k, nout) ||
+             bs_encode_available(kRouteBsMatrix, k, nout) || len <= staged8_gpu_max(k, nout)) {
+    v = kVarProgram32K;
+  } else if (use_mfma(0, k, len, false)) {
+    v = kVarMfma;
+  } else {
+    v = len >= planes_min_len() ? planes_rule_variant(len, nout) : kVarTable;
+  }
+  // a default LDS target whose launch equals a smaller target's is listed as that one
+  if (is_program(v) && !((mask >> v) & 1u)) {
+    const Fft16Device *d = nullptr;
+    if ((st = tune_program(T, field, k, nout, &d))) return st;
+    const uint64_t cols = field == 16 ? len / 2 : len;
+    const uint32_t gs = fft16_launch_gshift(*d, cols, field == 8, variant_lds(v));
+    for (int w = kVarProgram16K; w <= kVarProgram64K; ++w)
+      if (((mask >> w) & 1u) && fft16_launch_gshift(*d, cols, field == 8, variant_lds(w)) == gs) {
+        v = w;
+        break;
+      }
+  }
+  *variant = v;
+  return TRS_OK;
+}
+
+extern "C" int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len, int variant,
+                              int reps, double limit_ms, double *ms_out) {
+  if (!ms_out || k == 0 || nout == 0 || len == 0 || reps <= 0) return TRS_ERR_INVALID_ARGUMENT;
+  if (field != 8 && field != 16) return TRS_ERR_INVALID_ARGUMENT;
+  if (field == 16 && (len & 1)) return TRS_ERR_INVALID_ARGUMENT;
+  if (field == 8 && k + nout > 256) return TRS_ERR_INVALID_ARGUMENT;
+  if (variant < 0 || variant >= kVariants) return TRS_ERR_INVALID_ARGUMENT;
+  TuneSpace &T = tune_space();
+  std::lock_guard<std::mutex> lk(T.mu);
+  uint32_t mask = 0;
+  int st = tune_feasible(T, field, k, nout, len, &mask);
+  if (st) return st;
+  if (!((mask >> variant) & 1u)) return TRS_ERR_UNSUPPORTED;
+  const bool prog = is_program(variant);
+  const int bs_route = field == 8 && prog
+                           ? (bs_encode_available(kRouteBsFft, k, nout)      ? kRouteBsFft
+                              : bs_encode_available(kRouteBsMatrix, k, nout) ? kRouteBsMatrix
+                                                                               : 0)
+                           : 0;
+  const Fft16Device *prog16 = nullptr;
+  if (prog && !bs_route && (st = tune_program(T, field, k, nout, &prog16))) return st;
   if (!T.stream) {
     TRS_TRY(hipStreamCreateWithFlags(&T.stream, hipStreamNonBlocking));
     TRS_TRY(hipEventCreate(&T.ev0));
@@ -2590,97 +2937,100 @@ extern "C" int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len
   }
   // random nonzero coefficients (kernel time does not depend on their values)
   uint64_t seed = (uint64_t(field) << 40) ^ (uint64_t(k) << 20) ^ nout;
-  int st = TRS_OK;
-  std::vector<uint8_t> rows8;
   if (field == 8) {
-    rows8.resize(size_t(k) * nout);
+    std::vector<uint8_t> rows8(size_t(k) * nout);
     for (auto &c : rows8) c = uint8_t(1 + tune_rng(seed) % 255);
-    if (route == 0) st = upload_coef8(T.coef, rows8.data(), nout, k);
-    if (!st && route == 2) st = upload_frags(T.frag, rows8.data(), nout, k);
-  } else {
+    if (!prog && variant != kVarMfma) st = upload_coef8(T.coef, rows8.data(), nout, k);
+    if (!st && variant == kVarMfma) st = upload_frags(T.frag, rows8.data(), nout, k);
+  } else if (!prog) {
     std::vector<uint16_t> rows16(size_t(k) * nout);
     for (auto &c : rows16) c = uint16_t(1 + tune_rng(seed) % 65535);
     st = upload_coef16(T.coef, rows16.data(), nout, k);
   }
   if (st) return st;
   uint8_t *base = static_cast<uint8_t *>(T.data.p);
-  const uint64_t sw = (strip == 0 || strip >= len) ? len : strip;
-  // tower kernels address split planes of len / 2 columns; GF(2^8) kernels
-  // (the byte-plane interpreter included) one column per byte
-  const bool halves = field == 16;
-  const uint64_t ncols = halves ? len / 2 : len, scols = halves ? sw / 2 : sw;
-  PtrTable table16;
-  if (halves || interp) {
-    // tower matrix: one table; interpreter: one table per column strip
-    // (pointers at the strip's column offset in the low plane)
-    std::vector<const uint8_t *> p16;
-    const uint64_t step = interp ? scols : ncols;
-    for (uint64_t off = 0; off < ncols; off += step)
-      for (uint32_t i = 0; i < k + nout; ++i) p16.push_back(base + uint64_t(i) * len + off);
-    if ((st = table16.upload(p16, T.stream))) return st;
-  }
+  std::vector<const uint8_t *> ptrs;
+  for (uint32_t i = 0; i < k + nout; ++i) ptrs.push_back(base + uint64_t(i) * len);
   auto run_once = [&]() -> int {
-    for (uint64_t off = 0; off < ncols; off += scols) {
-      const uint64_t span = std::min<uint64_t>(scols, ncols - off);
-      hipError_t e = hipSuccess;
-      if (interp) {
-        Fft16Call c{};
-        c.ptrs = static_cast<const uint8_t *const *>(table16.d) + (off / scols) * (k + nout);
-        c.plane = ncols;
-        c.cols = span;
-        c.stripes = 1;
-        c.nstore = nout;
-        c.vec_ok = ((!halves || ncols % 16 == 0) && len % 16 == 0 && off % 16 == 0) ? 1u : 0u;
-        e = launch_fft16(*prog16, c, T.stream, nullptr, !halves);
-      } else if (field == 16) {
-        Gf16Call c{};
-        c.ptrs = static_cast<const uint8_t *const *>(table16.d);
-        c.coef = T.coef.ptr();
-        c.plane = len / 2;
-        c.col0 = off;
-        c.cols = span;
-        c.k = k;
-        c.nout = nout;
-        c.ostride = T.coef.ostride;
-        c.vec_ok = ((len / 2) % 16 == 0 && off % 16 == 0) ? 1u : 0u;
-        c.mode = kWrite;
-        e = launch_gf16_apply(c, T.stream);
-      } else if (route == 2) {
-        Gf8MfmaCall c{};
-        for (uint32_t i = 0; i < k + nout; ++i) c.ptr[i] = base + uint64_t(i) * len + off;
-        c.frag = T.frag.p;
-        c.len = span;
-        c.k = k;
-        c.nout = nout;
-        c.vec_ok = (len % 4 == 0 && off % 4 == 0) ? 1u : 0u;
-        e = launch_gf8_mfma(c, T.stream);
-      } else {
-        Gf8Call c{};
-        for (uint32_t i = 0; i < k + nout; ++i) c.ptr[i] = base + uint64_t(i) * len + off;
-        c.coef = T.coef.ptr();
-        c.len = span;
-        c.k = k;
-        c.nout = nout;
-        c.ostride = T.coef.ostride;
-        c.vec_ok = (len % 16 == 0 && off % 16 == 0) ? 1u : 0u;
-        // the matrix candidate dispatches by length as the codec's calls do (run_gf8)
-        e = route == 1                     ? launch_bs_encode(bs_route, c, T.stream)
-            : span >= planes_min_len() ? launch_gf8_planes(c, kWrite, T.stream)
-                                       : launch_gf8_apply(c, kWrite, T.stream);
-      }
-      if (e != hipSuccess) return hip_status(e);
+    if (prog16) {
+      return launch_fft16_one(*prog16, ptrs, nout, field == 16 ? len / 2 : len, T.stream,
+                              field == 8, nullptr, variant_lds(variant));
     }
-    return TRS_OK;
+    if (field == 16) return run_gf16(ptrs, k, nout, T.coef, len, kWrite, nullptr, T.stream);
+    if (variant == kVarMfma) return run_mfma(ptrs, k, nout, T.frag.p, len, T.stream);
+    if (bs_route) {
+      Gf8Call c{};
+      for (uint32_t i = 0; i < k + nout; ++i) c.ptr[i] = ptrs[i];
+      c.len = len;
+      c.k = k;
+      c.nout = nout;
+      c.vec_ok = len % 16 == 0 ? 1u : 0u;
+      return hip_status(launch_bs_encode(bs_route, c, T.stream));
+    }
+    return run_gf8(ptrs, k, nout, T.coef, len, kWrite, nullptr, T.stream, variant);
   };
-  if ((st = run_once())) return st;  // warm-up
+  float ms = 0;
+  // Warm-up.  A first launch can carry one-time costs (loading the kernel's
+  // code), so a warm-up over limit_ms runs once more before that time stands
+  // as the measurement of a candidate already far behind.
+  for (int w = 0; w < 2; ++w) {
+    TRS_TRY(hipEventRecord(T.ev0, T.stream));
+    if ((st = run_once())) return st;
+    TRS_TRY(hipEventRecord(T.ev1, T.stream));
+    TRS_TRY(hipEventSynchronize(T.ev1));
+    TRS_TRY(hipEventElapsedTime(&ms, T.ev0, T.ev1));
+    if (!(limit_ms > 0 && ms > limit_ms)) break;
+    if (w == 1) {
+      *ms_out = double(ms);
+      return TRS_OK;
+    }
+  }
   TRS_TRY(hipEventRecord(T.ev0, T.stream));
   for (int r = 0; r < reps; ++r)
     if ((st = run_once())) return st;
   TRS_TRY(hipEventRecord(T.ev1, T.stream));
   TRS_TRY(hipEventSynchronize(T.ev1));
-  float ms = 0;
   TRS_TRY(hipEventElapsedTime(&ms, T.ev0, T.ev1));
   *ms_out = double(ms) / reps;
+  return TRS_OK;
+}
+
+extern "C" int trs_tuner_install(const float *params, size_t n_params, const float *norm,
+                                 size_t n_norm, int top_n) {
+  if (!params || !norm || n_params != size_t(trs_mlp_param_count()) ||
+      n_norm != 2 * kTuneFeatures || top_n < 1)
+    return TRS_ERR_INVALID_ARGUMENT;
+  auto m = std::make_shared<TunerModel>();
+  int st = trs_mlp_new(0, &m->mlp);
+  if (st) return st;
+  if ((st = trs_mlp_set_params(m->mlp, params, n_params))) return st;
+  for (size_t i = 0; i < kTuneFeatures; ++i) {
+    m->mu[i] = norm[i];
+    m->sd[i] = norm[kTuneFeatures + i] > 0 ? norm[kTuneFeatures + i] : 1.f;
+  }
+  m->top_n = top_n;
+  Router &R = router();
+  std::unique_lock<std::shared_mutex> lk(R.mu);
+  R.model = std::move(m);
+  R.cache.clear();
+  return TRS_OK;
+}
+
+extern "C" int trs_tuner_remove(void) {
+  Router &R = router();
+  std::unique_lock<std::shared_mutex> lk(R.mu);
+  R.model.reset();
+  R.cache.clear();
+  return TRS_OK;
+}
+
+extern "C" int trs_tuner_lookup(int field, int op, size_t k, size_t nout, size_t len,
+                                int *variant) {
+  if (!variant) return TRS_ERR_INVALID_ARGUMENT;
+  Router &R = router();
+  std::shared_lock<std::shared_mutex> lk(R.mu);
+  auto it = R.cache.find(RouteKey{field, op, k, nout, length_bucket(len)});
+  *variant = it == R.cache.end() ? -1 : it->second;
   return TRS_OK;
 }
 

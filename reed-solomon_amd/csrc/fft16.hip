@@ -931,6 +931,15 @@ void launch_planes(uint32_t threads, const Fft16Args &a, const Fft16Ptrs *inline
 }
 }  // namespace
 
+uint32_t fft16_launch_gshift(const Fft16Device &d, uint64_t cols, bool byte_planes,
+                             uint64_t lds_target) {
+  const uint64_t groups = (cols + kGroupCols - 1) / kGroupCols;
+  const uint32_t reg_bytes = byte_planes ? kRegBytes / 2 : kRegBytes;
+  const size_t target =
+      lds_target ? size_t(lds_target) : byte_planes ? kLdsTarget / 2 : kLdsTarget;
+  return choose_gshift(d, groups, reg_bytes, std::min(target, kLdsMax));
+}
+
 hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t stream,
                         const Fft16Ptrs *inline_ptrs, bool byte_planes) {
   if (call.plane == 0 || call.stripes == 0) return hipSuccess;
@@ -939,7 +948,9 @@ hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t
   const uint64_t cols = call.cols ? call.cols : call.plane;
   const uint64_t groups = (cols + kGroupCols - 1) / kGroupCols;
   const uint32_t reg_bytes = byte_planes ? kRegBytes / 2 : kRegBytes;
-  const uint32_t gs = choose_gshift(d, groups, reg_bytes, byte_planes ? kLdsTarget / 2 : kLdsTarget);
+  const size_t target =
+      call.lds_target ? size_t(call.lds_target) : byte_planes ? kLdsTarget / 2 : kLdsTarget;
+  const uint32_t gs = choose_gshift(d, groups, reg_bytes, std::min(target, kLdsMax));
   const uint64_t cblocks = (groups + (1u << gs) - 1) >> gs;
   const uint64_t total = cblocks * call.stripes;
   if (groups > 0xffffffffu || total > 0x7fffffffu) return hipErrorInvalidValue;

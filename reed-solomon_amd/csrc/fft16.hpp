@@ -106,6 +106,8 @@ struct Fft16Call {
                                // column offset within their low plane
   uint32_t *flag;              // non-null: verify -- compare the stored registers with the
                                // shards instead of writing them, *flag = 1 on a mismatch
+  uint64_t lds_target;         // LDS bytes per workgroup the column groups are sized for
+                               // (0: 64 KiB for tower registers, 32 KiB for byte planes)
 };
 
 // Lowers p for the device (glue scheduled into independent sub-steps, whole
@@ -168,6 +170,11 @@ struct Fft16Ptrs {
 // after the low).
 hipError_t launch_fft16(const Fft16Device &d, const Fft16Call &call, hipStream_t stream,
                         const Fft16Ptrs *inline_ptrs = nullptr, bool byte_planes = false);
+// Column groups per workgroup (log2) a launch of d over `cols` columns takes
+// with LDS target `lds_target` (0: the default); launches with equal values
+// are the same launch.
+uint32_t fft16_launch_gshift(const Fft16Device &d, uint64_t cols, bool byte_planes,
+                             uint64_t lds_target);
 
 // One stripe of one program; launch_fft16_jobs runs many in one launch.
 struct Fft16JobSpec {

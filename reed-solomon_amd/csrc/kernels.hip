@@ -553,7 +553,7 @@ hipError_t launch_gf16_apply(const Gf16Call &call, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_gf8_planes(const Gf8Call &call, uint32_t mode, hipStream_t stream) {
+hipError_t launch_gf8_planes(const Gf8Call &call, uint32_t mode, hipStream_t stream, int nt) {
   if (call.len == 0 || call.nout == 0) return hipSuccess;
   const uint32_t gx = uint32_t((call.len + kPlanesChunkBytes - 1) / kPlanesChunkBytes);
   // 8-row tiles, unless that grid leaves the chip short of work (at most 640
@@ -561,7 +561,7 @@ hipError_t launch_gf8_planes(const Gf8Call &call, uint32_t mode, hipStream_t str
   // measured 1.2-1.6x faster there (k = 33..200 at 256 KiB - 1 MiB) and
   // 1.2-1.5x slower on full grids.
   const uint64_t wg8 = uint64_t(gx) * ((call.nout + kPlanesOut - 1) / kPlanesOut);
-  const int nt = call.nout <= 4 || wg8 <= 640 ? 4 : kPlanesOut;
+  if (nt != 4 && nt != kPlanesOut) nt = call.nout <= 4 || wg8 <= 640 ? 4 : kPlanesOut;
   const dim3 grid(gx, (call.nout + nt - 1) / nt);
   const dim3 block(kBlockThreads);
 #define TRS_GF8P(NT, MODE) hipLaunchKernelGGL((k_gf8_planes<NT, MODE>), grid, block, 0, stream, call)

@@ -79,8 +79,9 @@ struct BatchJob {
 
 // Host-side launchers (kernels.hip).  All return hipError_t of the launch.
 hipError_t launch_gf8_apply(const Gf8Call &call, uint32_t mode, hipStream_t stream);
-// The same apply on the bit-sliced tile (8 KiB chunks; NT 4 or kPlanesOut).
-hipError_t launch_gf8_planes(const Gf8Call &call, uint32_t mode, hipStream_t stream);
+// The same apply on the bit-sliced tile (8 KiB chunks; rows per tile `nt`, 4
+// or kPlanesOut, 0 = the launch rule below).
+hipError_t launch_gf8_planes(const Gf8Call &call, uint32_t mode, hipStream_t stream, int nt = 0);
 hipError_t launch_gf16_apply(const Gf16Call &call, hipStream_t stream);
 // Batched persistent kernels: tile_job[t] is the job index of tile t.
 hipError_t launch_batch8(const BatchJob *jobs, const uint32_t *tile_job, uint64_t total_tiles,
