@@ -204,6 +204,27 @@ Mask make_mask(const uint8_t *present, size_t n) {
 
 bool mask_get(const Mask &m, size_t i) { return (m[i / 64] >> (i % 64)) & 1; }
 
+// Presence mask and count of n flags (nonzero = present), eight flags at a
+// time: each byte is folded to its bit 0 (the OR of its bits), and the
+// multiply gathers bit 0 of byte i into bit 56 + i.
+void flags_mask(const uint8_t *f, size_t n, Mask &mask, size_t &count) {
+  mask.assign((n + 63) / 64, 0);
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    std::memcpy(&w, f + i, 8);
+    w |= w >> 4;
+    w |= w >> 2;
+    w |= w >> 1;
+    w &= 0x0101010101010101ull;
+    mask[i / 64] |= ((w * 0x0102040810204080ull) >> 56) << (i % 64);
+  }
+  for (; i < n; ++i)
+    if (f[i]) mask[i / 64] |= uint64_t(1) << (i % 64);
+  count = 0;
+  for (uint64_t w : mask) count += size_t(__builtin_popcountll(w));
+}
+
 // Everything one erasure pattern needs (src/reedsolomon.rs:117-130).
 struct Plan {
   Mask presence;
@@ -221,6 +242,9 @@ struct Plan {
   std::once_flag fft_once;
   hipError_t fft_status = hipSuccess;
   Fft16DevicePtr fft;
+  // The decoder's pointer slots as shard indices (valid, missing data,
+  // missing parity), for stripes addressed as base + index * stride.
+  DevBuf fft_idx;
 };
 
 size_t log2_ceil(size_t n) {
@@ -287,6 +311,13 @@ int plan_fft16(Plan &p, size_t n, const Fft16Device **out, bool byte_field = fal
     p.fft_status = fft16_device_upload(byte_field ? fft8_decode_lowered(n, p.valid, outputs)
                                                   : fft16_decode_lowered(n, p.valid, outputs),
                                        &p.fft);
+    if (p.fft_status != hipSuccess) return;
+    std::vector<uint32_t> idx(p.valid);
+    idx.insert(idx.end(), outputs.begin(), outputs.end());
+    p.fft_status = p.fft_idx.alloc(idx.size() * sizeof(uint32_t));
+    if (p.fft_status == hipSuccess)
+      p.fft_status = hipMemcpy(p.fft_idx.p, idx.data(), idx.size() * sizeof(uint32_t),
+                               hipMemcpyHostToDevice);
   });
   if (p.fft_status == hipErrorInvalidValue) return TRS_ERR_UNSUPPORTED;
   if (p.fft_status != hipSuccess) return hip_status(p.fft_status);
@@ -411,9 +442,8 @@ int slice_coef(uint8_t c, const uint32_t **out) {
   return TRS_OK;
 }
 
-// Device copy of a pointer table, stream-ordered (allocated, filled and freed
-// on `stream`).  The staged copy from pageable memory has finished reading
-// the host vector when hipMemcpyAsync returns.
+// Device copy of a pointer table, stream-ordered (allocated, filled through
+// pinned staging and freed on `stream`).
 struct PtrTable {
   void *d = nullptr;
   hipStream_t s = nullptr;
@@ -424,7 +454,7 @@ struct PtrTable {
     s = stream;
     const size_t bytes = v.size() * sizeof(void *);
     TRS_TRY(hipMallocAsync(&d, bytes ? bytes : 8, stream));
-    if (bytes) TRS_TRY(hipMemcpyAsync(d, v.data(), bytes, hipMemcpyHostToDevice, stream));
+    if (bytes) TRS_TRY(staged_upload(d, v.data(), bytes, stream));
     return TRS_OK;
   }
 };
@@ -2118,31 +2148,55 @@ int trs_rs16_reconstruct_stripes(const trs_rs16 *rs, uint8_t *const *shards,
   const size_t n = rs->n, k = rs->k;
   const int route = rs->decode_route.load();
   const std::vector<size_t> lens(n, len);
+  // Stripes whose shards sit one fixed stride apart (the layout of a
+  // (stripes, n, len) tensor) are addressed on the FFT route as base + index *
+  // stride through the plan's device index list: no per-call pointer table.
+  const uintptr_t stride = n > 1 ? reinterpret_cast<uintptr_t>(shards[1]) -
+                                       reinterpret_cast<uintptr_t>(shards[0])
+                                 : 0;
+  bool regular = n > 1;
+  for (size_t i = 0; i < stripes && regular; ++i) {
+    const uintptr_t base = reinterpret_cast<uintptr_t>(shards[i * n]);
+    regular = base != 0;
+    for (size_t j = 1; j < n && regular; ++j)
+      regular = reinterpret_cast<uintptr_t>(shards[i * n + j]) == base + j * stride;
+  }
   struct Work {
     std::shared_ptr<Plan> plan;
     std::vector<const uint8_t *> ptrs;
     size_t nout = 0;
     const Fft16Device *fft = nullptr;
+    const uint8_t *base = nullptr;  // indexed addressing (ptrs empty)
   };
   // Validate and plan every stripe before launching anything.
   std::vector<Work> work;
   for (size_t i = 0; i < stripes; ++i) {
+    // every buffer has the call's (nonzero, even) length, so of scan_shards'
+    // checks only the presence count remains
     Mask mask;
-    size_t count = 0, l = 0;
-    int st = scan_shards(lens.data(), present + i * n, n, true, mask, count, l);
-    if (st) return st;
+    size_t count = 0;
+    flags_mask(present + i * n, n, mask, count);
     if (count == n) continue;
     if (count < k) return TRS_TOO_FEW_SHARDS_PRESENT;
     Work w;
-    if ((st = plan16_for(rs, mask, &w.plan))) return st;
-    if ((st = gather(*w.plan, shards + i * n, lens.data(), k, len, data_only != 0, w.ptrs, w.nout)))
-      return st;
-    if (w.nout == 0) continue;
-    if (use_fft_decode(route, n, k, w.nout)) {
+    int st = plan16_for(rs, mask, &w.plan);
+    if (st) return st;
+    const Plan &p = *w.plan;
+    const size_t nout = p.missing_data.size() + (data_only ? 0 : p.missing_parity.size());
+    if (nout == 0) continue;
+    if (use_fft_decode(route, n, k, nout)) {
       if ((st = plan_fft16(*w.plan, n, &w.fft))) return st;
+      if (regular) {  // every buffer has the call's length and a non-null address
+        w.nout = nout;
+        w.base = shards[i * n];
+        work.push_back(std::move(w));
+        continue;
+      }
     } else if ((st = plan16_matrix(*w.plan, k))) {
       return st;
     }
+    if ((st = gather(*w.plan, shards + i * n, lens.data(), k, len, data_only != 0, w.ptrs, w.nout)))
+      return st;
     work.push_back(std::move(w));
   }
   hipStream_t s = as_stream(stream);
@@ -2154,22 +2208,37 @@ int trs_rs16_reconstruct_stripes(const trs_rs16 *rs, uint8_t *const *shards,
       if (st) return st;
       continue;
     }
+    if (w.base) continue;
     offset.push_back(all.size());
     all.insert(all.end(), w.ptrs.begin(), w.ptrs.end());
   }
-  if (all.empty()) return TRS_OK;
   PtrTable table;
-  int st = table.upload(all, s);
-  if (st) return st;
+  if (!all.empty()) {
+    const int st = table.upload(all, s);
+    if (st) return st;
+  }
   std::vector<Fft16JobSpec> jobs;
   size_t j = 0;
   for (Work &w : work) {
     if (!w.fft) continue;
-    bool al = true;
-    for (auto q : w.ptrs) al = al && aligned16(q);
-    jobs.push_back({w.fft, static_cast<const uint8_t *const *>(table.d) + offset[j++], len / 2,
-                    uint32_t(w.nout), al ? 1u : 0u});
+    Fft16JobSpec spec{};
+    spec.prog = w.fft;
+    spec.plane = len / 2;
+    spec.nstore = uint32_t(w.nout);
+    if (w.base) {
+      spec.idx = static_cast<const uint32_t *>(w.plan->fft_idx.p);
+      spec.base = w.base;
+      spec.stride = stride;
+      spec.vec_ok = (aligned16(w.base) && stride % 16 == 0) ? 1u : 0u;
+    } else {
+      bool al = true;
+      for (auto q : w.ptrs) al = al && aligned16(q);
+      spec.ptrs = static_cast<const uint8_t *const *>(table.d) + offset[j++];
+      spec.vec_ok = al ? 1u : 0u;
+    }
+    jobs.push_back(spec);
   }
+  if (jobs.empty()) return TRS_OK;
   return hip_status(launch_fft16_jobs(jobs, s));
 }
 

@@ -606,7 +606,17 @@ struct Fft16Args {
   uint32_t total;    // workgroups (single-program launch); first workgroup (job table)
   uint32_t vec_ok;
   uint32_t *flag;    // verify: compare the stored registers with the shards, set on a mismatch
+  // Indexed addressing (ptrs unused when idx is set): pointer slot i is shard
+  // idx[i] of a stripe laid out at `base`, shards `stride` bytes apart.
+  const uint32_t *idx;
+  const uint8_t *base;
+  uint64_t stride;
 };
+
+__device__ __forceinline__ const uint8_t *slot_ptr(const Fft16Args &a, const uint8_t *const *ptr,
+                                                   uint32_t slot) {
+  return a.idx ? a.base + uint64_t(a.idx[slot]) * a.stride : ptr[slot];
+}
 
 // XCD-aware order: consecutive workgroups of one XCD take consecutive
 // column blocks, so the 32-byte segments of one 128-byte line meet in one L2.
@@ -662,7 +672,7 @@ __device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
       const uint64_t col = uint64_t(cg) * kGroupCols;
       const uint64_t rem = a.cols - col;
       const bool full = a.vec_ok && rem >= kGroupCols;
-      const uint8_t *s = ptr[slot];
+      const uint8_t *s = slot_ptr(a, ptr, slot);
       uint32_t lo[8];
       load_planes(s + col, full, rem, lo);
 #pragma unroll
@@ -773,7 +783,7 @@ __device__ __forceinline__ void run_program(const Fft16Args &a, uint32_t g0,
     const uint64_t col = uint64_t(cg) * kGroupCols;
     const uint64_t rem = a.cols - col;
     const bool full = a.vec_ok && rem >= kGroupCols;
-    uint8_t *d = const_cast<uint8_t *>(ptr[a.nload + j]);
+    uint8_t *d = const_cast<uint8_t *>(slot_ptr(a, ptr, a.nload + j));
     if (a.flag) {
       // Compare in plane form: both sides are the same transpose of their
       // bytes, and columns past the end are zero on both sides (the program
@@ -1019,6 +1029,9 @@ hipError_t fft16_jobs_plan(const std::vector<Fft16JobSpec> &jobs, bool byte_plan
     a.cblocks = uint32_t(cblocks);
     a.total = uint32_t(wg_job.size());
     a.vec_ok = (j.vec_ok && (byte_planes || j.plane % 16 == 0)) ? 1u : 0u;
+    a.idx = j.idx;
+    a.base = j.base;
+    a.stride = j.stride;
     wg_job.insert(wg_job.end(), cblocks, uint32_t(table.size()));
     table.push_back(a);
     plan->lds = std::max(plan->lds, (d.regs * reg_bytes) << gs);
@@ -1060,6 +1073,51 @@ hipError_t fft16_jobs_launch(const Fft16JobsPlan &plan, const void *dev_blob, hi
   return hipGetLastError();
 }
 
+namespace {
+// Per-thread pinned staging buffers, kept for the life of the process
+// (freeing pinned memory from a thread-exit destructor could run after the
+// HIP runtime has shut down).
+struct StagingRing {
+  static constexpr int N = 4;
+  void *buf[N] = {};
+  size_t cap[N] = {};
+  hipEvent_t done[N] = {};
+  bool used[N] = {};
+  int next = 0;
+};
+}  // namespace
+
+hipError_t staged_upload(void *dst, const void *src, size_t bytes, hipStream_t stream) {
+  if (bytes == 0) return hipSuccess;
+  static thread_local StagingRing *ring = new StagingRing();
+  StagingRing &R = *ring;
+  const int i = R.next;
+  R.next = (R.next + 1) % StagingRing::N;
+  hipError_t e;
+  if (!R.done[i] && (e = hipEventCreateWithFlags(&R.done[i], hipEventDisableTiming)) != hipSuccess)
+    return e;
+  if (R.used[i] && (e = hipEventSynchronize(R.done[i])) != hipSuccess) return e;
+  if (bytes > R.cap[i]) {
+    // Grow every idle slot at once (at least 1 MiB), so the pinned
+    // allocations happen on one call rather than one per slot.
+    const size_t want = std::max<size_t>(bytes, size_t(1) << 20);
+    for (int j = 0; j < StagingRing::N; ++j) {
+      if (R.cap[j] >= want || (j != i && R.used[j] && hipEventQuery(R.done[j]) != hipSuccess))
+        continue;
+      if (R.buf[j]) (void)hipHostFree(R.buf[j]);
+      R.buf[j] = nullptr;
+      R.cap[j] = 0;
+      if ((e = hipHostMalloc(&R.buf[j], want, hipHostMallocDefault)) != hipSuccess) return e;
+      R.cap[j] = want;
+    }
+  }
+  std::memcpy(R.buf[i], src, bytes);
+  if ((e = hipMemcpyAsync(dst, R.buf[i], bytes, hipMemcpyHostToDevice, stream)) != hipSuccess)
+    return e;
+  R.used[i] = true;
+  return hipEventRecord(R.done[i], stream);
+}
+
 hipError_t noalloc_scratch(void **out) {
   constexpr int kMaxDevices = 64;
   static std::mutex mu;
@@ -1085,7 +1143,7 @@ hipError_t launch_fft16_jobs(const std::vector<Fft16JobSpec> &jobs, hipStream_t 
   if (e != hipSuccess || plan.total == 0) return e;
   void *mem = nullptr;
   if ((e = hipMallocAsync(&mem, plan.blob.size(), stream)) != hipSuccess) return e;
-  e = hipMemcpyAsync(mem, plan.blob.data(), plan.blob.size(), hipMemcpyHostToDevice, stream);
+  e = staged_upload(mem, plan.blob.data(), plan.blob.size(), stream);
   if (e == hipSuccess) e = fft16_jobs_launch(plan, mem, stream);
   const hipError_t f = hipFreeAsync(mem, stream);
   return e != hipSuccess ? e : f;

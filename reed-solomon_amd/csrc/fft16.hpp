@@ -183,6 +183,11 @@ struct Fft16JobSpec {
   uint64_t plane;
   uint32_t nstore;
   uint32_t vec_ok;
+  // Indexed addressing instead of ptrs (when idx is set): slot i is shard
+  // idx[i] (device array) of the stripe at base, shards stride bytes apart.
+  const uint32_t *idx = nullptr;
+  const uint8_t *base = nullptr;
+  uint64_t stride = 0;
 };
 hipError_t launch_fft16_jobs(const std::vector<Fft16JobSpec> &jobs, hipStream_t stream,
                              bool byte_planes = false);
@@ -200,6 +205,12 @@ struct Fft16JobsPlan {
 hipError_t fft16_jobs_plan(const std::vector<Fft16JobSpec> &jobs, bool byte_planes,
                            Fft16JobsPlan *plan);
 hipError_t fft16_jobs_launch(const Fft16JobsPlan &plan, const void *dev_blob, hipStream_t stream);
+
+// Host-to-device copy of a small per-call table (pointer tables, job
+// tables) through a per-thread ring of pinned staging buffers, so the copy
+// is asynchronous: the host copies `src` into a staging buffer whose previous
+// transfer has finished and returns after enqueueing the transfer on `stream`.
+hipError_t staged_upload(void *dst, const void *src, size_t bytes, hipStream_t stream);
 
 // Scratch of the allocation-free entry point: a file-scope device array of
 // kNoAllocScratchBytes in fft16.hip.  noalloc_scratch returns its address on

@@ -235,3 +235,42 @@ def test_reconstruct_stripes_each_pattern(trs, k, m, stripes):
     assert not buf[missing_parity].any()
     keep = ~missing_parity
     assert torch.equal(buf[keep], pristine[keep])
+
+
+def test_reconstruct_stripes_irregular_pointers(trs):
+    """Shards that are not one (stripes, n, len) tensor -- here every shard a
+    separate allocation, and one stripe's shards in reverse memory order --
+    take the per-call pointer table instead of the plan's indexed addressing;
+    both layouts in one call rebuild every erased shard."""
+    import ctypes
+
+    import torch
+
+    k, m, L, stripes = 17, 33, 4096, 3
+    n = k + m
+    rs = trs.ReedSolomon16(k, m)
+    G = O.build_matrix(16, k, n)
+    data = [seeded(k, L, 0x77 + s) for s in range(stripes)]
+    full = [np.concatenate([d, O.rs16_symbolwise(G[k:], list(d), L)]) for d in data]
+    shards = [[torch.from_numpy(full[s][i].copy()).cuda() for i in range(n)]
+              for s in range(stripes)]
+    rev = torch.from_numpy(np.ascontiguousarray(full[2][::-1])).cuda()  # stripe 2: reversed rows
+    shards[2] = [rev[n - 1 - i] for i in range(n)]
+    rng = np.random.default_rng(5)
+    present = np.ones((stripes, n), np.uint8)
+    for s in range(stripes):
+        present[s, rng.choice(n, m, replace=False)] = 0
+    for s in range(stripes):
+        for i in range(n):
+            if not present[s, i]:
+                shards[s][i].zero_()
+    ptrs = (ctypes.c_void_p * (stripes * n))(*[t.data_ptr() for st in shards for t in st])
+    rs.set_decode_route("fft")
+    rc = trs._lib.lib.trs_rs16_reconstruct_stripes(
+        rs._h, ptrs, present.ctypes.data_as(trs._lib.c_u8p), stripes, L, 0,
+        ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    torch.cuda.synchronize()
+    for s in range(stripes):
+        got = np.stack([t.cpu().numpy() for t in shards[s]])
+        assert (got == full[s]).all(), s
