@@ -624,6 +624,67 @@ def tower_leg(trs, torch, O, stripes: int = 64):
     }
 
 
+def mfma_run(trs, torch, stripes: int = 1024) -> int:
+    """The int8-MFMA kernels beside the bit-sliced planes kernel on the same
+    work, for the MFMA counter runs (SQ_INSTS_VALU_MFMA_MOPS_I8,
+    SQ_VALU_MFMA_BUSY_CYCLES): per call, reconstructs of 3 data shards of RS8
+    (10,4) and of 8 of (32,32) at 256 KiB (k_gf8_mfma against k_gf8_planes);
+    batched, the mixed reconstruct work list of the first `stripes` stripes of
+    the default workload (k_batch_mfma against k_batch8_planes).  Prints the
+    HIP-event time and algorithmic bytes of each."""
+    out = {}
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    L = 256 << 10
+    for (k, m, e) in ((10, 4, 3), (32, 32, 8)):
+        rs = trs.ReedSolomon(k, m)
+        buf = torch.empty((k + m, L), dtype=torch.uint8, device="cuda")
+        trs.fill_random(buf[:k].reshape(-1), SEED ^ k)
+        shards = [buf[i] for i in range(k + m)]
+        rs.encode(shards)
+        present = [i >= e for i in range(k + m)]
+        pairs = [(t, p) for t, p in zip(shards, present)]
+        for route in ("mfma", "table"):  # "table": the planes tile at this length
+            rs.set_apply_route(route)
+            ms = timed(lambda: rs.reconstruct(pairs), 50)
+            out[f"percall_{k}_{m}_{route}"] = {"ms": round(ms, 4), "bytes": (k + e) * L,
+                                               "gbs": round((k + e) * L / ms / 1e6, 1)}
+        del buf
+    meta = build_workload(stripes, np.random.default_rng(SEED))
+    sizes = [(k + m) * Ls for (k, m, Ls, _) in meta]
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    buf = torch.empty(int(offsets[-1]), dtype=torch.uint8, device="cuda")
+    for route in ("mfma", "auto"):
+        codecs = {sh: trs.ReedSolomon(*sh) for sh in SHAPES}
+        for c in codecs.values():
+            c.set_apply_route(route)
+        b = trs.Batch()
+        for s, (k, m, Ls, erased) in enumerate(meta):
+            o = int(offsets[s])
+            if route == "mfma":
+                trs.fill_random(buf[o:o + k * Ls], SEED + s)
+            shards = [buf[o + i * Ls:o + (i + 1) * Ls] for i in range(k + m)]
+            if route == "mfma":
+                codecs[(k, m)].encode(shards)
+            b.add_reconstruct(codecs[(k, m)], shards, [i not in erased for i in range(k + m)])
+        b.prepare()
+        ms = timed(b.run, 5)
+        out[f"batched_{route}"] = {"ms": round(ms, 3), "bytes": b.bytes(),
+                                   "gbs": round(b.bytes() / ms / 1e6, 1)}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def counter_run(trs, torch, stripes: int) -> int:
     """The launches whose HBM traffic `roofline.traffic` reports -- the
     batched encode (k_batch_bs_*) and reconstruct (k_batch8_planes) work lists of the
@@ -694,6 +755,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=800)
     ap.add_argument("--no-tower", action="store_true")
     ap.add_argument("--no-costmodel", action="store_true")
+    ap.add_argument("--mfma", action="store_true",
+                    help="run the int8-MFMA kernels beside the planes kernel (for counter runs)")
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-wide", action="store_true",
                     help="skip the k > 32 leg (staged FFT and FFT decoder vs matrix)")
@@ -721,6 +784,8 @@ def main() -> int:
     stream = torch.cuda.current_stream()
     sptr = ctypes.c_void_p(stream.cuda_stream)
 
+    if args.mfma:
+        return mfma_run(trs, torch)
     if args.counters:
         return counter_run(trs, torch, args.stripes)
 
