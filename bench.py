@@ -59,18 +59,24 @@ def build_workload(stripes: int, rng: np.random.Generator):
 SERVE_SHAPES = [(10, 4), (7, 13), (10, 10), (14, 14), (16, 16), (18, 6), (32, 32)]
 
 
-def served_leg(trs, torch, requests: int, rates, seed: int):
+def served_leg(trs, torch, requests: int, rates, seed: int, pool: int = 512):
     """Served mixed workload: Poisson arrivals, 70 % encode / 30 % reconstruct,
     shapes of the C1/C2 configurations, 90 % GF(2^8) / 10 % GF(2^16), shard
     length log-uniform [4 KiB, 4 MiB], erasures uniform in 1..m, device-resident
-    shards; continuous batching (4 slots, 64 GiB HBM budget) against the same
-    requests issued as plain per-request calls."""
+    shards.  `requests` requests per rate are drawn from a pool of `pool`
+    encoded stripes (a request on a stripe rewrites bytes it already holds, so
+    overlapping requests on one stripe are harmless); continuous batching
+    (4 slots, 64 GiB budget) against the same requests issued as plain
+    per-request calls.  Latency is split into queued / host launch / device
+    phases, and the lowest rate is repeated with the server's threads polling
+    for 200 us before they block (the default blocks at once).  The plain
+    calls' own throughput bounds the rate they could sustain at all."""
     rng = np.random.default_rng(seed)
     c8 = {s: trs.ReedSolomon(*s) for s in SERVE_SHAPES}
     c16 = {}
-    reqs = []
+    stripes = []
     lo, hi = math.log(4096), math.log(4 << 20)
-    for i in range(requests):
+    for i in range(pool):
         gf16 = rng.random() < 0.10
         k, m = SERVE_SHAPES[int(rng.integers(0, len(SERVE_SHAPES)))]
         L = int(round(math.exp(rng.uniform(lo, hi)) / 64.0)) * 64
@@ -79,8 +85,12 @@ def served_leg(trs, torch, requests: int, rates, seed: int):
         rs = c16[(k, m)] if gf16 else c8[(k, m)]
         buf = torch.empty((k + m) * L, dtype=torch.uint8, device="cuda")
         trs.fill_random(buf[:k * L], seed + i)
-        shards = [buf[j * L:(j + 1) * L] for j in range(k + m)]
+        shards = trs.pack_shards([buf[j * L:(j + 1) * L] for j in range(k + m)])
         rs.encode(shards)
+        stripes.append((rs, shards, k, m, L))
+    reqs = []
+    for _ in range(requests):
+        rs, shards, k, m, L = stripes[int(rng.integers(0, pool))]
         if rng.random() < 0.7:
             reqs.append(("enc", rs, shards, None, (k + m) * L))
         else:
@@ -105,16 +115,20 @@ def served_leg(trs, torch, requests: int, rates, seed: int):
     plain_s = time.perf_counter() - t0
     plain = {"gbs": round(total_bytes / plain_s / 1e9, 2),
              "p50_us": round(float(np.percentile(lat, 50)), 1),
-             "p99_us": round(float(np.percentile(lat, 99)), 1)}
+             "p99_us": round(float(np.percentile(lat, 99)), 1),
+             "max_req_s": round(len(reqs) / plain_s, 1)}
 
-    out = {"requests": requests, "bytes": total_bytes, "plain_per_request": plain, "rates": []}
+    out = {"requests_per_rate": requests, "pool_stripes": pool, "bytes": total_bytes,
+           "plain_per_request": plain, "rates": []}
     srv = trs.Server(slots=4, hbm_budget_bytes=64 << 30, max_batch=256, window_us=50)
-    for rate in rates:
-        gaps = rng.exponential(1.0 / rate, len(reqs))
-        sched = np.cumsum(gaps)
+
+    def run(rate, n, spin):
+        srv.set_spin(spin)
+        sub = reqs[:n]
+        sched = np.cumsum(rng.exponential(1.0 / rate, n))
         tickets, submit_t = [], []
         start = time.perf_counter()
-        for (kind, rs, shards, present, _), at in zip(reqs, sched):
+        for (kind, rs, shards, present, _), at in zip(sub, sched):
             while time.perf_counter() - start < at:
                 pass
             submit_t.append(time.perf_counter())
@@ -123,20 +137,33 @@ def served_leg(trs, torch, requests: int, rates, seed: int):
             else:
                 tickets.append(srv.submit_reconstruct(rs, shards, present))
         srv.drain()
-        lats, ends = [], []
+        lats, phases, ends = [], [], []
         for t, st in zip(tickets, submit_t):
-            l_us, _ = srv.take(t)
+            l_us, ph, _ = srv.take_detail(t)
             lats.append(l_us)
+            phases.append(ph)
             ends.append(st + l_us * 1e-6)
         span = max(ends) - min(submit_t)
-        out["rates"].append({
+        ph = np.asarray(phases)
+        return {
             "rate_req_s": rate,
-            "gbs": round(total_bytes / span / 1e9, 2),
+            "requests": n,
+            "spin_us": spin,
+            "gbs": round(sum(r[4] for r in sub) / span / 1e9, 2),
             "p50_us": round(float(np.percentile(lats, 50)), 1),
             "p99_us": round(float(np.percentile(lats, 99)), 1),
+            "p50_phases_us": {"queued": round(float(np.median(ph[:, 0])), 1),
+                              "launch": round(float(np.median(ph[:, 1])), 1),
+                              "device": round(float(np.median(ph[:, 2])), 1)},
             "offered_s": round(float(sched[-1]), 4),
+            "submitted_req_s": round(n / max(submit_t[-1] - submit_t[0], 1e-9), 1),
             "span_s": round(span, 4),
-        })
+        }
+
+    for rate in rates:
+        out["rates"].append(run(rate, len(reqs), 0))
+    out["polling_lowest_rate"] = run(rates[0], min(len(reqs), 3000), 200)
+    out["server_stats"] = srv.stats()
     srv.close()
     return out
 
@@ -752,7 +779,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=800)
+    ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--no-tower", action="store_true")
     ap.add_argument("--no-costmodel", action="store_true")
     ap.add_argument("--mfma", action="store_true",

@@ -356,7 +356,8 @@ size_t trs_batch8_size(const trs_batch8 *b);
  * into the batch of a free slot (slot = stream + batch) while the bytes of
  * in-flight work stay within hbm_budget_bytes (0 = unlimited), and launches
  * it, so requests arriving while others run start as soon as a slot and budget
- * free up.  window_us: how long the dispatcher waits for more arrivals before
+ * free up.  A GF(2^8) request that is alone in its slot runs as its plain
+ * per-call launch (no batch tables to build and upload).  window_us: how long the dispatcher waits for more arrivals before
  * launching a partial batch while other slots have work in flight (with every
  * slot idle a request launches at once).  Shard buffers must stay valid until
  * the ticket completes. */
@@ -381,6 +382,30 @@ int trs_server_drain(trs_server *s);
  * a finished ticket; the ticket is forgotten afterwards. */
 int trs_server_take(trs_server *s, uint64_t ticket, double *latency_us, uint64_t *bytes,
                     int *status);
+/* trs_server_take plus where the latency went: phases_us[0] queued (submit to
+ * taken into a slot's batch), [1] host launch (batch tables built, uploaded
+ * and kernels enqueued), [2] device and completion (enqueued to seen done).
+ * phases_us may be NULL; a request with nothing to do reports zeros. */
+int trs_server_take_detail(trs_server *s, uint64_t ticket, double *latency_us,
+                           double *phases_us, uint64_t *bytes, int *status);
+/* Device memory the server's own batch tables may not take (default 256 MiB).
+ * Before a slot's table buffer grows, the dispatcher asks hipMemGetInfo for
+ * free HBM; a batch whose tables need more than free less this reserve is
+ * split in two (the later half returns to the head of the queue) until it
+ * fits.  A GF(2^8) request alone in a slot runs as its plain per-call launch
+ * (shard pointers in the kernel arguments), which needs no tables, so
+ * splitting always ends; should an allocation still fail, the slot's requests
+ * complete with TRS_ERR_OUT_OF_MEMORY and the server goes on. */
+int trs_server_set_hbm_reserve(trs_server *s, uint64_t reserve_bytes);
+/* Microseconds the dispatcher (waiting for a submission) and the completion
+ * threads (waiting for a batch) poll before they block (default 200, at most
+ * 100000; 0 = block at once).  Polling trades a busy host thread for not
+ * paying a thread wake-up on each request at low load. */
+int trs_server_set_spin(trs_server *s, uint32_t spin_us);
+/* Counters since creation: out4[0] requests completed through a slot, [1]
+ * batches launched, [2] batches split because their tables did not fit free
+ * HBM less the reserve, [3] requests failed with TRS_ERR_OUT_OF_MEMORY. */
+int trs_server_stats(trs_server *s, uint64_t *out4);
 
 /* ============ host-resident shards: the reference's slice API ============== */
 /* The reference's methods take host slices (src/reedsolomon.rs:271-535,

@@ -45,6 +45,8 @@ __all__ = [
     "PreparedDecoder16",
     "Batch",
     "Server",
+    "PackedShards",
+    "pack_shards",
     "mul_slice",
     "mul_slice_xor",
     "fill_random",
@@ -117,7 +119,25 @@ def _dev_ptr(t) -> int:
     return t.data_ptr()
 
 
+class PackedShards(list):
+    """A list of device shards whose pointer and length arrays are built
+    once: for callers that issue the same stripe many times (a server's
+    clients, benchmarks), every call then skips the per-shard packing.  The
+    shards must not be replaced or resized while the pack is in use."""
+
+    def __init__(self, shards: Sequence):
+        super().__init__(shards)
+        self._trs_packed = _ptrs(list(self))
+
+
+def pack_shards(shards: Sequence) -> PackedShards:
+    return PackedShards(shards)
+
+
 def _ptrs(tensors: Sequence) -> tuple:
+    packed = getattr(tensors, "_trs_packed", None)
+    if packed is not None:
+        return packed
     arr = (ctypes.c_void_p * max(1, len(tensors)))()
     lens = (ctypes.c_size_t * max(1, len(tensors)))()
     for i, t in enumerate(tensors):
@@ -749,6 +769,33 @@ class Server:
         self._keep.pop(ticket, None)
         _check(st.value)
         return lat.value, nb.value
+
+    def take_detail(self, ticket: int):
+        """(latency_us, (queued_us, launch_us, device_us), bytes) of a finished
+        ticket: queued = submit to taken into a batch, launch = host-side build
+        and enqueue, device = enqueued to seen complete."""
+        lat, nb, st = ctypes.c_double(0), ctypes.c_uint64(0), ctypes.c_int(0)
+        ph = (ctypes.c_double * 3)()
+        _check(lib.trs_server_take_detail(self._h, ticket, ctypes.byref(lat), ph,
+                                          ctypes.byref(nb), ctypes.byref(st)))
+        self._keep.pop(ticket, None)
+        _check(st.value)
+        return lat.value, (ph[0], ph[1], ph[2]), nb.value
+
+    def set_hbm_reserve(self, reserve_bytes: int) -> None:
+        """Free HBM the server's own batch tables must leave untouched."""
+        _check(lib.trs_server_set_hbm_reserve(self._h, reserve_bytes))
+
+    def stats(self) -> dict:
+        """Counters: requests completed, batches launched, batches split for
+        memory, requests failed for memory."""
+        out = (ctypes.c_uint64 * 4)()
+        _check(lib.trs_server_stats(self._h, out))
+        return {"done": out[0], "batches": out[1], "splits": out[2], "oom": out[3]}
+
+    def set_spin(self, spin_us: int) -> None:
+        """Microseconds the server's threads poll before blocking (0 = block)."""
+        _check(lib.trs_server_set_spin(self._h, spin_us))
 
 
 def _slice(out, inp, c: int, xor: bool) -> None:

@@ -163,6 +163,11 @@ _sig("trs_server_wait", c_int, c_vp, ctypes.c_uint64, ctypes.POINTER(c_int))
 _sig("trs_server_drain", c_int, c_vp)
 _sig("trs_server_take", c_int, c_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double), c_u64p,
      ctypes.POINTER(c_int))
+_sig("trs_server_take_detail", c_int, c_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
+     ctypes.POINTER(ctypes.c_double), c_u64p, ctypes.POINTER(c_int))
+_sig("trs_server_set_hbm_reserve", c_int, c_vp, ctypes.c_uint64)
+_sig("trs_server_set_spin", c_int, c_vp, ctypes.c_uint32)
+_sig("trs_server_stats", c_int, c_vp, c_u64p)
 
 
 def declared_symbols() -> list:

@@ -2478,10 +2478,19 @@ struct trs_batch8 {
     uint32_t grid = 0;
   };
   std::vector<Req> reqs;
-  // One device buffer for everything prepare uploads (pointer tables, each
-  // list's jobs and tile map, the interpreter job table), so a prepare makes
-  // a single host-to-device copy.
+  // One device buffer for everything prepare lays out (pointer tables, each
+  // list's jobs and tile map, the interpreter job table).  Prepare builds its
+  // image in `image`; the next run copies it up on the run's stream through
+  // the pinned staging ring (one asynchronous copy, ordered before the
+  // launches), so neither call blocks on a pageable copy.
   DevBuf blob;
+  std::vector<uint8_t> image;
+  size_t image_bytes = 0;
+  bool upload_pending = false;
+  // Largest growth of `blob` that prepare may allocate (the server lowers it
+  // to the device's free memory less its reserve); past it prepare fails
+  // with TRS_ERR_OUT_OF_MEMORY before allocating.
+  size_t grow_limit = SIZE_MAX;
   List table, mfma;
   const void *fft_table = nullptr;  // interpreter job table, inside `blob`
   Fft16JobsPlan fft_plan;
@@ -2535,6 +2544,7 @@ void trs_batch8_clear(trs_batch8 *b) {
   (void)b->wait_done();
   b->reqs.clear();
   b->prepared = false;
+  b->upload_pending = false;
   b->drop_graph();
   b->table.njobs = b->mfma.njobs = 0;
   b->table.total_tiles = b->mfma.total_tiles = 0;
@@ -2723,12 +2733,14 @@ int trs_batch8_prepare(trs_batch8 *b) {
   TRS_TRY(fft16_jobs_plan(fjobs, true, &b->fft_plan));
   const size_t fft_off = size;
   size = align(size + b->fft_plan.blob.size());
+  if (size > b->blob.n && size - b->blob.n > b->grow_limit) return TRS_ERR_OUT_OF_MEMORY;
   TRS_TRY(b->blob.ensure(size));
   const uintptr_t base = reinterpret_cast<uintptr_t>(b->blob.p);
   auto patch = [base](const uint8_t *const *q) {
     return reinterpret_cast<const uint8_t *const *>(base + reinterpret_cast<uintptr_t>(q));
   };
-  std::vector<uint8_t> host(size, 0);
+  std::vector<uint8_t> &host = b->image;
+  host.assign(size, 0);
   if (!hp.empty()) std::memcpy(host.data(), hp.data(), hp.size() * sizeof(void *));
   for (Piece &pc : pieces) {
     for (BatchJob &J : *pc.jobs) J.ptrs = patch(J.ptrs);
@@ -2745,7 +2757,8 @@ int trs_batch8_prepare(trs_batch8 *b) {
       b->fft_table = reinterpret_cast<const void *>(base + fft_off);
     }
   }
-  TRS_TRY(hipMemcpy(b->blob.p, host.data(), size, hipMemcpyHostToDevice));
+  b->image_bytes = size;
+  b->upload_pending = true;
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -2809,6 +2822,11 @@ int trs_batch8_run(trs_batch8 *b, void *stream) {
   }
   hipStream_t s = as_stream(stream);
   if (!b->done) TRS_TRY(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+  if (b->upload_pending) {
+    // prepare waited for earlier runs, so nothing reads `blob` any more
+    TRS_TRY(staged_upload(b->blob.p, b->image.data(), b->image_bytes, s));
+    b->upload_pending = false;
+  }
   if (!b->use_graph) {
     const int st = batch8_launch(b, s);
     if (st) return st;
@@ -3132,6 +3150,11 @@ struct SrvReq {
   uint64_t ticket = 0;
   bool gf16 = false;
   trs_batch8::Req r8;                  // GF(2^8) request (batched)
+  // GF(2^8) request run alone: the codec and the kind select the plain
+  // call's own launch (kernel arguments carry the shard pointers, so nothing
+  // is copied to the device first)
+  const trs_rs8 *rs8 = nullptr;
+  bool rec8 = false;
   std::shared_ptr<Plan> plan16;        // tower reconstruct plan (keeps coef alive)
   const DevCoef *coef16 = nullptr;     // tower coefficients
   std::vector<const uint8_t *> ptrs16;
@@ -3144,7 +3167,8 @@ struct SrvReq {
 struct TicketState {
   int status = 0;
   bool done = false;
-  Clock::time_point submit, finish;
+  // submitted; taken into a slot's batch; on the stream; seen complete
+  Clock::time_point submit, dispatch, launched, finish;
   uint64_t bytes = 0;
 };
 
@@ -3165,6 +3189,16 @@ struct trs_server {
   uint64_t budget = 0;
   size_t max_batch = 0;
   uint32_t window_us = 0;
+  // Device memory the server's own tables may not take: a batch whose tables
+  // need more than free HBM less this reserve is split, and a lone request
+  // that still does not fit fails with TRS_ERR_OUT_OF_MEMORY.
+  std::atomic<uint64_t> hbm_reserve{uint64_t(256) << 20};
+  // How long the dispatcher and completion threads poll before they block.
+  std::atomic<uint32_t> spin_us{0};
+  std::atomic<uint64_t> npending{0};  // mirrors pending.size() for lock-free polling
+  // counters (under mu): requests completed, batches launched, batches split
+  // for memory, requests failed for memory
+  uint64_t n_done = 0, n_batches = 0, n_splits = 0, n_oom = 0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<SrvReq> pending;
@@ -3180,9 +3214,106 @@ struct trs_server {
 
 namespace {
 
+// Poll `ready` for up to `spin_us` before the caller blocks: a thread that
+// sleeps in the kernel takes tens of microseconds to be woken, which at low
+// load is most of a small request's latency.
+template <class F>
+void spin_until(uint32_t spin_us, F ready) {
+  if (!spin_us) return;
+  const auto until = Clock::now() + std::chrono::microseconds(spin_us);
+  while (!ready() && Clock::now() < until) std::this_thread::yield();
+}
+
+// Launches the requests placed in `slot->reqs` (called with the lock held,
+// returns with it held): marks the slot busy, builds and enqueues its work
+// outside the lock, records the slot's completion event and hands the slot
+// to its completion thread.
+void launch_slot(trs_server *S, Slot *slot, size_t si, std::unique_lock<std::mutex> &lk) {
+  slot->busy = true;
+  S->inflight += slot->bytes;
+  lk.unlock();
+  // build and launch the slot's batch outside the lock
+  int st = TRS_OK;
+  bool alone = false;
+  while (true) {
+    alone = slot->reqs.size() == 1 && !slot->reqs[0].gf16 && slot->reqs[0].rs8;
+    if (alone) {
+      // one GF(2^8) request: its plain per-call launch, as trs_rs8_encode /
+      // trs_rs8_reconstruct make it, with no batch tables to upload
+      const SrvReq &q = slot->reqs[0];
+      const trs_rs8 *rs = q.rs8;
+      st = q.rec8 ? rec8_launch(*q.r8.plan, q.r8.k, rs->n, q.r8.nout, q.r8.ptrs, q.r8.len,
+                                slot->stream, rs->apply_route.load(), rs->decode_route.load())
+                  : encode8_auto(rs, q.r8.ptrs, q.r8.len, Exec(slot->stream));
+      break;
+    }
+    trs_batch8_clear(&slot->batch);
+    for (auto &q : slot->reqs)
+      if (!q.gf16) slot->batch.reqs.push_back(q.r8);
+    if (slot->batch.reqs.empty()) break;
+    // Tables that fit the slot's buffer need no memory; growing it is
+    // admitted against free HBM less the reserve.
+    slot->batch.grow_limit = 0;
+    st = trs_batch8_prepare(&slot->batch);
+    if (st != TRS_ERR_OUT_OF_MEMORY) break;
+    size_t free_b = 0, total_b = 0;
+    st = hip_status(hipMemGetInfo(&free_b, &total_b));
+    if (st) break;
+    const uint64_t reserve = S->hbm_reserve.load();
+    slot->batch.grow_limit = free_b > reserve ? size_t(free_b - reserve) : 0;
+    st = trs_batch8_prepare(&slot->batch);
+    if (st != TRS_ERR_OUT_OF_MEMORY || slot->reqs.size() == 1) break;
+    // split: the later half goes back to the head of the queue, in order
+    // (a lone GF(2^8) request needs no tables at all)
+    const size_t keep = slot->reqs.size() / 2;
+    uint64_t back = 0;
+    lk.lock();
+    for (size_t i = slot->reqs.size(); i-- > keep;) {
+      back += slot->reqs[i].bytes;
+      S->pending.push_front(std::move(slot->reqs[i]));
+    }
+    slot->reqs.resize(keep);
+    slot->bytes -= back;
+    S->inflight -= back;
+    S->npending.store(S->pending.size());
+    ++S->n_splits;
+    lk.unlock();
+    st = TRS_OK;
+  }
+  if (!alone && !st && !slot->batch.reqs.empty())
+    st = trs_batch8_run(&slot->batch, slot->stream);
+  for (auto &q : slot->reqs) {
+    if (!q.gf16 || st) continue;
+    st = q.fft16 ? run_fft16_program(*q.fft16, q.ptrs16, q.nout16, q.len16, slot->stream)
+                 : run_gf16(q.ptrs16, q.k16, q.nout16, *q.coef16, q.len16, kWrite, nullptr,
+                            slot->stream);
+  }
+  if (!st) st = hip_status(hipEventRecord(slot->done, slot->stream));
+  const auto launched = Clock::now();
+  lk.lock();
+  for (auto &q : slot->reqs) S->tickets[q.ticket].launched = launched;
+  ++S->n_batches;
+  if (st == TRS_ERR_OUT_OF_MEMORY) S->n_oom += slot->reqs.size();
+  slot->launch_status = st;
+  slot->launched = true;
+  S->slot_cv[si]->notify_one();
+}
+
+void server_dispatch_loop(trs_server *S);
+
 void server_dispatch(trs_server *S) {
+  server_dispatch_loop(S);
+  staged_upload_release();  // the batches' table uploads used this thread's ring
+}
+
+void server_dispatch_loop(trs_server *S) {
   std::unique_lock<std::mutex> lk(S->mu);
   while (true) {
+    if (S->pending.empty() && !S->stop) {
+      lk.unlock();
+      spin_until(S->spin_us.load(), [&] { return S->npending.load() != 0; });
+      lk.lock();
+    }
     S->cv_work.wait(lk, [&] {
       if (S->stop) return true;
       if (S->pending.empty()) return false;
@@ -3215,6 +3346,7 @@ void server_dispatch(trs_server *S) {
     // admission by HBM budget, in arrival order
     slot->reqs.clear();
     slot->bytes = 0;
+    const auto taken = Clock::now();
     while (!S->pending.empty() && slot->reqs.size() < S->max_batch) {
       const uint64_t b = S->pending.front().bytes;
       if (S->inflight + slot->bytes + b > S->budget && !(S->inflight == 0 && slot->reqs.empty()))
@@ -3222,7 +3354,9 @@ void server_dispatch(trs_server *S) {
       slot->bytes += b;
       slot->reqs.push_back(std::move(S->pending.front()));
       S->pending.pop_front();
+      S->tickets[slot->reqs.back().ticket].dispatch = taken;
     }
+    S->npending.store(S->pending.size());
     if (slot->reqs.empty()) {
       // the next request does not fit beside in-flight work: wait for completions
       S->cv_work.wait(lk, [&] {
@@ -3231,29 +3365,7 @@ void server_dispatch(trs_server *S) {
       });
       continue;
     }
-    slot->busy = true;
-    S->inflight += slot->bytes;
-    lk.unlock();
-    // build and launch the slot's batch outside the lock
-    int st = TRS_OK;
-    trs_batch8_clear(&slot->batch);
-    for (auto &q : slot->reqs)
-      if (!q.gf16) slot->batch.reqs.push_back(q.r8);
-    if (!slot->batch.reqs.empty()) {
-      st = trs_batch8_prepare(&slot->batch);
-      if (!st) st = trs_batch8_run(&slot->batch, slot->stream);
-    }
-    for (auto &q : slot->reqs) {
-      if (!q.gf16 || st) continue;
-      st = q.fft16 ? run_fft16_program(*q.fft16, q.ptrs16, q.nout16, q.len16, slot->stream)
-                   : run_gf16(q.ptrs16, q.k16, q.nout16, *q.coef16, q.len16, kWrite, nullptr,
-                              slot->stream);
-    }
-    if (!st) st = hip_status(hipEventRecord(slot->done, slot->stream));
-    lk.lock();
-    slot->launch_status = st;
-    slot->launched = true;
-    S->slot_cv[si]->notify_one();
+    launch_slot(S, slot, si, lk);
   }
 }
 
@@ -3266,7 +3378,10 @@ void server_complete(trs_server *S, size_t si) {
     const int launch = slot->launch_status;
     lk.unlock();
     int st = launch;
-    if (!st) st = hip_status(hipEventSynchronize(slot->done));
+    if (!st) {
+      spin_until(S->spin_us.load(), [&] { return hipEventQuery(slot->done) != hipErrorNotReady; });
+      st = hip_status(hipEventSynchronize(slot->done));
+    }
     const auto now = Clock::now();
     lk.lock();
     for (auto &q : slot->reqs) {
@@ -3275,6 +3390,7 @@ void server_complete(trs_server *S, size_t si) {
       t.status = st;
       t.finish = now;
     }
+    S->n_done += slot->reqs.size();
     slot->reqs.clear();
     S->inflight -= slot->bytes;
     slot->bytes = 0;
@@ -3287,7 +3403,7 @@ void server_complete(trs_server *S, size_t si) {
 }
 
 int server_enqueue(trs_server *S, SrvReq &&q, uint64_t *ticket) {
-  std::lock_guard<std::mutex> lk(S->mu);
+  std::unique_lock<std::mutex> lk(S->mu);
   q.ticket = S->next_ticket++;
   TicketState t;
   t.submit = Clock::now();
@@ -3298,10 +3414,25 @@ int server_enqueue(trs_server *S, SrvReq &&q, uint64_t *ticket) {
   }
   S->tickets[q.ticket] = t;
   *ticket = q.ticket;
-  if (q.bytes) {
-    S->pending.push_back(std::move(q));
-    S->cv_work.notify_one();
+  if (!q.bytes) return TRS_OK;
+  // With nothing queued and every slot idle (the dispatcher would launch at
+  // once), the submitting thread launches the request itself: no wake-up of
+  // the dispatcher thread on the way.  A budget too small for the request
+  // still admits it, as the dispatcher does when nothing is in flight.
+  bool idle = S->pending.empty() && !S->stop;
+  for (auto &sl : S->slots) idle = idle && !sl->busy;
+  if (idle) {
+    Slot *slot = S->slots[0].get();
+    slot->reqs.clear();
+    slot->bytes = q.bytes;
+    S->tickets[q.ticket].dispatch = Clock::now();
+    slot->reqs.push_back(std::move(q));
+    launch_slot(S, slot, 0, lk);
+    return TRS_OK;
   }
+  S->pending.push_back(std::move(q));
+  S->npending.store(S->pending.size());
+  S->cv_work.notify_one();
   return TRS_OK;
 }
 
@@ -3363,6 +3494,7 @@ int trs_server_submit_encode8(trs_server *S, const trs_rs8 *rs, uint8_t *const *
   if (st) return st;
   SrvReq q;
   q.r8 = std::move(tmp.reqs.front());
+  q.rs8 = rs;
   q.bytes = uint64_t(q.r8.k + q.r8.nout) * q.r8.len;
   return server_enqueue(S, std::move(q), ticket);
 }
@@ -3377,6 +3509,8 @@ int trs_server_submit_reconstruct8(trs_server *S, const trs_rs8 *rs, uint8_t *co
   SrvReq q;
   if (!tmp.reqs.empty()) {
     q.r8 = std::move(tmp.reqs.front());
+    q.rs8 = rs;
+    q.rec8 = true;
     q.bytes = uint64_t(q.r8.k + q.r8.nout) * q.r8.len;
   }
   return server_enqueue(S, std::move(q), ticket);
@@ -3475,16 +3609,52 @@ int trs_server_drain(trs_server *S) {
 // the entry is forgotten afterwards.
 int trs_server_take(trs_server *S, uint64_t ticket, double *latency_us, uint64_t *bytes,
                     int *status) {
+  return trs_server_take_detail(S, ticket, latency_us, nullptr, bytes, status);
+}
+
+int trs_server_take_detail(trs_server *S, uint64_t ticket, double *latency_us,
+                           double *phases_us, uint64_t *bytes, int *status) {
   if (!S) return TRS_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lk(S->mu);
   auto it = S->tickets.find(ticket);
   if (it == S->tickets.end() || !it->second.done) return TRS_ERR_INVALID_ARGUMENT;
-  if (latency_us)
-    *latency_us =
-        std::chrono::duration<double, std::micro>(it->second.finish - it->second.submit).count();
-  if (bytes) *bytes = it->second.bytes;
-  if (status) *status = it->second.status;
+  const TicketState &t = it->second;
+  auto us = [](Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::micro>(b - a).count();
+  };
+  if (latency_us) *latency_us = us(t.submit, t.finish);
+  if (phases_us) {
+    // a request with nothing to do never reaches a slot
+    const bool ran = t.dispatch != Clock::time_point{};
+    phases_us[0] = ran ? us(t.submit, t.dispatch) : 0.0;
+    phases_us[1] = ran ? us(t.dispatch, t.launched) : 0.0;
+    phases_us[2] = ran ? us(t.launched, t.finish) : 0.0;
+  }
+  if (bytes) *bytes = t.bytes;
+  if (status) *status = t.status;
   S->tickets.erase(it);
+  return TRS_OK;
+}
+
+int trs_server_set_hbm_reserve(trs_server *S, uint64_t reserve_bytes) {
+  if (!S) return TRS_ERR_INVALID_ARGUMENT;
+  S->hbm_reserve.store(reserve_bytes);
+  return TRS_OK;
+}
+
+int trs_server_stats(trs_server *S, uint64_t *out4) {
+  if (!S || !out4) return TRS_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(S->mu);
+  out4[0] = S->n_done;
+  out4[1] = S->n_batches;
+  out4[2] = S->n_splits;
+  out4[3] = S->n_oom;
+  return TRS_OK;
+}
+
+int trs_server_set_spin(trs_server *S, uint32_t spin_us) {
+  if (!S || spin_us > 100000) return TRS_ERR_INVALID_ARGUMENT;
+  S->spin_us.store(spin_us);
   return TRS_OK;
 }
 

@@ -1085,12 +1085,30 @@ struct StagingRing {
   bool used[N] = {};
   int next = 0;
 };
+// One ring per host thread, made on first use.  Threads of the process's own
+// lifetime keep theirs (freeing pinned memory during exit is unsafe); a
+// thread that ends before the process releases its ring first.
+thread_local StagingRing *t_ring = nullptr;
 }  // namespace
+
+void staged_upload_release() {
+  StagingRing *R = t_ring;
+  if (!R) return;
+  t_ring = nullptr;
+  for (int j = 0; j < StagingRing::N; ++j) {
+    if (R->done[j]) {
+      if (R->used[j]) (void)hipEventSynchronize(R->done[j]);
+      (void)hipEventDestroy(R->done[j]);
+    }
+    if (R->buf[j]) (void)hipHostFree(R->buf[j]);
+  }
+  delete R;
+}
 
 hipError_t staged_upload(void *dst, const void *src, size_t bytes, hipStream_t stream) {
   if (bytes == 0) return hipSuccess;
-  static thread_local StagingRing *ring = new StagingRing();
-  StagingRing &R = *ring;
+  if (!t_ring) t_ring = new StagingRing();
+  StagingRing &R = *t_ring;
   const int i = R.next;
   R.next = (R.next + 1) % StagingRing::N;
   hipError_t e;

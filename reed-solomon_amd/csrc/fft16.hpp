@@ -211,6 +211,9 @@ hipError_t fft16_jobs_launch(const Fft16JobsPlan &plan, const void *dev_blob, hi
 // is asynchronous: the host copies `src` into a staging buffer whose previous
 // transfer has finished and returns after enqueueing the transfer on `stream`.
 hipError_t staged_upload(void *dst, const void *src, size_t bytes, hipStream_t stream);
+// Frees the calling thread's staging ring (after its transfers finish); for
+// threads that end before the process does.
+void staged_upload_release();
 
 // Scratch of the allocation-free entry point: a file-scope device array of
 // kNoAllocScratchBytes in fft16.hip.  noalloc_scratch returns its address on
