@@ -385,7 +385,7 @@ def c1_leg(trs, torch, O, stripes: int = 256, L: int = 65536):
             e1.record()
             ev.append((e0, e1))
         torch.cuda.synchronize()
-        return float(np.mean([a.elapsed_time(b) for a, b in ev]))
+        return float(np.median([a.elapsed_time(b) for a, b in ev]))
 
     enc_ms = timed(enc)
     rec_ms = timed(rec, lambda: buf.masked_fill_(mask[:, :, None], 0))
@@ -618,15 +618,18 @@ def tower_leg(trs, torch, O, stripes: int = 64):
         first_s = time.perf_counter() - t0
         first_exact = bool(torch.equal(buf, pristine))
         ev = []
-        for _ in range(3 if route == "fft" else 2):
+        for _ in range(5 if route == "fft" else 2):
             erase_all()
+            # the masked fill returns before its kernels finish: wait, so
+            # that the interval holds the call (host work + kernels) alone
+            torch.cuda.synchronize()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
             reconstruct_all()
             e1.record()
             ev.append((e0, e1))
         torch.cuda.synchronize()
-        ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+        ms = float(np.median([a.elapsed_time(b) for a, b in ev]))
         rec[route] = {"ms": round(ms, 3), "gbs": round(rec_bytes / ms / 1e6, 2),
                       "first_call_s": round(first_s, 3), "first_exact": first_exact,
                       "cached_exact": bool(torch.equal(buf, pristine))}
