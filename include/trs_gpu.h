@@ -168,7 +168,10 @@ size_t trs_rs16_total_shard_count(const trs_rs16 *rs);
  * "matrix".  The reference routes per host (src/fft16.rs:70-106; both routes
  * are byte-identical); here the route is "fft" whenever the shape has a staged
  * program (2 <= k <= 1024, k + m <= 4096) whose register file fits a
- * workgroup's LDS, the GPU's measured crossover. */
+ * workgroup's LDS.  The answer takes no shard length, as the reference's:
+ * for short data (k <= 16) the automatic encode still runs a matrix kernel
+ * where it measured faster (the Karatsuba kernel from 4 MiB shards, int8
+ * MFMA for m <= 4 from 512 KiB to 4 MiB). */
 const char *trs_rs16_encode_route(const trs_rs16 *rs);
 /* Host copy of the n x k tower generator (row-major, n*k uint16). */
 int trs_rs16_generator(const trs_rs16 *rs, uint16_t *out, size_t out_len);
@@ -280,6 +283,16 @@ int trs_fft8_decode_program(size_t k, size_t n, const uint8_t *present, uint32_t
  * power of two >= k + m, and k + m <= 2048.  Route 2 past 2048 shards:
  * TRS_ERR_UNSUPPORTED.  Prepared decoders keep the route they were made with. */
 int trs_rs16_set_decode_route(trs_rs16 *rs, int route);
+/* Matrix-apply route of the tower codec (matrix-route encode incl.
+ * trs_rs16_encode_sep_matrix, matrix-plan reconstruct, and decoders prepared
+ * after the call): 0 = automatic, 1 = the Karatsuba table kernel, 2 = int8
+ * MFMA (k <= 16 and 2 (k + m) <= 256; TRS_ERR_UNSUPPORTED otherwise).  The
+ * MFMA form treats each split-plane shard as two byte shards (its low and
+ * high planes) and multiplies by the 16m x 16k GF(2) expansion of the tower
+ * matrix, keeping the low bit of each exact int32 sum.  Every route is
+ * byte-identical; no reference counterpart (its tower kernels are CPU code,
+ * src/gf/tower_x86.rs:33-133). */
+int trs_rs16_set_apply_route(trs_rs16 *rs, int route);
 /* Reconstruct of `stripes` stripes of the codec's shape in one call, each with
  * its own presence flags: shards holds stripes * (k + m) pointers, present
  * stripes * (k + m) flags, every shard `len` bytes.  Per stripe this is

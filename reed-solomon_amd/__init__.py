@@ -547,6 +547,15 @@ class ReedSolomon16(_Codec):
         (additive-FFT erasure decoder) or "auto"."""
         _check(lib.trs_rs16_set_decode_route(self._h, self.DECODE_ROUTES.index(route)))
 
+    APPLY_ROUTES = {"auto": 0, "table": 1, "mfma": 2}
+
+    def set_apply_route(self, route: str) -> None:
+        """Device route for tower matrix applies (matrix-route encode,
+        encode_sep_matrix, matrix-plan reconstruct): "auto", "table" (the
+        Karatsuba kernel) or "mfma" (int8 MFMA over the 16m x 16k bit
+        expansion; k <= 16).  All byte-identical."""
+        _check(lib.trs_rs16_set_apply_route(self._h, self.APPLY_ROUTES[route]))
+
     def reconstruct_stripes(self, stripes, present, data_only: bool = False) -> None:
         """Rebuilds the missing shards of many stripes in one call: ``stripes``
         is a uint8 device tensor (stripes, k + m, length) with a contiguous last

@@ -95,6 +95,7 @@ _sig("trs_rs16_encode_sep_fft", c_int, c_vp, c_ptrs, c_sizep, c_size, c_ptrs, c_
 _sig("trs_rs16_encode_stripes", c_int, c_vp, c_ptrs, c_size, c_size, c_int, c_vp)
 _sig("trs_rs16_encode_stripes_noalloc", c_int, c_vp, c_ptrs, c_size, c_size, c_vp)
 _sig("trs_rs16_set_decode_route", c_int, c_vp, c_int)
+_sig("trs_rs16_set_apply_route", c_int, c_vp, c_int)
 _sig("trs_rs16_reconstruct_stripes", c_int, c_vp, c_ptrs, c_u8p, c_size, c_size, c_int, c_vp)
 _sig("trs_fft16_program_info", c_int, c_size, c_size, ctypes.POINTER(c_size), c_size)
 _sig("trs_decode_matrix", c_int, c_int, c_size, c_size, c_u8p, c_vp, c_size,

@@ -169,11 +169,13 @@ int upload_coef16(DevCoef &dc, const uint16_t *rows, size_t nout, size_t k) {
 // output bit (o, r) through the accumulator layout (register v of lane half h
 // holds row 8*(v/4) + 4*h + v%4; registers 0..7 / 8..15 are the 8 bits of
 // outputs 2h / 2h+1 of the tile) and kk to input bit (i, b) = (4s + kk/8, kk%8).
-std::vector<uint8_t> build_mfma_frags(const uint8_t *rows, size_t nout, size_t k) {
+// bit(o, r, in, b) is the GF(2) matrix entry of output bit (o, r), input bit
+// (in, b).
+template <class Bit>
+std::vector<uint8_t> build_mfma_frags_bits(size_t nout, size_t k, Bit bit) {
   const size_t ks = mfma_steps(uint32_t(k));
   const size_t passes = (nout + 7) / 8;
   std::vector<uint8_t> f(passes * ks * 2 * 64 * 16, 0);
-  const Gf8 &g = gf8();
   for (size_t p = 0; p < passes; ++p)
     for (size_t s = 0; s < ks; ++s)
       for (size_t mt = 0; mt < 2; ++mt)
@@ -187,10 +189,32 @@ std::vector<uint8_t> build_mfma_frags(const uint8_t *rows, size_t nout, size_t k
           for (size_t e = 0; e < 16; ++e) {
             const size_t in = 4 * s + 2 * (l / 32) + e / 8;
             const size_t b = e % 8;
-            if (o < nout && in < k) dst[e] = (g.mul[rows[o * k + in]][1u << b] >> r) & 1u;
+            if (o < nout && in < k) dst[e] = bit(o, r, in, b);
           }
         }
   return f;
+}
+
+// GF(2^8): bit r of c_oi * 2^b.
+std::vector<uint8_t> build_mfma_frags(const uint8_t *rows, size_t nout, size_t k) {
+  const Gf8 &g = gf8();
+  return build_mfma_frags_bits(nout, k, [&](size_t o, size_t r, size_t in, size_t b) {
+    return uint8_t((g.mul[rows[o * k + in]][1u << b] >> r) & 1u);
+  });
+}
+
+// The tower over split-plane shards as a byte problem: the low and high
+// planes of tower shard i are byte inputs 2i and 2i+1 (bits 0..7 and 8..15
+// of its symbols), likewise for outputs, and multiplying by a tower constant
+// is GF(2)-linear on the 16 bits, so an nout x k tower matrix is a (2 nout) x
+// (2 k) byte problem whose bit matrix holds bit 8p + r of c_ji * x^(8q + b)
+// at output bit (2j + p, r), input bit (2i + q, b).
+std::vector<uint8_t> build_mfma_frags16(const uint16_t *rows, size_t nout, size_t k) {
+  const Gf16 &g = gf16();
+  return build_mfma_frags_bits(2 * nout, 2 * k, [&](size_t o, size_t r, size_t in, size_t b) {
+    const uint16_t prod = g.mul(rows[(o / 2) * k + in / 2], uint16_t(1u << (8 * (in % 2) + b)));
+    return uint8_t((prod >> (8 * (o % 2) + r)) & 1u);
+  });
 }
 
 using Mask = std::vector<uint64_t>;
@@ -233,7 +257,8 @@ struct Plan {
   std::vector<uint32_t> missing_parity;  // ascending
   DevCoef coef;                          // [missing data rows; missing parity rows] x k
   std::vector<uint8_t> rows8;            // host copy (GF(2^8) plans)
-  DevBuf frag;                           // MFMA A fragments of the same rows (k <= 32)
+  DevBuf frag;                           // MFMA A fragments of the same rows (GF(2^8) k <= 32,
+                                         // tower k <= 16)
   // Tower plans build their matrix rows (coef) on first use of the matrix
   // route, and the tower-FFT erasure decoder on first use of the FFT route
   // (it loads the valid shards and stores [missing data; missing parity]).
@@ -328,6 +353,20 @@ int plan_fft16(Plan &p, size_t n, const Fft16Device **out, bool byte_field = fal
 int upload_frags(DevBuf &dst, const uint8_t *rows, size_t nout, size_t k) {
   if (!mfma_available(uint32_t(k)) || nout == 0) return TRS_OK;
   const auto f = build_mfma_frags(rows, nout, k);
+  TRS_TRY(dst.alloc(f.size()));
+  TRS_TRY(hipMemcpy(dst.p, f.data(), f.size(), hipMemcpyHostToDevice));
+  return TRS_OK;
+}
+
+// Tower shapes whose byte problem fits the MFMA kernel (2k <= 32 byte inputs)
+// and the kernel arguments' pointer list (2 (k + nout) <= 256).
+bool mfma16_available(size_t k, size_t nout) {
+  return mfma_available(uint32_t(2 * k)) && 2 * (k + nout) <= size_t(kInlinePtrs);
+}
+
+int upload_frags16(DevBuf &dst, const uint16_t *rows, size_t nout, size_t k) {
+  if (!mfma16_available(k, nout) || nout == 0) return TRS_OK;
+  const auto f = build_mfma_frags16(rows, nout, k);
   TRS_TRY(dst.alloc(f.size()));
   TRS_TRY(hipMemcpy(dst.p, f.data(), f.size(), hipMemcpyHostToDevice));
   return TRS_OK;
@@ -748,8 +787,11 @@ struct trs_rs16 {
   size_t k = 0, m = 0, n = 0;
   Matrix<F16> G;
   DevCoef parity;
+  DevBuf parity_frag;  // int8-MFMA fragments of the parity rows (k <= 16)
   PlanCache plans;
   std::atomic<int> decode_route{0};  // reconstruct: 0 auto, 1 matrix, 2 tower FFT
+  // matrix applies: 0 auto, 1 the Karatsuba table kernel, 2 int8 MFMA (k <= 16)
+  std::atomic<int> apply_route{0};
   // The staged tower-FFT program (src/reedsolomon16.rs:300-308), built for
   // 2 <= k <= 1024 and k + m <= 4096.  `staged`: the encode route takes it.
   // The reference routes by a multiply-count rule measured on x86 GFNI
@@ -767,7 +809,8 @@ struct trs_rs16 {
 
 struct trs_dec16 {
   size_t k = 0, n = 0;
-  int decode_route = 0;  // the codec's decode route when the decoder was prepared
+  int decode_route = 0;  // the codec's decode and apply routes when the decoder was prepared
+  int apply_route = 0;
   std::shared_ptr<Plan> plan;
 };
 
@@ -812,8 +855,9 @@ int plan16_matrix(Plan &p, size_t k) {
       p.coef_status = TRS_SINGULAR_MATRIX;
       return;
     }
-    p.coef_status = upload_coef16(p.coef, rows.data(),
-                                  p.missing_data.size() + p.missing_parity.size(), k);
+    const size_t nrows = p.missing_data.size() + p.missing_parity.size();
+    p.coef_status = upload_coef16(p.coef, rows.data(), nrows, k);
+    if (!p.coef_status) p.coef_status = upload_frags16(p.frag, rows.data(), nrows, k);
   });
   return p.coef_status;
 }
@@ -916,6 +960,7 @@ bool use_mfma(int route, size_t k, size_t len, bool batched) {
   return route == 0 && !batched && len < kMfmaAutoMaxLen;
 }
 
+
 int run_mfma(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout, const void *frag,
              size_t len, hipStream_t stream) {
   if (nout == 0 || len == 0) return TRS_OK;
@@ -932,6 +977,21 @@ int run_mfma(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout, co
   c.nout = uint32_t(nout);
   c.vec_ok = al ? 1u : 0u;
   return hip_status(launch_gf8_mfma(c, stream));
+}
+
+// A tower apply on the int8-MFMA kernel as its byte problem: shard i's low
+// and high planes are byte shards 2i and 2i + 1 of len / 2 bytes.
+int run_mfma16(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout, const void *frag,
+               size_t len, hipStream_t stream) {
+  if (nout == 0 || len == 0) return TRS_OK;
+  if (!frag || !mfma16_available(k, nout)) return TRS_ERR_UNSUPPORTED;
+  const size_t plane = len / 2;
+  std::vector<const uint8_t *> b(2 * (k + nout));
+  for (size_t i = 0; i < k + nout; ++i) {
+    b[2 * i] = ptrs[i];
+    b[2 * i + 1] = ptrs[i] + plane;
+  }
+  return run_mfma(b, 2 * k, 2 * nout, frag, plane, stream);
 }
 
 // Split-plane shards of `len` bytes (plane = len / 2).
@@ -995,6 +1055,37 @@ enum Variant : int {
   kVarProgram64K = 6,
   kVariants = 7
 };
+
+// Where the tower's int8-MFMA byte problem beats the Karatsuba kernel on a
+// single call (per-call sweep over k in {2..16}, outputs in {1..40}, 4 KiB to
+// 8 MiB shards): the Karatsuba kernel pays ~20-30 us for its device pointer
+// table, MFMA carries the pointers in its arguments but re-reads the inputs
+// once per 4 outputs, so MFMA wins short shards and few outputs.
+bool mfma16_rule(size_t k, size_t nout, size_t len) {
+  if (len <= (size_t(256) << 10)) return true;
+  if (len <= (size_t(1) << 20)) return nout <= 8 || (nout <= 16 && k >= 8);
+  return nout <= 4 && len < (size_t(3) << 20);
+}
+
+// The automatic tower encode: the FFT program wherever it fits, except for
+// short data (k <= 16) where the sweep above puts a matrix kernel ahead: from
+// 4 MiB the Karatsuba kernel (k <= 10 or m <= 8: 1.2-5x the program), and
+// from 512 KiB to 4 MiB MFMA for m <= 4 (1.5-2x).
+int enc16_rule_variant(size_t k, size_t m, size_t len, bool program, bool frag) {
+  if (!program) return frag && mfma16_rule(k, m, len) ? kVarMfma : kVarTable;
+  if (k <= 16 && len >= (size_t(4) << 20) && (k <= 10 || m <= 8)) return kVarTable;
+  if (frag && m <= 4 && len >= (size_t(512) << 10) && len < (size_t(4) << 20)) return kVarMfma;
+  return kVarProgram64K;
+}
+
+// Tower matrix applies: kVarTable (the Karatsuba kernel) or kVarMfma (the
+// byte problem on int8 MFMA, where fragments exist).  route as
+// trs_rs16_set_apply_route: 1 and 2 force a kernel, 0 follows the rule.
+int apply16_rule(int route, size_t k, size_t nout, size_t len, bool frag) {
+  if (!frag || route == 1) return kVarTable;
+  if (route == 2) return kVarMfma;
+  return mfma16_rule(k, nout, len) ? kVarMfma : kVarTable;
+}
 constexpr size_t kTuneFeatures = 12;
 
 uint64_t variant_lds(int v) {
@@ -1782,8 +1873,9 @@ int trs_rs16_new(size_t k, size_t m, trs_rs16 **out) {
   rs->m = m;
   rs->n = k + m;
   if (!build_generator<F16>(k, k + m, rs->G)) return TRS_SINGULAR_MATRIX;
-  const int st = upload_coef16(rs->parity, rs->G.row(k), m, k);
+  int st = upload_coef16(rs->parity, rs->G.row(k), m, k);
   if (st) return st;
+  if ((st = upload_frags16(rs->parity_frag, rs->G.row(k), m, k))) return st;
   if (k >= 2 && k <= kMaxFftDataShards && k + m <= 4096) {
     rs->program = build_staged_program16(k, m);
     rs->has_program = true;
@@ -1844,6 +1936,15 @@ int run_fft16_program(const Fft16Device &d, const std::vector<const uint8_t *> &
   return launch_fft16_one(d, ptrs, nstore, len / 2, stream);
 }
 
+// The matrix encode of one stripe on the apply route's kernel (the
+// Karatsuba table kernel or int8 MFMA), without routing by the tuner.
+int matrix16_encode(const trs_rs16 *rs, const std::vector<const uint8_t *> &ptrs, size_t len,
+                    hipStream_t stream) {
+  const int v = apply16_rule(rs->apply_route.load(), rs->k, rs->m, len, rs->parity_frag.p != nullptr);
+  return v == kVarMfma ? run_mfma16(ptrs, rs->k, rs->m, rs->parity_frag.p, len, stream)
+                       : run_gf16(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, stream);
+}
+
 // RS16 encode through the rules (the FFT program at its default 64 KiB LDS
 // target whenever the shape has one) or, with a model installed, the router
 // over the matrix kernel and the program's LDS targets.
@@ -1856,13 +1957,24 @@ int encode16_ptrs(const trs_rs16 *rs, const std::vector<const uint8_t *> &ptrs, 
   }
   auto run = [&](int v) {
     if (v == kVarTable) return run_gf16(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, stream);
+    if (v == kVarMfma) return run_mfma16(ptrs, rs->k, rs->m, rs->parity_frag.p, len, stream);
     if (!d) return int(TRS_ERR_UNSUPPORTED);
     return launch_fft16_one(*d, ptrs, fft16_device_stores(*d), len / 2, stream, false, nullptr,
                             variant_lds(v));
   };
-  return router_run(RouteKey{16, 0, rs->k, rs->m, length_bucket(len)}, len,
-                    d ? kVarProgram64K : kVarTable, stream,
-                    [&] { return (1u << kVarTable) | (d ? program_variants(*d, len / 2, false) : 0u); },
+  const int apply_route = rs->apply_route.load();
+  const bool frag = rs->parity_frag.p != nullptr;
+  // a forced apply route keeps the program and forces the matrix kernel
+  const int rule = apply_route != 0 && !d ? apply16_rule(apply_route, rs->k, rs->m, len, frag)
+                   : apply_route != 0     ? kVarProgram64K
+                                          : enc16_rule_variant(rs->k, rs->m, len, d != nullptr, frag);
+  // a forced apply route (trs_rs16_set_apply_route) is not tuned
+  if (apply_route != 0) return run(rule);
+  return router_run(RouteKey{16, 0, rs->k, rs->m, length_bucket(len)}, len, rule, stream,
+                    [&] {
+                      return (1u << kVarTable) | (rs->parity_frag.p ? 1u << kVarMfma : 0u) |
+                             (d ? program_variants(*d, len / 2, false) : 0u);
+                    },
                     run);
 }
 
@@ -1976,7 +2088,7 @@ int trs_rs16_encode_sep_matrix(const trs_rs16 *rs, const uint8_t *const *data,
   ptrs.insert(ptrs.end(), parity, parity + np);
   for (auto q : ptrs)
     if (!q) return TRS_ERR_INVALID_ARGUMENT;
-  return run_gf16(ptrs, rs->k, rs->m, rs->parity, len, kWrite, nullptr, as_stream(stream));
+  return matrix16_encode(rs, ptrs, len, as_stream(stream));
 }
 
 int trs_rs16_encode_sep_fft(const trs_rs16 *rs, const uint8_t *const *data, const size_t *dl,
@@ -2204,7 +2316,9 @@ int trs_rs16_reconstruct_stripes(const trs_rs16 *rs, uint8_t *const *shards,
   std::vector<size_t> offset;
   for (Work &w : work) {
     if (!w.fft) {
-      const int st = run_gf16(w.ptrs, k, w.nout, w.plan->coef, len, kWrite, nullptr, s);
+      const int v = apply16_rule(rs->apply_route.load(), k, w.nout, len, w.plan->frag.p != nullptr);
+      const int st = v == kVarMfma ? run_mfma16(w.ptrs, k, w.nout, w.plan->frag.p, len, s)
+                                   : run_gf16(w.ptrs, k, w.nout, w.plan->coef, len, kWrite, nullptr, s);
       if (st) return st;
       continue;
     }
@@ -2256,7 +2370,7 @@ int trs_rs16_encode_stripes(const trs_rs16 *rs, uint8_t *const *shards, size_t s
   if (fft) return run_fft16(rs, ptrs, stripes, len, s);
   for (size_t i = 0; i < stripes; ++i) {
     std::vector<const uint8_t *> one(ptrs.begin() + i * rs->n, ptrs.begin() + (i + 1) * rs->n);
-    const int st = run_gf16(one, rs->k, rs->m, rs->parity, len, kWrite, nullptr, s);
+    const int st = matrix16_encode(rs, one, len, s);
     if (st) return st;
   }
   return TRS_OK;
@@ -2282,7 +2396,8 @@ int rs16_verify(const trs_rs16 *rs, const uint8_t *const *shards, const size_t *
 
 // The apply of a tower reconstruct over the plan's survivors and outputs.
 int rec16_launch(Plan &plan, size_t k, size_t total, size_t nout,
-                 const std::vector<const uint8_t *> &ptrs, size_t len, hipStream_t s, int route) {
+                 const std::vector<const uint8_t *> &ptrs, size_t len, hipStream_t s, int route,
+                 int apply_route) {
   if (use_fft_decode(route, total, k, nout)) {
     const Fft16Device *d = nullptr;
     const int st = plan_fft16(plan, total, &d);
@@ -2291,12 +2406,21 @@ int rec16_launch(Plan &plan, size_t k, size_t total, size_t nout,
   }
   const int st = plan16_matrix(plan, k);
   if (st) return st;
-  return run_gf16(ptrs, k, nout, plan.coef, len, kWrite, nullptr, s);
+  // the decode plan's rows on the Karatsuba kernel or on int8 MFMA
+  auto run = [&](int v) {
+    return v == kVarMfma ? run_mfma16(ptrs, k, nout, plan.frag.p, len, s)
+                         : run_gf16(ptrs, k, nout, plan.coef, len, kWrite, nullptr, s);
+  };
+  const int rule = apply16_rule(apply_route, k, nout, len, plan.frag.p != nullptr);
+  if (apply_route != 0) return run(rule);
+  return router_run(RouteKey{16, 1, k, nout, length_bucket(len)}, len, rule, s,
+                    [&] { return (1u << kVarTable) | (plan.frag.p ? 1u << kVarMfma : 0u); },
+                    run);
 }
 
 int reconstruct16(const trs_rs16 *rs, size_t k, size_t total, std::shared_ptr<Plan> plan,
                   uint8_t *const *shards, const size_t *lens, const uint8_t *present, size_t n,
-                  int data_only, const Exec &ex, int route) {
+                  int data_only, const Exec &ex, int route, int apply_route) {
   if (n && (!shards || !lens || !present)) return TRS_ERR_INVALID_ARGUMENT;
   if (n != total) return n < total ? TRS_TOO_FEW_SHARDS : TRS_TOO_MANY_SHARDS;
   Mask mask;
@@ -2319,7 +2443,7 @@ int reconstruct16(const trs_rs16 *rs, size_t k, size_t total, std::shared_ptr<Pl
   Plan &p = *plan;
   return dispatch(ex, ptrs, k, len,
                   [&](const std::vector<const uint8_t *> &q, size_t l, hipStream_t s) {
-                    return rec16_launch(p, k, total, nout, q, l, s, route);
+                    return rec16_launch(p, k, total, nout, q, l, s, route, apply_route);
                   });
 }
 
@@ -2336,7 +2460,7 @@ int rs16_reconstruct_rows(const trs_rs16 *rs, uint8_t *rows, size_t rows_len, si
   std::vector<size_t> lens(rs->n, row_len);
   for (size_t i = 0; i < rs->n; ++i) shards[i] = rows + i * row_len;
   return reconstruct16(rs, rs->k, rs->n, nullptr, shards.data(), lens.data(), present, rs->n, 0,
-                       ex, rs->decode_route.load());
+                       ex, rs->decode_route.load(), rs->apply_route.load());
 }
 }  // namespace
 
@@ -2385,14 +2509,21 @@ int trs_rs16_reconstruct(const trs_rs16 *rs, uint8_t *const *shards, const size_
                          const uint8_t *present, size_t n, int data_only, void *stream) {
   if (!rs) return TRS_ERR_INVALID_ARGUMENT;
   return reconstruct16(rs, rs->k, rs->n, nullptr, shards, lens, present, n, data_only,
-                       as_stream(stream), rs->decode_route.load());
+                       as_stream(stream), rs->decode_route.load(), rs->apply_route.load());
 }
 
 int trs_rs16_reconstruct_host(const trs_rs16 *rs, uint8_t *const *shards, const size_t *lens,
                               const uint8_t *present, size_t n, int data_only) {
   if (!rs) return TRS_ERR_INVALID_ARGUMENT;
   return reconstruct16(rs, rs->k, rs->n, nullptr, shards, lens, present, n, data_only,
-                       Exec::on_host(2), rs->decode_route.load());
+                       Exec::on_host(2), rs->decode_route.load(), rs->apply_route.load());
+}
+
+int trs_rs16_set_apply_route(trs_rs16 *rs, int route) {
+  if (!rs || route < 0 || route > 2) return TRS_ERR_INVALID_ARGUMENT;
+  if (route == 2 && !rs->parity_frag.p) return TRS_ERR_UNSUPPORTED;
+  rs->apply_route.store(route);
+  return TRS_OK;
 }
 
 int trs_rs16_set_decode_route(trs_rs16 *rs, int route) {
@@ -2430,6 +2561,7 @@ int trs_rs16_prepare_decode(const trs_rs16 *rs, const uint8_t *present, size_t n
   d->k = rs->k;
   d->n = rs->n;
   d->decode_route = rs->decode_route.load();
+  d->apply_route = rs->apply_route.load();
   d->plan = plan;
   *out = d.release();
   return TRS_OK;
@@ -2439,14 +2571,14 @@ int trs_dec16_reconstruct(const trs_dec16 *dec, uint8_t *const *shards, const si
                           const uint8_t *present, size_t n, int data_only, void *stream) {
   if (!dec) return TRS_ERR_INVALID_ARGUMENT;
   return reconstruct16(nullptr, dec->k, dec->n, dec->plan, shards, lens, present, n, data_only,
-                       as_stream(stream), dec->decode_route);
+                       as_stream(stream), dec->decode_route, dec->apply_route);
 }
 
 int trs_dec16_reconstruct_host(const trs_dec16 *dec, uint8_t *const *shards, const size_t *lens,
                                const uint8_t *present, size_t n, int data_only) {
   if (!dec) return TRS_ERR_INVALID_ARGUMENT;
   return reconstruct16(nullptr, dec->k, dec->n, dec->plan, shards, lens, present, n, data_only,
-                       Exec::on_host(2), dec->decode_route);
+                       Exec::on_host(2), dec->decode_route, dec->apply_route);
 }
 
 void trs_dec16_free(trs_dec16 *dec) { delete dec; }
@@ -2918,6 +3050,7 @@ int tune_feasible(TuneSpace &T, int field, uint32_t k, uint32_t nout, uint64_t l
                   uint32_t *mask) {
   if (field == 16) {
     *mask = 1u << kVarTable;
+    if (mfma16_available(k, nout)) *mask |= 1u << kVarMfma;
   } else {
     *mask = matrix_variants8(k, nout);
     if (bs_encode_available(kRouteBsFft, k, nout) || bs_encode_available(kRouteBsMatrix, k, nout)) {
@@ -2965,8 +3098,9 @@ extern "C" int trs_tune_rule(int field, uint32_t k, uint32_t nout, uint64_t len,
   int st = tune_feasible(T, field, k, nout, len, &mask);
   if (st) return st;
   int v;
-  if (field == 16) {  // the FFT program whenever it fits, at its 64 KiB default
-    v = (mask >> kVarTable) == 1u ? kVarTable : kVarProgram64K;
+  if (field == 16) {  // the codec's automatic tower encode
+    const bool program = (mask >> kVarProgram16K) != 0;
+    v = enc16_rule_variant(k, nout, len, program, ((mask >> kVarMfma) & 1u) != 0);
   } else if (bs_encode_available(kRouteBsFft, k, nout) ||
              bs_encode_available(kRouteBsMatrix, k, nout) || len <= staged8_gpu_max(k, nout)) {
     v = kVarProgram32K;
@@ -3032,7 +3166,8 @@ extern "C" int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len
   } else if (!prog) {
     std::vector<uint16_t> rows16(size_t(k) * nout);
     for (auto &c : rows16) c = uint16_t(1 + tune_rng(seed) % 65535);
-    st = upload_coef16(T.coef, rows16.data(), nout, k);
+    st = variant == kVarMfma ? upload_frags16(T.frag, rows16.data(), nout, k)
+                             : upload_coef16(T.coef, rows16.data(), nout, k);
   }
   if (st) return st;
   uint8_t *base = static_cast<uint8_t *>(T.data.p);
@@ -3043,7 +3178,9 @@ extern "C" int trs_tune_apply(int field, uint32_t k, uint32_t nout, uint64_t len
       return launch_fft16_one(*prog16, ptrs, nout, field == 16 ? len / 2 : len, T.stream,
                               field == 8, nullptr, variant_lds(variant));
     }
-    if (field == 16) return run_gf16(ptrs, k, nout, T.coef, len, kWrite, nullptr, T.stream);
+    if (field == 16)
+      return variant == kVarMfma ? run_mfma16(ptrs, k, nout, T.frag.p, len, T.stream)
+                                 : run_gf16(ptrs, k, nout, T.coef, len, kWrite, nullptr, T.stream);
     if (variant == kVarMfma) return run_mfma(ptrs, k, nout, T.frag.p, len, T.stream);
     if (bs_route) {
       Gf8Call c{};
@@ -3779,6 +3916,7 @@ int trs_rs16_clone(const trs_rs16 *rs, trs_rs16 **out) {
   trs_rs16 *c = nullptr;
   const int st = trs_rs16_new(rs->k, rs->m, &c);
   if (st) return st;
+  c->apply_route.store(rs->apply_route.load());
   c->decode_route.store(rs->decode_route.load());
   {
     std::shared_lock<std::shared_mutex> lk(rs->plans.mu);
