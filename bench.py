@@ -658,7 +658,9 @@ def mfma_run(trs, torch, stripes: int = 1024) -> int:
     """The int8-MFMA kernels beside the bit-sliced planes kernel on the same
     work, for the MFMA counter runs (SQ_INSTS_VALU_MFMA_MOPS_I8,
     SQ_VALU_MFMA_BUSY_CYCLES): per call, reconstructs of 3 data shards of RS8
-    (10,4) and of 8 of (32,32) at 256 KiB (k_gf8_mfma against k_gf8_planes);
+    (10,4) and of 8 of (32,32) at 256 KiB (k_gf8_mfma against k_gf8_planes),
+    and tower (10,4) and (16,16) encodes at 1 MiB (k_gf8_mfma on the byte
+    problem against k_gf16_apply_inline);
     batched, the mixed reconstruct work list of the first `stripes` stripes of
     the default workload (k_batch_mfma against k_batch8_planes).  Prints the
     HIP-event time and algorithmic bytes of each."""
@@ -689,6 +691,20 @@ def mfma_run(trs, torch, stripes: int = 1024) -> int:
             ms = timed(lambda: rs.reconstruct(pairs), 50)
             out[f"percall_{k}_{m}_{route}"] = {"ms": round(ms, 4), "bytes": (k + e) * L,
                                                "gbs": round((k + e) * L / ms / 1e6, 1)}
+        del buf
+    # the tower: split-plane (10,4) and (16,16) encodes at 1 MiB on int8 MFMA
+    # (the 16m x 16k bit expansion) against the Karatsuba kernel
+    Lt = 1 << 20
+    for (k, m) in ((10, 4), (16, 16)):
+        rs = trs.ReedSolomon16(k, m)
+        buf = torch.empty((k + m, Lt), dtype=torch.uint8, device="cuda")
+        trs.fill_random(buf[:k].reshape(-1), SEED ^ (k << 8))
+        data, parity = [buf[i] for i in range(k)], [buf[k + j] for j in range(m)]
+        for route in ("mfma", "table"):
+            rs.set_apply_route(route)
+            ms = timed(lambda: rs.encode_sep_matrix(data, parity), 50)
+            out[f"tower_percall_{k}_{m}_{route}"] = {"ms": round(ms, 4), "bytes": (k + m) * Lt,
+                                                     "gbs": round((k + m) * Lt / ms / 1e6, 1)}
         del buf
     meta = build_workload(stripes, np.random.default_rng(SEED))
     sizes = [(k + m) * Ls for (k, m, Ls, _) in meta]

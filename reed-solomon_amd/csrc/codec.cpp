@@ -1002,6 +1002,24 @@ int run_gf16(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout,
   const size_t plane = len / 2;
   bool al = (plane % 16) == 0;
   for (auto q : ptrs) al = al && aligned16(q);
+  auto fill = [&](Gf16Call &c) {
+    c.coef = coef.ptr();
+    c.plane = plane;
+    c.col0 = 0;
+    c.cols = plane;
+    c.k = uint32_t(k);
+    c.nout = uint32_t(nout);
+    c.ostride = coef.ostride;
+    c.vec_ok = al ? 1u : 0u;
+    c.mode = mode;
+    c.flag = flag;
+  };
+  if (k + nout <= size_t(kInlinePtrs)) {  // pointers in the kernel arguments
+    Gf16InlineCall c{};
+    fill(c.call);
+    for (size_t i = 0; i < k + nout; ++i) c.ptr[i] = ptrs[i];
+    return hip_status(launch_gf16_apply_inline(c, stream));
+  }
   PtrTable table;
   int st = table.upload(ptrs, stream);
   if (st) return st;
@@ -1057,25 +1075,36 @@ enum Variant : int {
 };
 
 // Where the tower's int8-MFMA byte problem beats the Karatsuba kernel on a
-// single call (per-call sweep over k in {2..16}, outputs in {1..40}, 4 KiB to
-// 8 MiB shards): the Karatsuba kernel pays ~20-30 us for its device pointer
-// table, MFMA carries the pointers in its arguments but re-reads the inputs
-// once per 4 outputs, so MFMA wins short shards and few outputs.
+// single call (per-call sweep over k in {2, 4, 8, 10, 16}, outputs in {1, 2,
+// 4, 8, 16, 40}, 4 KiB to 8 MiB shards).  Both carry their shard pointers in
+// the kernel arguments; MFMA re-reads the inputs once per 4 outputs, the
+// Karatsuba kernel streams them once per 4 or 8 outputs at a higher rate, so
+// MFMA wins short shards (except the tiniest shapes, a tie) and, up to 2 MiB,
+// few outputs over enough inputs.
 bool mfma16_rule(size_t k, size_t nout, size_t len) {
-  if (len <= (size_t(256) << 10)) return true;
-  if (len <= (size_t(1) << 20)) return nout <= 8 || (nout <= 16 && k >= 8);
-  return nout <= 4 && len < (size_t(3) << 20);
+  if (len <= (size_t(256) << 10)) return k >= 4 || nout >= 8;
+  if (len <= (size_t(1) << 20)) return k >= 4 && nout <= 8;
+  if (len < (size_t(2) << 20)) return k >= 8 && nout <= 4;
+  return false;
 }
 
-// The automatic tower encode: the FFT program wherever it fits, except for
-// short data (k <= 16) where the sweep above puts a matrix kernel ahead: from
-// 4 MiB the Karatsuba kernel (k <= 10 or m <= 8: 1.2-5x the program), and
-// from 512 KiB to 4 MiB MFMA for m <= 4 (1.5-2x).
+// The automatic tower encode.  Past 16 data shards, the FFT program wherever
+// it fits (it won 1767 of 1774 sampled tower shapes).  For k <= 16 the same
+// sweep puts the program ahead only for power-of-two k with enough parity
+// (short shards: k/4 <= m < 2k; from 512 KiB: k >= 8 and 8 <= m <= 2.5k;
+// from 4 MiB: k = 16 and m > 8); elsewhere the matrix kernel of mfma16_rule
+// wins, by up to 5x at long shards.  Mean regret against the sweep's best:
+// 1 % for this rule, 114 % for the program everywhere.
 int enc16_rule_variant(size_t k, size_t m, size_t len, bool program, bool frag) {
-  if (!program) return frag && mfma16_rule(k, m, len) ? kVarMfma : kVarTable;
-  if (k <= 16 && len >= (size_t(4) << 20) && (k <= 10 || m <= 8)) return kVarTable;
-  if (frag && m <= 4 && len >= (size_t(512) << 10) && len < (size_t(4) << 20)) return kVarMfma;
-  return kVarProgram64K;
+  const int mat = frag && mfma16_rule(k, m, len) ? kVarMfma : kVarTable;
+  if (!program) return mat;
+  if (k > 16) return kVarProgram64K;
+  const bool pow2 = (k & (k - 1)) == 0;
+  if (len >= (size_t(4) << 20)) return k <= 10 || m <= 8 ? kVarTable : kVarProgram64K;
+  if (len >= (size_t(512) << 10))
+    return pow2 && k >= 8 && m >= 8 && 2 * m <= 5 * k ? kVarProgram64K : mat;
+  if (mat != kVarMfma || (pow2 && 4 * m >= k && m < 2 * k)) return kVarProgram64K;
+  return mat;
 }
 
 // Tower matrix applies: kVarTable (the Karatsuba kernel) or kVarMfma (the

@@ -372,9 +372,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch8_planes(const BatchJob 
 // (a_hi, a_lo), t0 = c_lo*a_lo, t1 = (c_hi^c_lo)*(a_hi^a_lo), t2 = (c_hi*L)*a_hi,
 // out_lo ^= t0^t2, out_hi ^= t0^t1 (src/gf/tower_x86.rs:80-93).  The three
 // product sums are accumulated separately and combined at the end.
-template <int NT, uint32_t MODE>
-__device__ __forceinline__ void gf16_body(const Gf16Call &a, uint64_t off, uint64_t rem,
-                                          bool full) {
+template <int NT, uint32_t MODE, typename PtrAt>
+__device__ __forceinline__ void gf16_body(const Gf16Call &a, PtrAt ptr_at, uint64_t off,
+                                          uint64_t rem, bool full) {
   const uint32_t jbase = blockIdx.y * NT;
   const uint32_t nt = min(uint32_t(NT), a.nout - jbase);
   const size_t csz = (size_t)a.k * a.ostride / 4;
@@ -385,7 +385,7 @@ __device__ __forceinline__ void gf16_body(const Gf16Call &a, uint64_t off, uint6
     for (int w = 0; w < 4; ++w) A[j][w] = B[j][w] = C[j][w] = 0;
 
   for (uint32_t i = 0; i < a.k; ++i) {
-    const uint8_t *s = ld_const_ptr(a.ptrs + i);
+    const uint8_t *s = ptr_at(i);
     uint32_t lo[4], hi[4], sum[4];
     load_lane(s + off, full, rem, lo);
     load_lane(s + a.plane + off, full, rem, hi);
@@ -401,7 +401,7 @@ __device__ __forceinline__ void gf16_body(const Gf16Call &a, uint64_t off, uint6
 #pragma unroll
   for (int j = 0; j < NT; ++j) {
     if (uint32_t(j) < nt) {
-      uint8_t *o = const_cast<uint8_t *>(ld_const_ptr(a.ptrs + a.k + jbase + j));
+      uint8_t *o = const_cast<uint8_t *>(ptr_at(a.k + jbase + j));
       uint32_t olo[4], ohi[4];
 #pragma unroll
       for (int w = 0; w < 4; ++w) {
@@ -425,15 +425,25 @@ __device__ __forceinline__ void gf16_body(const Gf16Call &a, uint64_t off, uint6
 
 // The full / tail choice is the same for every lane of a workgroup (as in
 // gf8_tile).
-template <int NT, uint32_t MODE>
-__global__ __launch_bounds__(kBlockThreads) void k_gf16_apply(const Gf16Call a) {
+template <int NT, uint32_t MODE, typename PtrAt>
+__device__ __forceinline__ void gf16_tile(const Gf16Call &a, PtrAt ptr_at) {
   const uint64_t tile = uint64_t(blockIdx.x) * kChunkBytes;
   const uint64_t rel = tile + uint64_t(threadIdx.x) * kLaneBytes;
   if (rel >= a.cols) return;
   if (a.vec_ok && ((a.cols & 15) == 0 || tile + kChunkBytes <= a.cols))
-    gf16_body<NT, MODE>(a, a.col0 + rel, a.cols - rel, true);
+    gf16_body<NT, MODE>(a, ptr_at, a.col0 + rel, a.cols - rel, true);
   else
-    gf16_body<NT, MODE>(a, a.col0 + rel, a.cols - rel, false);
+    gf16_body<NT, MODE>(a, ptr_at, a.col0 + rel, a.cols - rel, false);
+}
+
+template <int NT, uint32_t MODE>
+__global__ __launch_bounds__(kBlockThreads) void k_gf16_apply(const Gf16Call a) {
+  gf16_tile<NT, MODE>(a, [&](uint32_t i) { return ld_const_ptr(a.ptrs + i); });
+}
+
+template <int NT, uint32_t MODE>
+__global__ __launch_bounds__(kBlockThreads) void k_gf16_apply_inline(const Gf16InlineCall a) {
+  gf16_tile<NT, MODE>(a.call, [&](uint32_t i) { return uni(a.ptr[i]); });
 }
 
 // Wire-format conversion between interleaved u16 symbols (symbol j in bytes
@@ -534,15 +544,41 @@ hipError_t launch_gf8_apply(const Gf8Call &call, uint32_t mode, hipStream_t stre
   return hipGetLastError();
 }
 
-hipError_t launch_gf16_apply(const Gf16Call &call, hipStream_t stream) {
-  if (call.cols == 0 || call.nout == 0) return hipSuccess;
+namespace {
+dim3 gf16_grid(const Gf16Call &call, int &nt) {
   const uint64_t lanes = (call.cols + kLaneBytes - 1) / kLaneBytes;
   const uint32_t gx = uint32_t((lanes + kBlockThreads - 1) / kBlockThreads);
-  const int nt = call.nout <= 4 ? 4 : 8;
-  const dim3 grid(gx, (call.nout + nt - 1) / nt);
+  nt = call.nout <= 4 ? 4 : 8;
+  return dim3(gx, (call.nout + nt - 1) / nt);
+}
+}  // namespace
+
+hipError_t launch_gf16_apply(const Gf16Call &call, hipStream_t stream) {
+  if (call.cols == 0 || call.nout == 0) return hipSuccess;
+  int nt = 4;
+  const dim3 grid = gf16_grid(call, nt);
   const dim3 block(kBlockThreads);
 #define TRS_GF16(NT, MODE) hipLaunchKernelGGL((k_gf16_apply<NT, MODE>), grid, block, 0, stream, call)
   if (call.mode == kVerify) {
+    if (nt == 4) TRS_GF16(4, kVerify);
+    else TRS_GF16(8, kVerify);
+  } else {
+    if (nt == 4) TRS_GF16(4, kWrite);
+    else TRS_GF16(8, kWrite);
+  }
+#undef TRS_GF16
+  return hipGetLastError();
+}
+
+hipError_t launch_gf16_apply_inline(const Gf16InlineCall &call, hipStream_t stream) {
+  if (call.call.cols == 0 || call.call.nout == 0) return hipSuccess;
+  if (call.call.k + call.call.nout > uint32_t(kInlinePtrs)) return hipErrorInvalidValue;
+  int nt = 4;
+  const dim3 grid = gf16_grid(call.call, nt);
+  const dim3 block(kBlockThreads);
+#define TRS_GF16(NT, MODE) \
+  hipLaunchKernelGGL((k_gf16_apply_inline<NT, MODE>), grid, block, 0, stream, call)
+  if (call.call.mode == kVerify) {
     if (nt == 4) TRS_GF16(4, kVerify);
     else TRS_GF16(8, kVerify);
   } else {

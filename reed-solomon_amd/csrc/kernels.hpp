@@ -53,6 +53,13 @@ struct Gf16Call {
   uint32_t *flag;
 };
 
+// The same apply with up to kInlinePtrs shard pointers in the kernel
+// arguments (call.ptrs unused), so a call needs no device pointer table.
+struct Gf16InlineCall {
+  Gf16Call call;
+  const uint8_t *ptr[kInlinePtrs];  // [0,k) inputs, [k,k+nout) outputs
+};
+
 // One GF(2^8) apply on int8 MFMA (mfma.hip): frag holds the bit-expanded
 // coefficient matrix as per-lane A fragments, [pass][K step][2 M tiles][64
 // lanes][16 bytes]; pass p covers outputs 8p..8p+7, K step s inputs 4s..4s+3.
@@ -83,6 +90,7 @@ hipError_t launch_gf8_apply(const Gf8Call &call, uint32_t mode, hipStream_t stre
 // or kPlanesOut, 0 = the launch rule below).
 hipError_t launch_gf8_planes(const Gf8Call &call, uint32_t mode, hipStream_t stream, int nt = 0);
 hipError_t launch_gf16_apply(const Gf16Call &call, hipStream_t stream);
+hipError_t launch_gf16_apply_inline(const Gf16InlineCall &call, hipStream_t stream);
 // Batched persistent kernels: tile_job[t] is the job index of tile t.
 hipError_t launch_batch8(const BatchJob *jobs, const uint32_t *tile_job, uint64_t total_tiles,
                          uint32_t grid, hipStream_t stream);
