@@ -120,18 +120,55 @@ size_t pad_rows(size_t nout) {
 
 // Coefficients in the kernels' layout: for each plane, byte (i*ostride + j)
 // is output row j's coefficient for input i; rows past nout are zero.
+// GF(2^8) coefficients are followed by the bit-sliced kernels' form of the
+// same rows (planes_ptr): each row j scaled by 2^t_j, then the ostride bytes
+// t_j (int8).
 struct DevCoef {
   DevBuf buf;
   uint32_t k = 0, nout = 0, ostride = 0;
+  bool planes = false;
   const uint32_t *ptr() const { return static_cast<const uint32_t *>(buf.p); }
+  const uint32_t *planes_ptr() const {
+    return planes ? ptr() + size_t(k) * ostride / 4 : nullptr;
+  }
 };
+
+// The bit-sliced kernels XOR x*2^b into an output for every set bit b of its
+// coefficient, so their cost per coefficient is 8 XORs per set bit.  Scaling
+// output row j by 2^t (t in [-7, 7]) and multiplying the result by 2^-t once,
+// at 3 XORs per step in plane form, changes the bits but not the bytes
+// written; t_j minimises 8 * (set bits of the scaled row) + 3 |t|.
+int8_t planes_scale(const uint8_t *row, size_t k, size_t stride) {
+  const Gf8 &g = gf8();
+  int best_t = 0;
+  long best = -1;
+  for (int t = -7; t <= 7; ++t) {
+    const uint8_t s = g.exp[(t + 255) % 255];
+    long cost = 3L * (t < 0 ? -t : t);
+    for (size_t i = 0; i < k; ++i) cost += 8L * __builtin_popcount(g.mul[row[i * stride]][s]);
+    if (best < 0 || cost < best) {
+      best = cost;
+      best_t = t;
+    }
+  }
+  return int8_t(best_t);
+}
 
 // rows: nout x k GF(2^8) coefficients, row-major.
 int upload_coef8(DevCoef &dc, const uint8_t *rows, size_t nout, size_t k) {
   const size_t os = pad_rows(nout);
-  std::vector<uint8_t> t(k * os, 0);
+  std::vector<uint8_t> t(2 * k * os + os, 0);
   for (size_t j = 0; j < nout; ++j)
     for (size_t i = 0; i < k; ++i) t[i * os + j] = rows[j * k + i];
+  const Gf8 &g = gf8();
+  uint8_t *scaled = t.data() + k * os;
+  for (size_t j = 0; j < nout; ++j) {
+    const int8_t sc = planes_scale(rows + j * k, k, 1);
+    const uint8_t s = g.exp[(sc + 255) % 255];
+    for (size_t i = 0; i < k; ++i) scaled[i * os + j] = g.mul[rows[j * k + i]][s];
+    scaled[k * os + j] = uint8_t(sc);
+  }
+  dc.planes = true;
   TRS_TRY(dc.buf.alloc(t.size()));
   TRS_TRY(hipMemcpy(dc.buf.p, t.data(), t.size(), hipMemcpyHostToDevice));
   dc.k = uint32_t(k);
@@ -155,6 +192,7 @@ int upload_coef16(DevCoef &dc, const uint16_t *rows, size_t nout, size_t k) {
       t[plane + i * os + j] = hi ^ lo;
       t[2 * plane + i * os + j] = f.mul[hi][kTowerL];
     }
+  dc.planes = false;
   TRS_TRY(dc.buf.alloc(t.size()));
   TRS_TRY(hipMemcpy(dc.buf.p, t.data(), t.size(), hipMemcpyHostToDevice));
   dc.k = uint32_t(k);
@@ -939,9 +977,12 @@ int run_gf8(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout, con
   c.ostride = coef.ostride;
   c.vec_ok = al ? 1u : 0u;
   c.flag = flag;
-  if (variant == 1) return hip_status(launch_gf8_planes(c, mode, stream, 4));
-  if (variant == 2) return hip_status(launch_gf8_planes(c, mode, stream, kPlanesOut));
-  if (variant < 0 && len >= planes_min_len()) return hip_status(launch_gf8_planes(c, mode, stream));
+  const bool planes = variant == 1 || variant == 2 || (variant < 0 && len >= planes_min_len());
+  if (planes && coef.planes) {  // the bit-sliced tile takes the scaled rows
+    c.coef = coef.planes_ptr();
+    return hip_status(
+        launch_gf8_planes(c, mode, stream, variant == 1 ? 4 : variant == 2 ? kPlanesOut : 0));
+  }
   return hip_status(launch_gf8_apply(c, mode, stream));
 }
 
@@ -2819,13 +2860,16 @@ int trs_batch8_prepare(trs_batch8 *b) {
       fjobs.push_back({r.fft, J.ptrs, r.len, r.nout, al ? 1u : 0u});
       continue;
     }
-    J.coef = r.coef->ptr();
+    const bool planes_job = b->planes && !r.frag && !r.prog;
+    if (planes_job && !r.coef->planes) return TRS_ERR_INVALID_ARGUMENT;
+    // the bit-sliced tile takes the scaled rows (and their exponents)
+    J.coef = planes_job ? r.coef->planes_ptr() : r.coef->ptr();
     J.len = r.len;
     J.k = r.k;
     J.nout = r.nout;
     J.ostride = r.coef->ostride;
     J.vec_ok = al ? 1u : 0u;
-    const uint32_t rows_per_tile = b->planes && !r.frag && !r.prog ? kPlanesOut : kOutStride;
+    const uint32_t rows_per_tile = planes_job ? kPlanesOut : kOutStride;
     J.out_tiles = (r.nout + rows_per_tile - 1) / rows_per_tile;
     J.prog = r.prog;
     if (r.frag) {

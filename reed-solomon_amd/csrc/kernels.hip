@@ -208,6 +208,50 @@ __device__ __forceinline__ void planes_times2(uint32_t (&y)[8]) {
   y[0] = t;
 }
 
+// y / 2: the inverse of planes_times2.
+__device__ __forceinline__ void planes_half(uint32_t (&y)[8]) {
+  const uint32_t t = y[0];
+  y[0] = y[1];
+  y[1] = y[2] ^ t;
+  y[2] = y[3] ^ t;
+  y[3] = y[4] ^ t;
+  y[4] = y[5];
+  y[5] = y[6];
+  y[6] = y[7];
+  y[7] = t;
+}
+
+template <int S>
+__device__ __forceinline__ void planes_steps(uint32_t (&y)[8], bool half) {
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (half) planes_half(y);
+    else planes_times2(y);
+  }
+}
+
+// y *= 2^-t for a wave-uniform t in [-7, 7]: one straight-line sequence of
+// |t| steps per value, so the plane renaming costs no moves inside a case.
+__device__ __forceinline__ void planes_unscale(uint32_t (&y)[8], int t) {
+  switch (t) {
+    case 1: planes_steps<1>(y, true); break;
+    case 2: planes_steps<2>(y, true); break;
+    case 3: planes_steps<3>(y, true); break;
+    case 4: planes_steps<4>(y, true); break;
+    case 5: planes_steps<5>(y, true); break;
+    case 6: planes_steps<6>(y, true); break;
+    case 7: planes_steps<7>(y, true); break;
+    case -1: planes_steps<1>(y, false); break;
+    case -2: planes_steps<2>(y, false); break;
+    case -3: planes_steps<3>(y, false); break;
+    case -4: planes_steps<4>(y, false); break;
+    case -5: planes_steps<5>(y, false); break;
+    case -6: planes_steps<6>(y, false); break;
+    case -7: planes_steps<7>(y, false); break;
+    default: break;
+  }
+}
+
 // acc[j] ^= c_j * y for the NT coefficients at cw (same layout as mac_rows);
 // y is consumed (left at y * 2^7).
 template <int NT>
@@ -231,7 +275,9 @@ __device__ __forceinline__ void mac_planes(uint32_t (&acc)[NT][8], uint32_t (&y)
 // One lane's share of a bit-sliced apply tile: outputs jbase..jbase+nt over
 // the lane's split 32 bytes whose first piece is at `off` (an active lane:
 // off < len).  The next input's bytes are loaded before the current input's
-// multiply-accumulate.
+// multiply-accumulate.  coef holds the rows scaled by 2^t_j and, after the
+// k * ostride coefficient bytes, the ostride exponents t_j (int8): each
+// output is multiplied by 2^-t_j before it is stored.
 template <int NT, uint32_t MODE, typename PtrAt>
 __device__ __forceinline__ void gf8_planes_body(PtrAt ptr_at, const uint32_t *coef, uint64_t len,
                                                 int mode, uint32_t k, uint32_t nout,
@@ -253,6 +299,14 @@ __device__ __forceinline__ void gf8_planes_body(PtrAt ptr_at, const uint32_t *co
     if (i + 1 < k) load_split_raw(ptr_at(i + 1), off, len, mode, nxt);
     bytes_to_planes(y);
     mac_planes<NT>(acc, y, coef + ((size_t)i * ostride + jbase) / 4);
+  }
+  {
+    uint32_t tw[NT / 4];
+#pragma unroll
+    for (int q = 0; q < NT / 4; ++q) tw[q] = ld_const(coef + ((size_t)k * ostride + jbase) / 4 + q);
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+      if (uint32_t(j) < nt) planes_unscale(acc[j], int(int8_t(tw[j >> 2] >> (8 * (j & 3)))));
   }
 
   uint32_t bad = 0;
