@@ -17,40 +17,53 @@
 
 namespace trs {
 
+// Exchanges the bits of b under mask with the bits of a under ~mask, shifted
+// by n (~mask == mask << n for the three masks used here).  SEL: two shifts
+// and two selects (S0 & S2) | (S1 & ~S2) instead of two shifts and three
+// XOR / AND operations; one more live register during the swap.
+template <bool SEL = false>
 __device__ __forceinline__ void swap_bits(uint32_t &a, uint32_t &b, uint32_t mask, int n) {
-  const uint32_t t = ((a >> n) ^ b) & mask;
-  b ^= t;
-  a ^= t << n;
+  if constexpr (SEL) {
+    const uint32_t na = __builtin_amdgcn_bitop3_b32(a, b << n, mask, 0xe4);
+    b = __builtin_amdgcn_bitop3_b32(a >> n, b, mask, 0xe4);
+    a = na;
+  } else {
+    const uint32_t t = ((a >> n) ^ b) & mask;
+    b ^= t;
+    a ^= t << n;
+  }
 }
 
+template <bool SEL = false>
 __device__ __forceinline__ void bytes_to_planes(uint32_t (&w)[8]) {
-  swap_bits(w[0], w[1], 0x55555555u, 1);
-  swap_bits(w[2], w[3], 0x55555555u, 1);
-  swap_bits(w[4], w[5], 0x55555555u, 1);
-  swap_bits(w[6], w[7], 0x55555555u, 1);
-  swap_bits(w[0], w[2], 0x33333333u, 2);
-  swap_bits(w[1], w[3], 0x33333333u, 2);
-  swap_bits(w[4], w[6], 0x33333333u, 2);
-  swap_bits(w[5], w[7], 0x33333333u, 2);
-  swap_bits(w[0], w[4], 0x0f0f0f0fu, 4);
-  swap_bits(w[1], w[5], 0x0f0f0f0fu, 4);
-  swap_bits(w[2], w[6], 0x0f0f0f0fu, 4);
-  swap_bits(w[3], w[7], 0x0f0f0f0fu, 4);
+  swap_bits<SEL>(w[0], w[1], 0x55555555u, 1);
+  swap_bits<SEL>(w[2], w[3], 0x55555555u, 1);
+  swap_bits<SEL>(w[4], w[5], 0x55555555u, 1);
+  swap_bits<SEL>(w[6], w[7], 0x55555555u, 1);
+  swap_bits<SEL>(w[0], w[2], 0x33333333u, 2);
+  swap_bits<SEL>(w[1], w[3], 0x33333333u, 2);
+  swap_bits<SEL>(w[4], w[6], 0x33333333u, 2);
+  swap_bits<SEL>(w[5], w[7], 0x33333333u, 2);
+  swap_bits<SEL>(w[0], w[4], 0x0f0f0f0fu, 4);
+  swap_bits<SEL>(w[1], w[5], 0x0f0f0f0fu, 4);
+  swap_bits<SEL>(w[2], w[6], 0x0f0f0f0fu, 4);
+  swap_bits<SEL>(w[3], w[7], 0x0f0f0f0fu, 4);
 }
 
+template <bool SEL = false>
 __device__ __forceinline__ void planes_to_bytes(uint32_t (&w)[8]) {
-  swap_bits(w[0], w[4], 0x0f0f0f0fu, 4);
-  swap_bits(w[1], w[5], 0x0f0f0f0fu, 4);
-  swap_bits(w[2], w[6], 0x0f0f0f0fu, 4);
-  swap_bits(w[3], w[7], 0x0f0f0f0fu, 4);
-  swap_bits(w[0], w[2], 0x33333333u, 2);
-  swap_bits(w[1], w[3], 0x33333333u, 2);
-  swap_bits(w[4], w[6], 0x33333333u, 2);
-  swap_bits(w[5], w[7], 0x33333333u, 2);
-  swap_bits(w[0], w[1], 0x55555555u, 1);
-  swap_bits(w[2], w[3], 0x55555555u, 1);
-  swap_bits(w[4], w[5], 0x55555555u, 1);
-  swap_bits(w[6], w[7], 0x55555555u, 1);
+  swap_bits<SEL>(w[0], w[4], 0x0f0f0f0fu, 4);
+  swap_bits<SEL>(w[1], w[5], 0x0f0f0f0fu, 4);
+  swap_bits<SEL>(w[2], w[6], 0x0f0f0f0fu, 4);
+  swap_bits<SEL>(w[3], w[7], 0x0f0f0f0fu, 4);
+  swap_bits<SEL>(w[0], w[2], 0x33333333u, 2);
+  swap_bits<SEL>(w[1], w[3], 0x33333333u, 2);
+  swap_bits<SEL>(w[4], w[6], 0x33333333u, 2);
+  swap_bits<SEL>(w[5], w[7], 0x33333333u, 2);
+  swap_bits<SEL>(w[0], w[1], 0x55555555u, 1);
+  swap_bits<SEL>(w[2], w[3], 0x55555555u, 1);
+  swap_bits<SEL>(w[4], w[5], 0x55555555u, 1);
+  swap_bits<SEL>(w[6], w[7], 0x55555555u, 1);
 }
 
 // 32 bytes at p into planes; `full` lanes use two 16-byte loads (non-temporal
@@ -178,11 +191,11 @@ __device__ __forceinline__ void load_split_raw(const uint8_t *base, uint64_t off
   }
 }
 
-template <bool NT = false>
+template <bool NT = false, bool SEL = false>
 __device__ __forceinline__ void load_split(const uint8_t *base, uint64_t off, uint64_t len,
                                            int mode, uint32_t (&w)[8]) {
   load_split_raw<NT>(base, off, len, mode, w);
-  bytes_to_planes(w);
+  bytes_to_planes<SEL>(w);
 }
 
 // Verify epilogue of a program: instead of storing `planes`, compare them
@@ -202,13 +215,13 @@ __device__ __forceinline__ void verify_split(const uint8_t *base, uint64_t off, 
   for (int i = 0; i < 8; ++i) bad |= w[i] ^ x[i];
 }
 
-template <bool NT = false>
+template <bool NT = false, bool SEL = false>
 __device__ __forceinline__ void store_split(uint8_t *base, uint64_t off, uint64_t len, int mode,
                                             const uint32_t (&planes)[8]) {
   uint32_t w[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) w[i] = planes[i];
-  planes_to_bytes(w);
+  planes_to_bytes<SEL>(w);
   uint8_t *p = base + off;
   if (mode == kSplitFull || mode == kSplitMasked) {
     st16<NT>(p, make_uint4(w[0], w[1], w[2], w[3]));

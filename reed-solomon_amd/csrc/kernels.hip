@@ -272,13 +272,95 @@ __device__ __forceinline__ void mac_planes(uint32_t (&acc)[NT][8], uint32_t (&y)
   }
 }
 
+// acc ^= the XOR of the powers p[b] over the set bits b of the 4-bit N, two
+// terms per three-input XOR: ceil(popcount(N) / 2) operations per plane.
+template <int N>
+__device__ __forceinline__ void nib_xor(uint32_t (&acc)[8], const uint32_t (&p0)[8],
+                                        const uint32_t (&p1)[8], const uint32_t (&p2)[8],
+                                        const uint32_t (&p3)[8]) {
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    uint32_t t[4];
+    int n = 0;
+    if (N & 1) t[n++] = p0[r];
+    if (N & 2) t[n++] = p1[r];
+    if (N & 4) t[n++] = p2[r];
+    if (N & 8) t[n++] = p3[r];
+    uint32_t v = acc[r];
+    if (n >= 2) v = __builtin_amdgcn_bitop3_b32(v, t[0], t[1], 0x96);
+    else if (n == 1) v ^= t[0];
+    if (n == 4) v = __builtin_amdgcn_bitop3_b32(v, t[2], t[3], 0x96);
+    else if (n == 3) v ^= t[2];
+    acc[r] = v;
+  }
+}
+
+// The 16-way choice on the wave-uniform nibble at bit s of w as a tree of
+// if / else on its bits (B = 3 .. 0), so each leaf updates acc in place.
+// Each test reads w through readfirstlane, which the compiler does not move
+// across the branches: without it, the tests are hoisted and merged into
+// lane-mask conditions or a switch whose joins copy the accumulators.
+template <int B, int N>
+__device__ __forceinline__ void nib_tree(uint32_t (&acc)[8], uint32_t w, uint32_t s,
+                                         const uint32_t (&p0)[8], const uint32_t (&p1)[8],
+                                         const uint32_t (&p2)[8], const uint32_t (&p3)[8]) {
+  if constexpr (B < 0) {
+    if constexpr (N != 0) nib_xor<N>(acc, p0, p1, p2, p3);
+  } else {
+    if ((__builtin_amdgcn_readfirstlane(w) >> (s + B)) & 1u)
+      nib_tree<B - 1, N | (1 << B)>(acc, w, s, p0, p1, p2, p3);
+    else
+      nib_tree<B - 1, N>(acc, w, s, p0, p1, p2, p3);
+  }
+}
+
+// acc[j] ^= c_j * y as mac_planes, with each coefficient taken a nibble at a
+// time: the four powers of a nibble stay live (they share five of every
+// eight planes with their neighbours, so 17 registers hold all four), one
+// wave-uniform 16-way branch per (output, nibble) selects the straight-line
+// XOR of its set powers, and pairs of terms fold into one three-input XOR.
+// y is consumed.
+template <int NT>
+__device__ __forceinline__ void mac_planes_nib(uint32_t (&acc)[NT][8], uint32_t (&y)[8],
+                                               const uint32_t *cw) {
+  uint32_t word[NT / 4];
+#pragma unroll
+  for (int q = 0; q < NT / 4; ++q) word[q] = ld_const(cw + q);  // wave-uniform: s_load
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    uint32_t p1[8], p2[8], p3[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) p1[r] = y[r];
+    planes_times2(p1);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) p2[r] = p1[r];
+    planes_times2(p2);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) p3[r] = p2[r];
+    planes_times2(p3);
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      nib_tree<3, 0>(acc[j], word[j >> 2], 8 * (j & 3) + 4 * h, y, p1, p2, p3);
+    }
+    if (h == 0) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) y[r] = p3[r];
+      planes_times2(y);
+    }
+  }
+}
+
 // One lane's share of a bit-sliced apply tile: outputs jbase..jbase+nt over
 // the lane's split 32 bytes whose first piece is at `off` (an active lane:
 // off < len).  The next input's bytes are loaded before the current input's
 // multiply-accumulate.  coef holds the rows scaled by 2^t_j and, after the
 // k * ostride coefficient bytes, the ostride exponents t_j (int8): each
 // output is multiplied by 2^-t_j before it is stored.
-template <int NT, uint32_t MODE, typename PtrAt>
+// FAST (the 16-byte access modes): nibble-wise multiply-accumulate and the
+// select form of the transposes.  The byte-access modes keep the bitwise
+// multiply-accumulate, whose smaller code and register set leave the
+// kernel's register count to the fast paths.
+template <int NT, uint32_t MODE, bool FAST = false, typename PtrAt>
 __device__ __forceinline__ void gf8_planes_body(PtrAt ptr_at, const uint32_t *coef, uint64_t len,
                                                 int mode, uint32_t k, uint32_t nout,
                                                 uint32_t ostride, uint32_t jbase, uint64_t off,
@@ -297,8 +379,9 @@ __device__ __forceinline__ void gf8_planes_body(PtrAt ptr_at, const uint32_t *co
 #pragma unroll
     for (int r = 0; r < 8; ++r) y[r] = nxt[r];
     if (i + 1 < k) load_split_raw(ptr_at(i + 1), off, len, mode, nxt);
-    bytes_to_planes(y);
-    mac_planes<NT>(acc, y, coef + ((size_t)i * ostride + jbase) / 4);
+    bytes_to_planes<FAST>(y);
+    if constexpr (FAST) mac_planes_nib<NT>(acc, y, coef + ((size_t)i * ostride + jbase) / 4);
+    else mac_planes<NT>(acc, y, coef + ((size_t)i * ostride + jbase) / 4);
   }
   {
     uint32_t tw[NT / 4];
@@ -315,17 +398,17 @@ __device__ __forceinline__ void gf8_planes_body(PtrAt ptr_at, const uint32_t *co
     if (uint32_t(j) < nt) {
       uint8_t *o = const_cast<uint8_t *>(ptr_at(k + jbase + j));
       if (MODE == kWrite) {
-        store_split(o, off, len, mode, acc[j]);
+        store_split<false, FAST>(o, off, len, mode, acc[j]);
       } else {
         uint32_t y[8];
-        load_split(o, off, len, mode, y);
+        load_split<false, FAST>(o, off, len, mode, y);
         if (MODE == kVerify) {
 #pragma unroll
           for (int r = 0; r < 8; ++r) bad |= y[r] ^ acc[j][r];
         } else {
 #pragma unroll
           for (int r = 0; r < 8; ++r) y[r] ^= acc[j][r];
-          store_split(o, off, len, mode, y);
+          store_split<false, FAST>(o, off, len, mode, y);
         }
       }
     }
@@ -342,10 +425,11 @@ __device__ __forceinline__ void gf8_tile_planes(PtrAt ptr_at, const uint32_t *co
                                                 uint64_t off, uint32_t *flag) {
   const int mode = split_mode(vec_ok, tile, len);
   if (mode == kSplitFull)
-    gf8_planes_body<NT, MODE>(ptr_at, coef, len, kSplitFull, k, nout, ostride, jbase, off, flag);
+    gf8_planes_body<NT, MODE, true>(ptr_at, coef, len, kSplitFull, k, nout, ostride, jbase, off,
+                                    flag);
   else if (mode == kSplitMasked)
-    gf8_planes_body<NT, MODE>(ptr_at, coef, len, kSplitMasked, k, nout, ostride, jbase, off,
-                              flag);
+    gf8_planes_body<NT, MODE, true>(ptr_at, coef, len, kSplitMasked, k, nout, ostride, jbase, off,
+                                    flag);
   else
     gf8_planes_body<NT, MODE>(ptr_at, coef, len, mode, k, nout, ostride, jbase, off, flag);
 }
@@ -399,10 +483,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch8(const BatchJob *jobs,
 // The same work list over the bit-sliced tile: tiles are (job, 8 KiB column
 // chunk, kPlanesOut-row output tile).  Eight rows (64 accumulator VGPRs, five
 // waves per SIMD) measured faster than sixteen (three waves) although jobs
-// with more than eight outputs then read their inputs once per tile.
-__global__ __launch_bounds__(kBlockThreads) void k_batch8_planes(const BatchJob *jobs,
-                                                                const uint32_t *tile_job,
-                                                                uint64_t total) {
+// with more than eight outputs then read their inputs once per tile.  The
+// nibble-wise multiply-accumulate needs 96 VGPRs for five waves; the
+// attribute holds the compiler to that.
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(5))) void
+k_batch8_planes(const BatchJob *jobs, const uint32_t *tile_job, uint64_t total) {
   for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
     const auto &J = ld_const_ref(jobs + ld_const(tile_job + t));
     const uint64_t len = J.len;
