@@ -133,11 +133,17 @@ struct DevCoef {
   }
 };
 
-// The bit-sliced kernels XOR x*2^b into an output for every set bit b of its
-// coefficient, so their cost per coefficient is 8 XORs per set bit.  Scaling
-// output row j by 2^t (t in [-7, 7]) and multiplying the result by 2^-t once,
-// at 3 XORs per step in plane form, changes the bits but not the bytes
-// written; t_j minimises 8 * (set bits of the scaled row) + 3 |t|.
+// The bit-sliced kernels apply a coefficient a nibble at a time, XORing the
+// nibble's set powers x*2^b into the output two per three-input XOR, so a
+// coefficient costs 8 * (ceil(s_lo / 2) + ceil(s_hi / 2)) operations for s_lo
+// and s_hi set bits in its nibbles.  Scaling output row j by 2^t (t in
+// [-7, 7]) and multiplying the result by 2^-t once, at 3 XORs per step in
+// plane form, changes the bits but not the bytes written; t_j minimises that
+// cost over the scaled row plus 3 |t|.
+int nibble_cost(uint8_t c) {
+  return (__builtin_popcount(c & 15u) + 1) / 2 + (__builtin_popcount(c >> 4) + 1) / 2;
+}
+
 int8_t planes_scale(const uint8_t *row, size_t k, size_t stride) {
   const Gf8 &g = gf8();
   int best_t = 0;
@@ -145,7 +151,7 @@ int8_t planes_scale(const uint8_t *row, size_t k, size_t stride) {
   for (int t = -7; t <= 7; ++t) {
     const uint8_t s = g.exp[(t + 255) % 255];
     long cost = 3L * (t < 0 ? -t : t);
-    for (size_t i = 0; i < k; ++i) cost += 8L * __builtin_popcount(g.mul[row[i * stride]][s]);
+    for (size_t i = 0; i < k; ++i) cost += 8L * nibble_cost(g.mul[row[i * stride]][s]);
     if (best < 0 || cost < best) {
       best = cost;
       best_t = t;
@@ -2869,7 +2875,7 @@ int trs_batch8_prepare(trs_batch8 *b) {
     J.nout = r.nout;
     J.ostride = r.coef->ostride;
     J.vec_ok = al ? 1u : 0u;
-    const uint32_t rows_per_tile = planes_job ? kPlanesOut : kOutStride;
+    const uint32_t rows_per_tile = planes_job ? kBatchPlanesOut : kOutStride;
     J.out_tiles = (r.nout + rows_per_tile - 1) / rows_per_tile;
     J.prog = r.prog;
     if (r.frag) {

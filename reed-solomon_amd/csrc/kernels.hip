@@ -486,7 +486,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch8(const BatchJob *jobs,
 // with more than eight outputs then read their inputs once per tile.  The
 // nibble-wise multiply-accumulate needs 96 VGPRs for five waves; the
 // attribute holds the compiler to that.
-__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(5))) void
+__global__ __launch_bounds__(kBlockThreads)
+__attribute__((amdgpu_waves_per_eu(kBatchPlanesOut == 8 ? 5 : 3))) void
 k_batch8_planes(const BatchJob *jobs, const uint32_t *tile_job, uint64_t total) {
   for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
     const auto &J = ld_const_ref(jobs + ld_const(tile_job + t));
@@ -499,9 +500,9 @@ k_batch8_planes(const BatchJob *jobs, const uint32_t *tile_job, uint64_t total) 
     const uint64_t off = tile + split_lane_offset(threadIdx.x);
     if (off < len) {
       const uint8_t *const *ptrs = J.ptrs;
-      gf8_tile_planes<kPlanesOut, kWrite>([&](uint32_t idx) { return ld_const_ptr(ptrs + idx); },
+      gf8_tile_planes<kBatchPlanesOut, kWrite>([&](uint32_t idx) { return ld_const_ptr(ptrs + idx); },
                                           J.coef, len, J.k, J.nout, J.ostride, J.vec_ok != 0,
-                                          ot * kPlanesOut, tile, off, nullptr);
+                                          ot * kBatchPlanesOut, tile, off, nullptr);
     }
   }
 }

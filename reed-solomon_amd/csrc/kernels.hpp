@@ -98,6 +98,11 @@ hipError_t launch_batch8(const BatchJob *jobs, const uint32_t *tile_job, uint64_
 // column chunks, 32 bytes per lane as bit planes).
 constexpr uint64_t kPlanesChunkBytes = 8192;
 constexpr int kPlanesOut = 8;  // output rows per planes tile (a multiple of 4; 8 measured best)
+#ifndef TRS_BATCH_PLANES_OUT
+#define TRS_BATCH_PLANES_OUT 8
+#endif
+// Output rows per tile of the batched reconstruct kernel (k_batch8_planes).
+constexpr int kBatchPlanesOut = TRS_BATCH_PLANES_OUT;
 hipError_t launch_batch8_planes(const BatchJob *jobs, const uint32_t *tile_job,
                                 uint64_t total_tiles, uint32_t grid, hipStream_t stream);
 hipError_t launch_fill(uint8_t *dst, uint64_t len, uint64_t seed, hipStream_t stream);

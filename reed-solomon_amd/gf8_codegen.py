@@ -107,22 +107,56 @@ def _dispatch(name: str, ins: str, len_: str, vec: str, ind: str, verify: bool =
     """One branch per tile on its access mode (bitslice.hpp, split_mode): the
     full and masked paths get straight-line copies of the program.  `verify`
     compares the outputs with the shards instead of storing them."""
-    fn = f"{name}<{'true' if verify else 'false'}>"
+    v = 'true' if verify else 'false'
     return "\n".join([
         f"{ind}const int mode = split_mode({vec}, tile, {len_});",
         f"{ind}uint32_t bad = 0;",
-        f"{ind}if (mode == kSplitFull) {fn}({ins}, out, off, {len_}, kSplitFull, bad);",
-        f"{ind}else if (mode == kSplitMasked) {fn}({ins}, out, off, {len_}, kSplitMasked, bad);",
-        f"{ind}else {fn}({ins}, out, off, {len_}, mode, bad);",
+        f"{ind}if (mode == kSplitFull) {name}<{v}, true>({ins}, out, off, {len_}, kSplitFull, bad);",
+        f"{ind}else if (mode == kSplitMasked) {name}<{v}, true>({ins}, out, off, {len_}, kSplitMasked, "
+        "bad);",
+        f"{ind}else {name}<{v}, false>({ins}, out, off, {len_}, mode, bad);",
     ])
+
+
+# Code shape of the 16-byte access paths (FAST; the byte-access paths keep
+# the plain form, so their byte loads do not raise the kernels' register
+# counts):
+#   LOAD_BLOCK  every input's loads are issued before the first transpose (a
+#               scheduling barrier between), so a tile's loads are in flight
+#               together; left to itself the compiler interleaves loads with
+#               transposes and waits after every few (at one wave per SIMD,
+#               (32,32), one input at a time);
+#   OPAQUE      each op's result passes through an empty asm statement, so the
+#               compiler cannot reassociate XOR terms across ops and keep the
+#               shared partial sums live (232 -> 199 VGPRs for (16,16));
+#   SCHED_GROUP a scheduling barrier every SCHED_GROUP ops (0 = none), so the
+#               schedule follows the program's own order.
+# Measured together on batched encode: 4670-4732 -> 4949-4993 GB/s.
+LOAD_BLOCK = int(os.environ.get("TRS_GEN_LOAD_BLOCK", "1"))
+OPAQUE = int(os.environ.get("TRS_GEN_OPAQUE", "1"))
+SCHED_GROUP = int(os.environ.get("TRS_GEN_SCHED_GROUP", "1"))
 
 
 def emit_body(k: int, ops, nreg: int) -> str:
     out = []
     out.append(f"  uint32_t R[{nreg}][8];")
-    for s in range(k):
-        out.append(f"  load_split<true>(ld_const_ptr(in + {s}), off, len, mode, R[{s}]);")
-    for op in ops:
+    if LOAD_BLOCK:
+        out.append("  if constexpr (FAST) {")
+        for s in range(k):
+            out.append(f"    load_split_raw<true>(ld_const_ptr(in + {s}), off, len, mode, R[{s}]);")
+        out.append("    __builtin_amdgcn_sched_barrier(0);")
+        for s in range(k):
+            out.append(f"    bytes_to_planes(R[{s}]);")
+        out.append("  } else {")
+        for s in range(k):
+            out.append(f"    load_split<true>(ld_const_ptr(in + {s}), off, len, mode, R[{s}]);")
+        out.append("  }")
+    else:
+        for s in range(k):
+            out.append(f"  load_split<true>(ld_const_ptr(in + {s}), off, len, mode, R[{s}]);")
+    for i, op in enumerate(ops):
+        if SCHED_GROUP and i % SCHED_GROUP == 0:
+            out.append("  if constexpr (FAST) __builtin_amdgcn_sched_barrier(0);")
         kind = op[0]
         if kind == "st":
             _, j, r = op
@@ -147,6 +181,8 @@ def emit_body(k: int, ops, nreg: int) -> str:
                         out.append(f"  R[{d}][{r}] ^= {terms};")
                 else:
                     out.append(f"  R[{d}][{r}] = {terms if terms else '0u'};")
+        if OPAQUE and kind in ("xo", "mx", "mc"):
+            out.append(f"  if constexpr (FAST) opaque(R[{op[1]}]);")
     return "\n".join(out)
 
 
@@ -189,6 +225,15 @@ def generate() -> str:
         "",
         "namespace trs {",
         "namespace {",
+        "",
+        "// The register's planes pass through an empty asm statement: the compiler",
+        "// no longer sees how they were computed, so it cannot reassociate and share",
+        "// XOR terms across program ops.",
+        "__device__ __forceinline__ void opaque(uint32_t (&x)[8]) {",
+        "#pragma unroll",
+        "  for (int r = 0; r < 8; ++r) asm(\"\" : \"+v\"(x[r]));",
+        "}",
+        "",
     ]
     table = []
     for (k, m) in SHAPES:
@@ -201,7 +246,7 @@ def generate() -> str:
             name = f"bs_{route}_{k}_{m}"
             parts.append(f"// ({k},{m}) {route}: {nreg} registers, ~{xor_cost(ops)} plane XORs "
                          "per 32-byte lane chunk")
-            parts.append(f"template <bool V> __device__ __forceinline__ void {name}("
+            parts.append(f"template <bool V, bool FAST> __device__ __forceinline__ void {name}("
                          "const uint8_t *const *in, uint8_t *const *out, uint64_t off, "
                          "uint64_t len, int mode, uint32_t &bad) {")
             parts.append(emit_body(k, ops, nreg))

@@ -17,14 +17,19 @@ def pytest_configure(config):
 
 
 def _ensure_built():
-    lib = os.path.join(ROOT, "reed-solomon_amd", "libtrs_gpu.so")
-    orc = os.path.join(ROOT, "oracle", "liboracle.so")
+    """Builds the library and the oracle when they are missing, and brings
+    them up to date when their object files are here (a build tree): make
+    then rebuilds whatever changed.  A tree that carries only the built .so
+    files (no objects) is used as it is."""
+    import glob
     import subprocess
 
-    if not os.path.exists(lib):
-        subprocess.run(["make", "-j8", "-C", os.path.join(ROOT, "reed-solomon_amd", "csrc")],
-                       check=True)
-    if not os.path.exists(orc):
+    csrc = os.path.join(ROOT, "reed-solomon_amd", "csrc")
+    lib = os.path.join(ROOT, "reed-solomon_amd", "libtrs_gpu.so")
+    orc = os.path.join(ROOT, "oracle", "liboracle.so")
+    if not os.path.exists(lib) or glob.glob(os.path.join(csrc, "*.o")):
+        subprocess.run(["make", "-j8", "-C", csrc], check=True)
+    if not os.path.exists(orc) or os.path.exists(os.path.join(ROOT, "oracle", "simd_fft_gen.c")):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True)
 
 
