@@ -381,3 +381,31 @@ def test_unaligned_rows_on_mfma_route(trs):
     drows = to_dev(rows)
     rs.reconstruct_rows(drows, row_len, present)
     assert (drows.cpu().numpy() == pristine).all()
+
+
+@pytest.mark.parametrize("k,m", [(32, 32), (16, 16)])
+def test_planes_tile_many_coefficients(trs, k, m):
+    """The bit-sliced planes tile's 16-byte paths (nibble-wise
+    multiply-accumulate, select-form transposes): single-call reconstructs of
+    ~150 KiB shards (full 8 KiB tiles and a masked last tile) over 24 random
+    erasure patterns, up to four 8-row output tiles each, rebuild every shard
+    byte for byte.  The patterns' decode rows hold most of the 256 byte
+    values, in every row position of a tile."""
+    rng = np.random.default_rng(k * 1000 + m)
+    G = O.build_matrix(8, k, k + m)
+    rs = trs.ReedSolomon(k, m)
+    rs.set_apply_route("table")  # >= 128 KiB: the planes kernel
+    length = 19 * 8192 + 48
+    data = seeded(k, length, 0xC0EF + k)
+    original = np.concatenate([data, O.rs8_encode(G, k, data)])
+    seen = set()
+    for _ in range(24):
+        e = int(rng.integers(1, m + 1))
+        erased = sorted(rng.choice(k + m, e, replace=False).tolist())
+        present = [i not in erased for i in range(k + m)]
+        valid, mdata, drows, _ = O.rs8_decode_plan(G, k, m, present)
+        seen.update(int(c) for c in np.asarray(drows).ravel())
+        cur = rows_to_dev(original)
+        rs.reconstruct(_erase_and_view(cur, set(erased)))
+        assert (to_host(cur) == original).all(), (k, m, erased)
+    assert len(seen) > 200, len(seen)
