@@ -137,8 +137,17 @@ OPAQUE = int(os.environ.get("TRS_GEN_OPAQUE", "1"))
 SCHED_GROUP = int(os.environ.get("TRS_GEN_SCHED_GROUP", "1"))
 
 
-def emit_body(k: int, ops, nreg: int) -> str:
+# Programs whose 16-byte paths transpose with the select form of the bit swap
+# (bitslice.hpp: 48 instead of 60 operations per 32 bytes, one more live
+# register).  It pays only where VALU bounds the program at one wave per
+# SIMD: (32,32) 2322 -> 2234 us per batched launch, while (7,13) and (16,16)
+# were 3-9 % slower with it.
+SELECT_SWAP = {("fft", 32, 32)}
+
+
+def emit_body(k: int, ops, nreg: int, sel: bool = False) -> str:
     out = []
+    fs = "FAST" if sel else "false"
     out.append(f"  uint32_t R[{nreg}][8];")
     if LOAD_BLOCK:
         out.append("  if constexpr (FAST) {")
@@ -146,7 +155,7 @@ def emit_body(k: int, ops, nreg: int) -> str:
             out.append(f"    load_split_raw<true>(ld_const_ptr(in + {s}), off, len, mode, R[{s}]);")
         out.append("    __builtin_amdgcn_sched_barrier(0);")
         for s in range(k):
-            out.append(f"    bytes_to_planes(R[{s}]);")
+            out.append(f"    bytes_to_planes<{fs}>(R[{s}]);")
         out.append("  } else {")
         for s in range(k):
             out.append(f"    load_split<true>(ld_const_ptr(in + {s}), off, len, mode, R[{s}]);")
@@ -162,7 +171,7 @@ def emit_body(k: int, ops, nreg: int) -> str:
             _, j, r = op
             out.append(f"  if constexpr (V) verify_split<true>(ld_const_ptr(out + {j}), off, len, "
                        f"mode, R[{r}], bad);")
-            out.append(f"  else store_split<true>(ld_const_ptr(out + {j}), off, len, mode, "
+            out.append(f"  else store_split<true, {fs}>(ld_const_ptr(out + {j}), off, len, mode, "
                        f"R[{r}]);")
         elif kind == "zero":
             out.append(f"  for (int r = 0; r < 8; ++r) R[{op[1]}][r] = 0;")
@@ -249,7 +258,7 @@ def generate() -> str:
             parts.append(f"template <bool V, bool FAST> __device__ __forceinline__ void {name}("
                          "const uint8_t *const *in, uint8_t *const *out, uint64_t off, "
                          "uint64_t len, int mode, uint32_t &bad) {")
-            parts.append(emit_body(k, ops, nreg))
+            parts.append(emit_body(k, ops, nreg, (route, k, m) in SELECT_SWAP))
             parts.append("}")
             parts.append(f"__global__ __launch_bounds__(kBlockThreads) {_waves_attr(route, k, m)}"
                          f"void k_{name}(const Gf8Call a) {{")
