@@ -398,7 +398,7 @@ __device__ __forceinline__ void gf8_planes_body(PtrAt ptr_at, const uint32_t *co
     if (uint32_t(j) < nt) {
       uint8_t *o = const_cast<uint8_t *>(ptr_at(k + jbase + j));
       if (MODE == kWrite) {
-        store_split<false, FAST>(o, off, len, mode, acc[j]);
+        store_split<true, FAST>(o, off, len, mode, acc[j]);
       } else {
         uint32_t y[8];
         load_split<false, FAST>(o, off, len, mode, y);
