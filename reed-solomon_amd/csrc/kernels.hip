@@ -357,10 +357,11 @@ __device__ __forceinline__ void mac_planes_nib(uint32_t (&acc)[NT][8], uint32_t 
 // k * ostride coefficient bytes, the ostride exponents t_j (int8): each
 // output is multiplied by 2^-t_j before it is stored.
 // FAST (the 16-byte access modes): nibble-wise multiply-accumulate and the
-// select form of the transposes.  The byte-access modes keep the bitwise
+// select form of the transposes.  LNT: the 16-byte input loads are
+// non-temporal (inputs that no other output tile reads again).  The byte-access modes keep the bitwise
 // multiply-accumulate, whose smaller code and register set leave the
 // kernel's register count to the fast paths.
-template <int NT, uint32_t MODE, bool FAST = false, typename PtrAt>
+template <int NT, uint32_t MODE, bool FAST = false, bool LNT = false, typename PtrAt>
 __device__ __forceinline__ void gf8_planes_body(PtrAt ptr_at, const uint32_t *coef, uint64_t len,
                                                 int mode, uint32_t k, uint32_t nout,
                                                 uint32_t ostride, uint32_t jbase, uint64_t off,
@@ -373,12 +374,12 @@ __device__ __forceinline__ void gf8_planes_body(PtrAt ptr_at, const uint32_t *co
     for (int r = 0; r < 8; ++r) acc[j][r] = 0;
 
   uint32_t nxt[8];
-  load_split_raw(ptr_at(0), off, len, mode, nxt);
+  load_split_raw<LNT>(ptr_at(0), off, len, mode, nxt);
   for (uint32_t i = 0; i < k; ++i) {
     uint32_t y[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) y[r] = nxt[r];
-    if (i + 1 < k) load_split_raw(ptr_at(i + 1), off, len, mode, nxt);
+    if (i + 1 < k) load_split_raw<LNT>(ptr_at(i + 1), off, len, mode, nxt);
     bytes_to_planes<FAST>(y);
     if constexpr (FAST) mac_planes_nib<NT>(acc, y, coef + ((size_t)i * ostride + jbase) / 4);
     else mac_planes<NT>(acc, y, coef + ((size_t)i * ostride + jbase) / 4);
@@ -418,17 +419,17 @@ __device__ __forceinline__ void gf8_planes_body(PtrAt ptr_at, const uint32_t *co
 
 // One branch per tile on its access mode (split_mode), so the common paths
 // are straight-line copies of the body.
-template <int NT, uint32_t MODE, typename PtrAt>
+template <int NT, uint32_t MODE, bool LNT = false, typename PtrAt>
 __device__ __forceinline__ void gf8_tile_planes(PtrAt ptr_at, const uint32_t *coef, uint64_t len,
                                                 uint32_t k, uint32_t nout, uint32_t ostride,
                                                 bool vec_ok, uint32_t jbase, uint64_t tile,
                                                 uint64_t off, uint32_t *flag) {
   const int mode = split_mode(vec_ok, tile, len);
   if (mode == kSplitFull)
-    gf8_planes_body<NT, MODE, true>(ptr_at, coef, len, kSplitFull, k, nout, ostride, jbase, off,
+    gf8_planes_body<NT, MODE, true, LNT>(ptr_at, coef, len, kSplitFull, k, nout, ostride, jbase, off,
                                     flag);
   else if (mode == kSplitMasked)
-    gf8_planes_body<NT, MODE, true>(ptr_at, coef, len, kSplitMasked, k, nout, ostride, jbase, off,
+    gf8_planes_body<NT, MODE, true, LNT>(ptr_at, coef, len, kSplitMasked, k, nout, ostride, jbase, off,
                                     flag);
   else
     gf8_planes_body<NT, MODE>(ptr_at, coef, len, mode, k, nout, ostride, jbase, off, flag);
@@ -500,9 +501,19 @@ k_batch8_planes(const BatchJob *jobs, const uint32_t *tile_job, uint64_t total) 
     const uint64_t off = tile + split_lane_offset(threadIdx.x);
     if (off < len) {
       const uint8_t *const *ptrs = J.ptrs;
-      gf8_tile_planes<kBatchPlanesOut, kWrite>([&](uint32_t idx) { return ld_const_ptr(ptrs + idx); },
-                                          J.coef, len, J.k, J.nout, J.ostride, J.vec_ok != 0,
-                                          ot * kBatchPlanesOut, tile, off, nullptr);
+      auto ptr_at = [&](uint32_t idx) { return ld_const_ptr(ptrs + idx); };
+      // A job with one output tile reads its inputs once: non-temporal
+      // loads keep them from evicting the inputs that multi-tile jobs read
+      // again per output tile (measured 4086-4133 -> 4398-4404 GB/s; making
+      // the last tile of every job non-temporal instead was slower).
+      if (out_tiles == 1)
+        gf8_tile_planes<kBatchPlanesOut, kWrite, true>(ptr_at, J.coef, len, J.k, J.nout,
+                                                       J.ostride, J.vec_ok != 0,
+                                                       ot * kBatchPlanesOut, tile, off, nullptr);
+      else
+        gf8_tile_planes<kBatchPlanesOut, kWrite>(ptr_at, J.coef, len, J.k, J.nout, J.ostride,
+                                                 J.vec_ok != 0, ot * kBatchPlanesOut, tile, off,
+                                                 nullptr);
     }
   }
 }
