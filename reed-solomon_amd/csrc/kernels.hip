@@ -495,8 +495,17 @@ k_batch8_planes(const BatchJob *jobs, const uint32_t *tile_job, uint64_t total) 
     const uint64_t len = J.len;
     const uint32_t out_tiles = J.out_tiles;
     const uint64_t local = t - J.tile_begin;
-    const uint64_t chunk = local / out_tiles;
-    const uint32_t ot = uint32_t(local - chunk * out_tiles);
+    // Within a job, tiles run in groups of eight chunks, output tile major:
+    // the output tiles of one chunk are 8 tile indices apart (fewer in a
+    // job's last group), so they land on one XCD (workgroups go to the XCDs
+    // round robin) at about the same time and their input re-reads meet in
+    // its L2.  One-tile jobs keep chunk order.
+    const uint64_t chunks = (len + kPlanesChunkBytes - 1) / kPlanesChunkBytes;
+    const uint64_t q = local / (8u * out_tiles);
+    const uint64_t r = local - q * 8u * out_tiles;
+    const uint64_t g = min(uint64_t(8), chunks - q * 8u);
+    const uint32_t ot = uint32_t(r / g);
+    const uint64_t chunk = q * 8u + (r - uint64_t(ot) * g);
     const uint64_t tile = chunk * kPlanesChunkBytes;
     const uint64_t off = tile + split_lane_offset(threadIdx.x);
     if (off < len) {
