@@ -195,6 +195,241 @@ def emit_body(k: int, ops, nreg: int, sel: bool = False) -> str:
     return "\n".join(out)
 
 
+# Work-list programs that run as two half programs on a pair of waves.  An
+# LCH encode is an IFFT over the k data registers (butterfly strides 1, 2, ...,
+# k/2) followed by an FFT (strides k/2, ..., 1): only the two stride-k/2 levels
+# pair a register of the lower half with one of the upper half.  Together they
+# map each pair (a, b) = (R[j], R[j + k/2]) to (a ^ c t, b ^ c t) with
+# t = a ^ b, the same constant c on both sides.  So one wave holds registers
+# 0..k/2-1 and the other k/2..k-1, each loads its k/2 data shards, runs its
+# half of the IFFT, swaps its registers with its partner through LDS, applies
+# the pair map, runs its half of the FFT and stores its k/2 parity shards.
+# Each wave then keeps k/2 + 1 registers live instead of k + 1: for (32,32),
+# two waves per SIMD instead of one.
+SPLIT = {("fft", 32, 32)} if int(os.environ.get("TRS_GEN_SPLIT", "1")) else set()
+
+
+def split_program(ops, nreg: int, k: int):
+    """Halves of a generated encode program for a wave pair.
+
+    Returns (pre, consts, post): pre[h] and post[h] are half h's ops before
+    and after the exchange, on local registers (register j of half h is
+    program register j + h*k/2); consts[j] is the pair map constant of
+    (R[j], R[j + k/2]).  Raises ValueError when the program does not have
+    that form.  The copies of the program (register renames between the IFFT
+    and the FFT) are folded into register names first."""
+    name = list(range(nreg))
+    prog = []
+    for op in ops:
+        kind = op[0]
+        if kind == "cp":
+            name[op[1]] = name[op[2]]
+        elif kind == "st":
+            prog.append(("st", op[1], name[op[2]]))
+        elif kind == "xo":
+            prog.append(("xo", name[op[1]], name[op[2]]))
+        elif kind in ("mx", "mc"):
+            prog.append((kind, name[op[1]], name[op[2]], op[3]))
+        else:
+            raise ValueError(f"op {op} in a split program")
+    h = k // 2
+    if k % 2 or any(r >= k for op in prog for r in ((op[2],) if op[0] == "st" else op[1:3])):
+        raise ValueError("registers beyond the data registers")
+    side = lambda r: r // h  # noqa: E731
+    cross = [i for i, op in enumerate(prog) if op[0] != "st" and side(op[1]) != side(op[2])]
+    if not cross:
+        raise ValueError("no cross-half ops")
+    first, last = cross[0], cross[-1]
+    mid = prog[first:last + 1]
+    if any(op[0] == "st" or side(op[1]) == side(op[2]) for op in mid):
+        raise ValueError("cross-half ops are not one block")
+    consts = []
+    for j in range(h):
+        v = {j: (1, 0), j + h: (0, 1)}  # each value as (coefficient of a, of b)
+        for op in mid:
+            d, s = op[1], op[2]
+            if d not in v and s not in v:
+                continue
+            if {d, s} != {j, j + h}:
+                raise ValueError("cross-half ops do not pair j with j + k/2")
+            if op[0] == "xo":
+                v[d] = (v[d][0] ^ v[s][0], v[d][1] ^ v[s][1])
+            elif op[0] == "mx":
+                v[d] = (v[d][0] ^ MUL[op[3]][v[s][0]], v[d][1] ^ MUL[op[3]][v[s][1]])
+            else:
+                raise ValueError("overwriting op among the cross-half ops")
+        (p, q), (r, s) = v[j], v[j + h]
+        if not (q == r and p == 1 ^ q and s == 1 ^ q):
+            raise ValueError(f"pair {j} map is not (a ^ c t, b ^ c t)")
+        consts.append(q)
+
+    def local(seg, hh):
+        out = []
+        for op in seg:
+            own = op[2] if op[0] == "st" else op[1]
+            if side(own) != hh:
+                continue
+            if op[0] == "st":
+                out.append(("st", op[1], op[2] - hh * h))
+            else:
+                out.append((op[0], op[1] - hh * h, op[2] - hh * h) + tuple(op[3:]))
+        return out
+
+    pre = [local(prog[:first], hh) for hh in (0, 1)]
+    post = [local(prog[last + 1:], hh) for hh in (0, 1)]
+    return pre, consts, post
+
+
+def _split_op_lines(op, sel: str) -> list:
+    kind = op[0]
+    if kind == "st":
+        return [f"  if (ALL || act) store_split<true, {sel}>(ld_const_ptr(out + {op[1]}), off, len, "
+                f"mode, R[{op[2]}]);"]
+    lines = []
+    if kind == "xo":
+        for r in range(8):
+            lines.append(f"  R[{op[1]}][{r}] ^= R[{op[2]}][{r}];")
+    else:
+        _, d, s, c = op
+        rows = bit_rows(c)
+        for r in range(8):
+            terms = " ^ ".join(f"R[{s}][{b}]" for b in rows[r])
+            if kind == "mx":
+                if terms:
+                    lines.append(f"  R[{d}][{r}] ^= {terms};")
+            else:
+                lines.append(f"  R[{d}][{r}] = {terms if terms else '0u'};")
+    lines.append("  opaque(R[" + str(op[1]) + "]);")
+    return lines
+
+
+def emit_split(name: str, k: int, ops, nreg: int, sel: bool) -> str:
+    """Device functions and the work-list kernel of a split program."""
+    pre, consts, post = split_program(ops, nreg, k)
+    h = k // 2
+    rounds = 2  # registers cross LDS in two rounds of h/2: 32 KiB per wave pair
+    per = h // rounds
+    fs = "FAST" if sel else "false"
+    out = []
+    for hh in (0, 1):
+        out.append(f"// half {hh}: data shards {hh * h}..{hh * h + h - 1}, its half of the IFFT")
+        out.append(f"template <bool FAST, bool ALL> __device__ __forceinline__ void {name}_pre{hh}("
+                   "const uint8_t *const *in, uint64_t off, uint64_t len, int mode, bool act, "
+                   f"uint32_t (&R)[{h}][8]) {{")
+        out.append("  if (ALL || act) {")
+        out.append("    if constexpr (FAST) {")
+        for s in range(h):
+            out.append(f"      load_split_raw<true>(ld_const_ptr(in + {hh * h + s}), off, len, mode, "
+                       f"R[{s}]);")
+        out.append("    } else {")
+        for s in range(h):  # one input at a time: byte loads are not hoisted
+            out.append(f"      load_split<true>(ld_const_ptr(in + {hh * h + s}), off, len, mode, "
+                       f"R[{s}]);")
+            out.append("      __builtin_amdgcn_sched_barrier(0);")
+        out.append("    }")
+        out.append("  } else {")
+        out.append(f"    for (int s = 0; s < {h}; ++s)")
+        out.append("      for (int r = 0; r < 8; ++r) R[s][r] = 0;")
+        out.append("  }")
+        out.append("  if constexpr (FAST) {")
+        out.append("    __builtin_amdgcn_sched_barrier(0);")
+        for s in range(h):
+            out.append(f"    bytes_to_planes<{fs}>(R[{s}]);")
+        out.append("  }")
+        # every path keeps the program's order: at k/2 + 1 live registers per
+        # wave the budget is 256 VGPRs, which reassociated XORs overrun
+        for op in pre[hh]:
+            out.append("  __builtin_amdgcn_sched_barrier(0);")
+            out.extend(_split_op_lines(op, fs))
+        out.append("}")
+        out.append(f"// half {hh}: its half of the FFT and parity shards")
+        out.append(f"template <bool FAST, bool ALL> __device__ __forceinline__ void {name}_post{hh}("
+                   "uint8_t *const *out, uint64_t off, uint64_t len, int mode, bool act, "
+                   f"uint32_t (&R)[{h}][8]) {{")
+        for op in post[hh]:
+            out.append("  __builtin_amdgcn_sched_barrier(0);")
+            out.extend(_split_op_lines(op, fs))
+        out.append("}")
+    out.append("// The pair map of the two stride-k/2 levels: each wave publishes its")
+    out.append("// registers, reads its partner's and adds c (own ^ partner) to its own.")
+    out.append("// X holds one round of both waves' registers; the barrier after a round's")
+    out.append("// reads keeps the next round (or tile) from overwriting them early.")
+    out.append(f"__device__ __forceinline__ void {name}_exchange(uint32_t (&R)[{h}][8], "
+               f"uint4 (*X)[{per}][2][64], uint32_t half, uint32_t lane) {{")
+    for rnd in range(rounds):
+        for r in range(per):
+            j = rnd * per + r
+            out.append(f"  X[half][{r}][0][lane] = make_uint4(R[{j}][0], R[{j}][1], R[{j}][2], "
+                       f"R[{j}][3]);")
+            out.append(f"  X[half][{r}][1][lane] = make_uint4(R[{j}][4], R[{j}][5], R[{j}][6], "
+                       f"R[{j}][7]);")
+        out.append("  __syncthreads();")
+        for r in range(per):
+            j = rnd * per + r
+            rows = bit_rows(consts[j])
+            out.append("  {")
+            out.append(f"    const uint4 p0 = X[half ^ 1u][{r}][0][lane], p1 = X[half ^ 1u][{r}][1][lane];")
+            out.append(f"    const uint32_t t[8] = {{R[{j}][0] ^ p0.x, R[{j}][1] ^ p0.y, R[{j}][2] ^ p0.z, "
+                       f"R[{j}][3] ^ p0.w, R[{j}][4] ^ p1.x, R[{j}][5] ^ p1.y, R[{j}][6] ^ p1.z, "
+                       f"R[{j}][7] ^ p1.w}};")
+            for row in range(8):
+                if rows[row]:
+                    terms = " ^ ".join(f"t[{b}]" for b in rows[row])
+                    out.append(f"    R[{j}][{row}] ^= {terms};")
+            out.append("  }")
+            out.append("  __builtin_amdgcn_sched_barrier(0);")  # one register's reads at a time
+        out.append("  __syncthreads();")
+    out.append("}")
+    out.append(f"template <uint32_t H, bool FAST, bool ALL> __device__ __forceinline__ void {name}_pair("
+               "const uint8_t *const *in, uint8_t *const *out, uint64_t off, uint64_t len, int mode, "
+               f"bool act, uint32_t lane, uint4 (*X)[{per}][2][64]) {{")
+    out.append(f"  uint32_t R[{h}][8];")
+    out.append(f"  if constexpr (H == 0) {name}_pre0<FAST, ALL>(in, off, len, mode, act, R);")
+    out.append(f"  else {name}_pre1<FAST, ALL>(in, off, len, mode, act, R);")
+    out.append(f"  {name}_exchange(R, X, H, lane);")
+    out.append(f"  if constexpr (H == 0) {name}_post0<FAST, ALL>(out, off, len, mode, act, R);")
+    out.append(f"  else {name}_post1<FAST, ALL>(out, off, len, mode, act, R);")
+    out.append("}")
+    # The work list's tiles stay the codec's 8 KiB tiles; a workgroup (one
+    # wave pair) runs a quarter of one, sub-tile u = 4 t + q covering bytes
+    # q*2048 .. q*2048 + 2047 of tile t.  Each wave runs its own copy of the
+    # loop (its half's registers are all that is live in it); both copies
+    # take the same branches, which depend only on the workgroup's sub-tile,
+    # and so meet at the same barriers.  Lanes past the shard end load zeros
+    # and store nothing.
+    out.append(f"template <uint32_t H> __device__ __forceinline__ void {name}_run(const BatchJob *jobs, "
+               f"const uint32_t *tile_job, uint64_t total, uint32_t lane, uint4 (*X)[{per}][2][64]) {{")
+    out.append("  for (uint64_t u = blockIdx.x; u < total * 4; u += gridDim.x) {")
+    out.append("    const uint64_t t = u >> 2;")
+    out.append("    const uint32_t q = uint32_t(u & 3);")
+    out.append("    const auto &J = ld_const_ref(jobs + ld_const(tile_job + t));")
+    out.append("    const uint64_t len = J.len;")
+    out.append("    const uint64_t tile = (t - J.tile_begin) * 8192;")
+    out.append("    if (tile + uint64_t(q) * 2048 >= len) continue;")
+    out.append("    const uint64_t off = tile + split_lane_offset(q * 64 + lane);")
+    out.append("    const bool act = off < len;")
+    out.append("    const uint8_t *const *in = J.ptrs;")
+    out.append(f"    uint8_t *const *out = const_cast<uint8_t *const *>(in + {k});")
+    out.append("    const int mode = split_mode(J.vec_ok != 0, tile, len);")
+    out.append(f"    if (mode == kSplitFull) {name}_pair<H, true, true>(in, out, off, len, mode, act, lane, X);")
+    out.append(f"    else if (mode == kSplitMasked) {name}_pair<H, true, false>(in, out, off, len, mode, act, "
+               "lane, X);")
+    out.append(f"    else {name}_pair<H, false, false>(in, out, off, len, mode, act, lane, X);")
+    out.append("  }")
+    out.append("}")
+    out.append("__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(2))) "
+               f"void k_batch_{name}(const BatchJob *jobs,")
+    out.append("    const uint32_t *tile_job, uint64_t total) {")
+    out.append(f"  __shared__ uint4 X[2][{per}][2][64];")
+    out.append("  const uint32_t lane = threadIdx.x & 63;")
+    out.append("  if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0)")
+    out.append(f"    {name}_run<0>(jobs, tile_job, total, lane, X);")
+    out.append("  else")
+    out.append(f"    {name}_run<1>(jobs, tile_job, total, lane, X);")
+    out.append("}")
+    return "\n".join(out)
+
+
 def xor_cost(ops) -> int:
     """VALU estimate per 32-byte lane chunk: 3-input XORs, excluding transposes."""
     n = 0
@@ -243,8 +478,12 @@ def generate() -> str:
         "  for (int r = 0; r < 8; ++r) asm(\"\" : \"+v\"(x[r]));",
         "}",
         "",
+        "// Threads of a split program's workgroup: one wave per half.",
+        "constexpr int kPairThreads = 128;",
+        "",
     ]
     table = []
+    programs = {}
     for (k, m) in SHAPES:
         routes = []
         if (k, m) in MAT_SHAPES:
@@ -281,6 +520,7 @@ def generate() -> str:
             parts.append("}")
             parts.append("")
             table.append((route, k, m, name))
+            programs[name] = (ops, nreg)
     # Persistent work-list kernels, one per generated program so that each gets
     # the register budget of its own program: tiles are (job, 8 KiB column
     # chunk), numbered job by job; tile_job[t] names the job of tile t.  The
@@ -288,6 +528,11 @@ def generate() -> str:
     # code whose loads can all be in flight together (a per-load branch would
     # put a wait after every load).
     for pid, (route, k, m, name) in enumerate(table, start=1):
+        if (route, k, m) in SPLIT:
+            ops, nreg = programs[name]
+            parts.append(emit_split(name, k, ops, nreg, (route, k, m) in SELECT_SWAP))
+            parts.append("")
+            continue
         waves = _waves_attr(route, k, m)
         parts.append(f"__global__ __launch_bounds__(kBlockThreads) {waves}void k_batch_{name}("
                      "const BatchJob *jobs,")
@@ -322,6 +567,11 @@ def generate() -> str:
     parts.append("  if (total_tiles == 0) return hipSuccess;")
     parts.append("  switch (prog) {")
     for pid, (route, k, m, name) in enumerate(table, start=1):
+        if (route, k, m) in SPLIT:  # four wave-pair workgroups per tile
+            parts.append(f"    case {pid}: hipLaunchKernelGGL(k_batch_{name}, dim3(uint32_t(std::min<uint64_t>("
+                         "uint64_t(grid) * 4, total_tiles * 4))), dim3(kPairThreads), 0, stream, jobs, "
+                         "tile_job, total_tiles); break;")
+            continue
         parts.append(f"    case {pid}: hipLaunchKernelGGL(k_batch_{name}, dim3(grid), dim3(kBlockThreads), 0, "
                      "stream, jobs, tile_job, total_tiles); break;")
     parts.append("    default: return hipErrorNotSupported;")

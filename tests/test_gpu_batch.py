@@ -77,6 +77,43 @@ def test_batch_rerun_is_idempotent(trs):
     assert (to_host(shards[k:]) == O.rs8_encode(G, k, data)).all()
 
 
+def test_batch_pair_program_tiles(trs):
+    """The (32,32) batched encode runs on wave pairs, one per 2 KiB window of
+    each 8 KiB tile, that swap registers through LDS between their halves of
+    the program.  Lengths end in each of the four windows, on full tiles,
+    on aligned tails that are and are not multiples of 16, and shards start
+    one byte past an allocation (byte accesses); all in one launch, against
+    the oracle's generator-matrix encode."""
+    import torch
+
+    k, m = 32, 32
+    rs = trs.ReedSolomon(k, m)
+    G = O.build_matrix(8, k, k + m)
+    lengths = [1, 16, 2048, 2064, 4096 + 16, 6144 + 1, 8192, 8192 + 2048 + 16,
+               3 * 8192 + 5000, 70000, 1 << 18]
+    b = trs.Batch()
+    cases = []
+    for i, L in enumerate(lengths):
+        for unaligned in (False, True):
+            data = seeded(k, L, 500 + 2 * i + int(unaligned))
+            if unaligned:
+                bufs = [torch.full((L + 1,), 0x5A, dtype=torch.uint8, device="cuda")
+                        for _ in range(k + m)]
+                shards = [t[1:] for t in bufs]
+                for j in range(k):
+                    shards[j].copy_(torch.from_numpy(data[j]))
+            else:
+                shards = rows_to_dev(np.concatenate([data, np.full((m, L), 0x5A, np.uint8)]))
+            b.add_encode(rs, shards)
+            cases.append((L, unaligned, shards, O.rs8_encode(G, k, data)))
+    b.prepare()
+    b.run()
+    torch.cuda.synchronize()
+    for L, unaligned, shards, want in cases:
+        got = to_host(shards[k:])
+        assert (got == want).all(), (L, unaligned, np.argwhere(got != want)[:4].tolist())
+
+
 def test_batch_graph_replays_match_eager(trs):
     """Graph mode: every replay of the captured launches (with eager work and
     synchronisations between replays) rebuilds the same bytes as the eager
