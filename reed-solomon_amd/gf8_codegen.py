@@ -283,7 +283,9 @@ def split_program(ops, nreg: int, k: int):
 def _split_op_lines(op, sel: str) -> list:
     kind = op[0]
     if kind == "st":
-        return [f"  if (ALL || act) store_split<true, {sel}>(ld_const_ptr(out + {op[1]}), off, len, "
+        return [f"  if constexpr (V) {{ if (ALL || act) verify_split<true>(ld_const_ptr(out + {op[1]}), "
+                f"off, len, mode, R[{op[2]}], bad); }}",
+                f"  else if (ALL || act) store_split<true, {sel}>(ld_const_ptr(out + {op[1]}), off, len, "
                 f"mode, R[{op[2]}]);"]
     lines = []
     if kind == "xo":
@@ -343,9 +345,9 @@ def emit_split(name: str, k: int, ops, nreg: int, sel: bool) -> str:
             out.extend(_split_op_lines(op, fs))
         out.append("}")
         out.append(f"// half {hh}: its half of the FFT and parity shards")
-        out.append(f"template <bool FAST, bool ALL> __device__ __forceinline__ void {name}_post{hh}("
-                   "uint8_t *const *out, uint64_t off, uint64_t len, int mode, bool act, "
-                   f"uint32_t (&R)[{h}][8]) {{")
+        out.append(f"template <bool V, bool FAST, bool ALL> __device__ __forceinline__ void "
+                   f"{name}_post{hh}(uint8_t *const *out, uint64_t off, uint64_t len, int mode, bool act, "
+                   f"uint32_t (&R)[{h}][8], uint32_t &bad) {{")
         for op in post[hh]:
             out.append("  __builtin_amdgcn_sched_barrier(0);")
             out.extend(_split_op_lines(op, fs))
@@ -380,15 +382,15 @@ def emit_split(name: str, k: int, ops, nreg: int, sel: bool) -> str:
             out.append("  __builtin_amdgcn_sched_barrier(0);")  # one register's reads at a time
         out.append("  __syncthreads();")
     out.append("}")
-    out.append(f"template <uint32_t H, bool FAST, bool ALL> __device__ __forceinline__ void {name}_pair("
-               "const uint8_t *const *in, uint8_t *const *out, uint64_t off, uint64_t len, int mode, "
-               f"bool act, uint32_t lane, uint4 (*X)[{per}][2][64]) {{")
+    out.append(f"template <uint32_t H, bool V, bool FAST, bool ALL> __device__ __forceinline__ void "
+               f"{name}_pair(const uint8_t *const *in, uint8_t *const *out, uint64_t off, uint64_t len, "
+               f"int mode, bool act, uint32_t lane, uint4 (*X)[{per}][2][64], uint32_t &bad) {{")
     out.append(f"  uint32_t R[{h}][8];")
     out.append(f"  if constexpr (H == 0) {name}_pre0<FAST, ALL>(in, off, len, mode, act, R);")
     out.append(f"  else {name}_pre1<FAST, ALL>(in, off, len, mode, act, R);")
     out.append(f"  {name}_exchange(R, X, H, lane);")
-    out.append(f"  if constexpr (H == 0) {name}_post0<FAST, ALL>(out, off, len, mode, act, R);")
-    out.append(f"  else {name}_post1<FAST, ALL>(out, off, len, mode, act, R);")
+    out.append(f"  if constexpr (H == 0) {name}_post0<V, FAST, ALL>(out, off, len, mode, act, R, bad);")
+    out.append(f"  else {name}_post1<V, FAST, ALL>(out, off, len, mode, act, R, bad);")
     out.append("}")
     # The work list's tiles stay the codec's 8 KiB tiles; a workgroup (one
     # wave pair) runs a quarter of one, sub-tile u = 4 t + q covering bytes
@@ -411,10 +413,12 @@ def emit_split(name: str, k: int, ops, nreg: int, sel: bool) -> str:
     out.append("    const uint8_t *const *in = J.ptrs;")
     out.append(f"    uint8_t *const *out = const_cast<uint8_t *const *>(in + {k});")
     out.append("    const int mode = split_mode(J.vec_ok != 0, tile, len);")
-    out.append(f"    if (mode == kSplitFull) {name}_pair<H, true, true>(in, out, off, len, mode, act, lane, X);")
-    out.append(f"    else if (mode == kSplitMasked) {name}_pair<H, true, false>(in, out, off, len, mode, act, "
-               "lane, X);")
-    out.append(f"    else {name}_pair<H, false, false>(in, out, off, len, mode, act, lane, X);")
+    out.append("    uint32_t bad = 0;")
+    out.append(f"    if (mode == kSplitFull) {name}_pair<H, false, true, true>(in, out, off, len, mode, act, "
+               "lane, X, bad);")
+    out.append(f"    else if (mode == kSplitMasked) {name}_pair<H, false, true, false>(in, out, off, len, mode, "
+               "act, lane, X, bad);")
+    out.append(f"    else {name}_pair<H, false, false, false>(in, out, off, len, mode, act, lane, X, bad);")
     out.append("  }")
     out.append("}")
     out.append("__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(2))) "
@@ -427,6 +431,38 @@ def emit_split(name: str, k: int, ops, nreg: int, sel: bool) -> str:
     out.append("  else")
     out.append(f"    {name}_run<1>(jobs, tile_job, total, lane, X);")
     out.append("}")
+    # Per-call kernels: workgroup b runs window b & 3 of tile b >> 2 (grid =
+    # one workgroup per 2 KiB of shard); a window past the end returns in
+    # both waves before any barrier.  The verify form compares each half's
+    # parity with the shards (ReedSolomon::verify) and sets the flag.
+    out.append(f"template <uint32_t H, bool V> __device__ __forceinline__ void {name}_call(const Gf8Call &a, "
+               f"uint32_t lane, uint4 (*X)[{per}][2][64]) {{")
+    out.append("  const uint64_t tile = uint64_t(blockIdx.x >> 2) * 8192;")
+    out.append("  const uint32_t q = blockIdx.x & 3;")
+    out.append("  if (tile + uint64_t(q) * 2048 >= a.len) return;")
+    out.append("  const uint64_t off = tile + split_lane_offset(q * 64 + lane);")
+    out.append("  const bool act = off < a.len;")
+    out.append(f"  uint8_t *const *out = const_cast<uint8_t *const *>(a.ptr + {k});")
+    out.append("  const int mode = split_mode(a.vec_ok != 0, tile, a.len);")
+    out.append("  uint32_t bad = 0;")
+    out.append(f"  if (mode == kSplitFull) {name}_pair<H, V, true, true>(a.ptr, out, off, a.len, mode, act, "
+               "lane, X, bad);")
+    out.append(f"  else if (mode == kSplitMasked) {name}_pair<H, V, true, false>(a.ptr, out, off, a.len, mode, "
+               "act, lane, X, bad);")
+    out.append(f"  else {name}_pair<H, V, false, false>(a.ptr, out, off, a.len, mode, act, lane, X, bad);")
+    out.append("  if constexpr (V)")
+    out.append("    if (bad) *a.flag = 1u;")
+    out.append("}")
+    for suffix, v in (("", "false"), ("_verify", "true")):
+        out.append("__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(2))) "
+                   f"void k_{name}{suffix}(const Gf8Call a) {{")
+        out.append(f"  __shared__ uint4 X[2][{per}][2][64];")
+        out.append("  const uint32_t lane = threadIdx.x & 63;")
+        out.append("  if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0)")
+        out.append(f"    {name}_call<0, {v}>(a, lane, X);")
+        out.append("  else")
+        out.append(f"    {name}_call<1, {v}>(a, lane, X);")
+        out.append("}")
     return "\n".join(out)
 
 
@@ -479,11 +515,10 @@ def generate() -> str:
         "}",
         "",
         "// Threads of a split program's workgroup: one wave per half.",
-        "constexpr int kPairThreads = 128;",
+        "[[maybe_unused]] constexpr int kPairThreads = 128;",
         "",
     ]
     table = []
-    programs = {}
     for (k, m) in SHAPES:
         routes = []
         if (k, m) in MAT_SHAPES:
@@ -494,6 +529,11 @@ def generate() -> str:
             name = f"bs_{route}_{k}_{m}"
             parts.append(f"// ({k},{m}) {route}: {nreg} registers, ~{xor_cost(ops)} plane XORs "
                          "per 32-byte lane chunk")
+            if (route, k, m) in SPLIT:  # per-call, verify and work-list kernels on wave pairs
+                parts.append(emit_split(name, k, ops, nreg, (route, k, m) in SELECT_SWAP))
+                parts.append("")
+                table.append((route, k, m, name))
+                continue
             parts.append(f"template <bool V, bool FAST> __device__ __forceinline__ void {name}("
                          "const uint8_t *const *in, uint8_t *const *out, uint64_t off, "
                          "uint64_t len, int mode, uint32_t &bad) {")
@@ -520,7 +560,6 @@ def generate() -> str:
             parts.append("}")
             parts.append("")
             table.append((route, k, m, name))
-            programs[name] = (ops, nreg)
     # Persistent work-list kernels, one per generated program so that each gets
     # the register budget of its own program: tiles are (job, 8 KiB column
     # chunk), numbered job by job; tile_job[t] names the job of tile t.  The
@@ -528,10 +567,7 @@ def generate() -> str:
     # code whose loads can all be in flight together (a per-load branch would
     # put a wait after every load).
     for pid, (route, k, m, name) in enumerate(table, start=1):
-        if (route, k, m) in SPLIT:
-            ops, nreg = programs[name]
-            parts.append(emit_split(name, k, ops, nreg, (route, k, m) in SELECT_SWAP))
-            parts.append("")
+        if (route, k, m) in SPLIT:  # emitted with its per-call kernels
             continue
         waves = _waves_attr(route, k, m)
         parts.append(f"__global__ __launch_bounds__(kBlockThreads) {waves}void k_batch_{name}("
@@ -587,7 +623,11 @@ def generate() -> str:
     for route, k, m, name in table:
         rid = 1 if route == "mat" else 2
         parts.append(f"  if (route == {rid} && a.k == {k} && a.nout == {m}) {{")
-        parts.append(f"    hipLaunchKernelGGL(k_{name}, grid, dim3(kBlockThreads), 0, stream, a);")
+        if (route, k, m) in SPLIT:
+            parts.append(f"    hipLaunchKernelGGL(k_{name}, dim3(uint32_t((a.len + 2047) / 2048)), "
+                         "dim3(kPairThreads), 0, stream, a);")
+        else:
+            parts.append(f"    hipLaunchKernelGGL(k_{name}, grid, dim3(kBlockThreads), 0, stream, a);")
         parts.append("    return hipGetLastError();")
         parts.append("  }")
     parts.append("  return hipErrorNotSupported;")
@@ -600,7 +640,11 @@ def generate() -> str:
     for route, k, m, name in table:
         rid = 1 if route == "mat" else 2
         parts.append(f"  if (route == {rid} && a.k == {k} && a.nout == {m}) {{")
-        parts.append(f"    hipLaunchKernelGGL(k_{name}_verify, grid, dim3(kBlockThreads), 0, stream, a);")
+        if (route, k, m) in SPLIT:
+            parts.append(f"    hipLaunchKernelGGL(k_{name}_verify, dim3(uint32_t((a.len + 2047) / 2048)), "
+                         "dim3(kPairThreads), 0, stream, a);")
+        else:
+            parts.append(f"    hipLaunchKernelGGL(k_{name}_verify, grid, dim3(kBlockThreads), 0, stream, a);")
         parts.append("    return hipGetLastError();")
         parts.append("  }")
     parts.append("  return hipErrorNotSupported;")
