@@ -481,7 +481,8 @@ def xor_cost(ops) -> int:
 
 # Waves per SIMD the kernels are compiled for (the register budget is
 # 512 / waves VGPRs).  Two for every program but these, which need more than
-# 256 VGPRs without spilling: (32,32) fft keeps ~60 plane registers live, and
+# 256 VGPRs without spilling: (32,32) fft as one program (without its wave
+# pairs, SPLIT) keeps ~60 plane registers live, and
 # the compiler interleaves the output rows of the wide constant-matrix
 # programs.  With the full/tail branch taken once per tile the compiler
 # issues loads early, which raises register pressure: unbounded, (18,6) fft
