@@ -207,6 +207,10 @@ def emit_body(k: int, ops, nreg: int, sel: bool = False) -> str:
 # Each wave then keeps k/2 + 1 registers live instead of k + 1: for (32,32),
 # two waves per SIMD instead of one.
 SPLIT = {("fft", 32, 32)} if int(os.environ.get("TRS_GEN_SPLIT", "1")) else set()
+# Waves per SIMD a split program's kernels are compiled for.  (16,16) splits
+# too (125-127 VGPRs, four waves per SIMD), but its work-list launch measured
+# 709 -> 901 us, so it keeps the single program at two waves per SIMD.
+SPLIT_WAVES = {("fft", 32, 32): 2}
 
 
 def split_program(ops, nreg: int, k: int):
@@ -305,7 +309,7 @@ def _split_op_lines(op, sel: str) -> list:
     return lines
 
 
-def emit_split(name: str, k: int, ops, nreg: int, sel: bool) -> str:
+def emit_split(name: str, k: int, ops, nreg: int, sel: bool, waves: int = 2) -> str:
     """Device functions and the work-list kernel of a split program."""
     pre, consts, post = split_program(ops, nreg, k)
     h = k // 2
@@ -421,7 +425,7 @@ def emit_split(name: str, k: int, ops, nreg: int, sel: bool) -> str:
     out.append(f"    else {name}_pair<H, false, false, false>(in, out, off, len, mode, act, lane, X, bad);")
     out.append("  }")
     out.append("}")
-    out.append("__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(2))) "
+    out.append(f"__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu({waves}))) "
                f"void k_batch_{name}(const BatchJob *jobs,")
     out.append("    const uint32_t *tile_job, uint64_t total) {")
     out.append(f"  __shared__ uint4 X[2][{per}][2][64];")
@@ -454,7 +458,7 @@ def emit_split(name: str, k: int, ops, nreg: int, sel: bool) -> str:
     out.append("    if (bad) *a.flag = 1u;")
     out.append("}")
     for suffix, v in (("", "false"), ("_verify", "true")):
-        out.append("__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(2))) "
+        out.append(f"__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu({waves}))) "
                    f"void k_{name}{suffix}(const Gf8Call a) {{")
         out.append(f"  __shared__ uint4 X[2][{per}][2][64];")
         out.append("  const uint32_t lane = threadIdx.x & 63;")
@@ -531,7 +535,8 @@ def generate() -> str:
             parts.append(f"// ({k},{m}) {route}: {nreg} registers, ~{xor_cost(ops)} plane XORs "
                          "per 32-byte lane chunk")
             if (route, k, m) in SPLIT:  # per-call, verify and work-list kernels on wave pairs
-                parts.append(emit_split(name, k, ops, nreg, (route, k, m) in SELECT_SWAP))
+                parts.append(emit_split(name, k, ops, nreg, (route, k, m) in SELECT_SWAP,
+                                        SPLIT_WAVES[(route, k, m)]))
                 parts.append("")
                 table.append((route, k, m, name))
                 continue

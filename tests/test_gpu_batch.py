@@ -77,16 +77,17 @@ def test_batch_rerun_is_idempotent(trs):
     assert (to_host(shards[k:]) == O.rs8_encode(G, k, data)).all()
 
 
-def test_batch_pair_program_tiles(trs):
+@pytest.mark.parametrize("k,m", [(32, 32), (16, 16)])
+def test_batch_pair_program_tiles(trs, k, m):
     """The (32,32) batched encode runs on wave pairs, one per 2 KiB window of
     each 8 KiB tile, that swap registers through LDS between their halves of
-    the program.  Lengths end in each of the four windows, on full tiles,
-    on aligned tails that are and are not multiples of 16, and shards start
-    one byte past an allocation (byte accesses); all in one launch, against
-    the oracle's generator-matrix encode."""
+    the program ((16,16), one program per wave, on the same cases).  Lengths
+    end in each of the four windows, on full tiles, on aligned tails that are
+    and are not multiples of 16, and shards start one byte past an
+    allocation (byte accesses); all in one launch, against the oracle's
+    generator-matrix encode."""
     import torch
 
-    k, m = 32, 32
     rs = trs.ReedSolomon(k, m)
     G = O.build_matrix(8, k, k + m)
     lengths = [1, 16, 2048, 2064, 4096 + 16, 6144 + 1, 8192, 8192 + 2048 + 16,
