@@ -165,7 +165,12 @@ def emit_body(k: int, ops, nreg: int, sel: bool = False) -> str:
             out.append(f"  load_split<true>(ld_const_ptr(in + {s}), off, len, mode, R[{s}]);")
     for i, op in enumerate(ops):
         if SCHED_GROUP and i % SCHED_GROUP == 0:
-            out.append("  if constexpr (FAST) __builtin_amdgcn_sched_barrier(0);")
+            # verify kernels keep the order on the byte paths too: their
+            # spills drop from 647 / 1733 to 4 / 23 VGPRs for (14,14) /
+            # (16,16) (a verify call's time, bound by waiting for its flag,
+            # did not change).  Encode kernels keep the plain byte paths,
+            # which spill less there ((18,6) 8 against 35 VGPRs).
+            out.append("  if constexpr (FAST || V) __builtin_amdgcn_sched_barrier(0);")
         kind = op[0]
         if kind == "st":
             _, j, r = op
@@ -191,7 +196,7 @@ def emit_body(k: int, ops, nreg: int, sel: bool = False) -> str:
                 else:
                     out.append(f"  R[{d}][{r}] = {terms if terms else '0u'};")
         if OPAQUE and kind in ("xo", "mx", "mc"):
-            out.append(f"  if constexpr (FAST) opaque(R[{op[1]}]);")
+            out.append(f"  if constexpr (FAST || V) opaque(R[{op[1]}]);")
     return "\n".join(out)
 
 
