@@ -1086,10 +1086,14 @@ int run_gf16(const std::vector<const uint8_t *> &ptrs, size_t k, size_t nout,
 }
 
 int read_flag(uint32_t *flag, hipStream_t stream, int *ok) {
-  uint32_t h = 0;
-  TRS_TRY(hipMemcpyAsync(&h, flag, sizeof h, hipMemcpyDeviceToHost, stream));
+  // The flag lands in a pinned word per host thread (a copy into pageable
+  // memory goes through the runtime's staging); kept for the life of the
+  // process, like the device flag.
+  thread_local void *h = nullptr;
+  if (!h) TRS_TRY(hipHostMalloc(&h, sizeof(uint32_t), hipHostMallocDefault));
+  TRS_TRY(hipMemcpyAsync(h, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
   TRS_TRY(hipStreamSynchronize(stream));
-  *ok = h ? 0 : 1;
+  *ok = *static_cast<volatile uint32_t *>(h) ? 0 : 1;
   return TRS_OK;
 }
 
