@@ -314,6 +314,30 @@ def test_every_encode_route_matches_oracle(trs, k, m):
     assert rs.encode_route(65536).startswith("gpu-")
 
 
+def test_pair_program_per_call_unaligned(trs):
+    """The per-call (32,32) encode and verify kernels (one wave pair per
+    2 KiB of shard) on shards that start one byte past an allocation (byte
+    accesses), with lengths that end inside different 2 KiB windows of a
+    tile; verify catches a flipped last byte."""
+    import torch
+
+    k, m = 32, 32
+    G = O.build_matrix(8, k, k + m)
+    rs = trs.ReedSolomon(k, m)
+    for length in (2048 + 16, 4096 + 5, 6144 + 1, 8192 + 2048 + 3):
+        data = seeded(k, length, 77 + length)
+        want = O.rs8_encode(G, k, data)
+        bufs = [torch.zeros(length + 1, dtype=torch.uint8, device="cuda") for _ in range(k + m)]
+        shards = [t[1:] for t in bufs]
+        for j in range(k):
+            shards[j].copy_(torch.from_numpy(data[j]))
+        rs.encode_sep_route("bitsliced-fft", shards[:k], shards[k:])
+        assert (to_host(shards[k:]) == want).all(), length
+        assert rs.verify(shards), length
+        shards[k + 7][length - 1] ^= 1
+        assert not rs.verify(shards), length
+
+
 @pytest.mark.parametrize("route", ["table", "mfma"])
 @pytest.mark.parametrize("k,m", [(10, 4), (7, 13), (10, 10), (32, 32), (17, 5), (3, 2), (29, 9)])
 def test_reconstruct_apply_routes(trs, k, m, route):
